@@ -1,0 +1,140 @@
+/*
+ * egt.h -- C ABI of the MI355X (gfx950) grid-kNN -> elliptic Gabriel graph -> Taubin smoothing library.
+ *
+ * Drop-in boundary for the reference's KNNInterface / GridStructure / GridMeshCuda / EGT_CUDA /
+ * Normal_CUDA / getOutputPoints path (agalex1974/MultiGPUEGTSmoothing). Each entry point below names the
+ * reference interface it replaces (file:line in the reference repository). Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - One handle = one point set (cloud or mesh) resident on the current HIP device, with its own stream.
+ *    Calls on one handle must be serialized by the caller; different handles are independent.
+ *  - "Sorted space": points are normalized into the reference's cube and permuted by a stable sort on the
+ *    row-major cell key x + y*B + z*B*B at bucket count B (GridKNNCuda.cu:74-88, 898-936, 1107-1128).
+ *    Device buffers handed out by the accessors are in that space; host outputs are in original order.
+ *  - Every call returns an egt_status; egt_last_error_string() describes the last failure on this thread.
+ *  - num_cards must be 1: the whole point set lives on one GPU.
+ */
+#ifndef EGT_H
+#define EGT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    EGT_OK = 0,
+    EGT_ERR_INVALID_ARG = 1,
+    EGT_ERR_OOM = 2,
+    EGT_ERR_HIP = 3,
+    EGT_ERR_UNSUPPORTED = 4,
+    EGT_ERR_IO = 5,
+    EGT_ERR_STATE = 6
+} egt_status;
+
+typedef struct egt_ctx* egt_handle;
+
+/* Per-handle counters (no reference counterpart: the reference prints them to stdout,
+ * GridKNNCuda.cu:2422-2425, 2558-2561; Normal_CUDA.cu:339-436). */
+typedef struct {
+    int64_t n_points;
+    int32_t k;
+    int32_t bucket_count;
+    int64_t invalid_rows;      /* kNN rows blanked by the reference's validity rule */
+    int64_t knn_candidates;    /* candidate distances evaluated by the kNN search */
+    int64_t egg_kept;          /* neighbour slots kept by the EGG filter (CSR nnz) */
+    int64_t learned_queries;   /* queries that started from the learned radius proposal */
+    int64_t learned_fallbacks; /* of those, queries whose proposal failed the certificate */
+    int64_t msf_edges;         /* spanning-forest edges of the orientation step */
+    int64_t msf_trees;         /* trees (components with at least one edge) */
+    double msf_weight;         /* total forest weight */
+    float knn_ms, egg_ms, taubin_ms, normals_ms, grid_ms; /* device time of the last call of each stage */
+} egt_stats;
+
+/* Options set through egt_set_option (no reference counterpart). */
+typedef enum {
+    EGT_OPT_START_LEVEL = 0,     /* kNN: pyramid level of the first search box; -1 = automatic (default) */
+    EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
+    EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
+    EGT_OPT_MAX_PTS_IN_CUBE = 3  /* kNN validity rule: cell population that invalidates a query (1e5) */
+} egt_option;
+
+const char* egt_last_error_string(void);
+
+/* GridStructure(const Mesh&, int num_cards, float eps, int bucketCount, int maxDivision)
+ * GridKNNCuda.cuh:15-16, GridKNNCuda.cu:1165-1205 (+ InitStructure :1023-1133).
+ * Host SoA input in original order. bucketCount must be a power of two in [1, 1024]. */
+egt_status egt_grid_create(const float* x, const float* y, const float* z, int64_t n, int num_cards, float eps,
+                           int bucket_count, int max_division, egt_handle* out);
+
+/* Same, with x, y, z already in device memory on the current device (no reference counterpart: the
+ * reference always copies from the host, GridKNNCuda.cu:1030-1040). The inputs are not modified. */
+egt_status egt_grid_create_device(const float* dx, const float* dy, const float* dz, int64_t n, int num_cards,
+                                  float eps, int bucket_count, int max_division, egt_handle* out);
+
+/* GridStructure::GRIDCUDAKNNSELF_COMPACT(int k, float thres), GridKNNCuda.cuh:22, GridKNNCuda.cu:2322-2634.
+ * thres in input units (main.cpp:109 passes 1e8). Rows are the exact k nearest neighbours ordered by
+ * (distance, sorted index); rows the reference would blank are all -1. */
+egt_status egt_grid_knn_self(egt_handle h, int k, float thres);
+
+/* GridMeshCuda(const Mesh&, int num_cards, float eps, int bucketCount), GridMeshCuda1.cuh:16,
+ * GridMeshCuda1.cu:247-414. tri: ntri x 3 uint32 original vertex indices (Mesh.h:15-19). */
+egt_status egt_mesh_create(const float* x, const float* y, const float* z, int64_t n, const uint32_t* tri,
+                           int64_t ntri, int num_cards, float eps, int bucket_count, egt_handle* out);
+
+/* GridMeshCuda::GetNeighborsFromTriangles(int k), GridMeshCuda1.cuh:17, GridMeshCuda1.cu:416-511. */
+egt_status egt_mesh_neighbors(egt_handle h, int k);
+
+/* EGT_CUDA(KNNInterface&).PerformSmoothing(int iterationCount, float lambda, float mu, float alpha),
+ * EGT_CUDA.cuh:13-14, EGT_CUDA.cu:476-648. Side effects as in the reference: the handle's neighbour rows
+ * become the EGG-filtered rows and its points become the smoothed points (still sorted, normalized). */
+egt_status egt_smooth(egt_handle h, int iterations, float lambda, float mu, float alpha);
+
+/* Normal_CUDA(KNNInterface&).GetNormals(float* nx, float* ny, float* nz), Normal_CUDA.cuh:12-13,
+ * Normal_CUDA.cu:304-775. Host arrays of n floats, original order. */
+egt_status egt_normals(egt_handle h, float* nx, float* ny, float* nz);
+
+/* getOutputPoints(float* x, float* y, float* z, KNNInterface*), getOutputPoints.cuh:10,
+ * getOutputPoints.cu:34-63. Host arrays, original order, de-normalized. Unlike the reference this does not
+ * overwrite the handle's sorted points (:52-54), so the handle stays usable. */
+egt_status egt_get_output_points(egt_handle h, float* x, float* y, float* z);
+
+/* KNNInterface::GetKNNIndexes(int* knnIndexes), KNNInterface.cuh:23, GridKNNCuda.cu:1941-1968,
+ * GridMeshCuda1.cu:513-540. Host n*k int32, original index space, -1 holes. */
+egt_status egt_get_knn_original(egt_handle h, int32_t* out);
+
+/* KNNInterface::GetRefPointsX/Y/Z() -> float**, KNNInterface.cuh:10-12 (card 0). Borrowed device pointers. */
+egt_status egt_get_device_points(egt_handle h, float** x, float** y, float** z);
+
+/* KNNInterface::GetKNNIndexesInPartitions() -> int**, KNNInterface.cuh:18 (card 0). Borrowed, n*k, sorted. */
+egt_status egt_get_device_knn(egt_handle h, int32_t** knn);
+
+/* KNNInterface::GetOriginalRefIndexes() -> int**, KNNInterface.cuh:21 (card 0): sorted -> original. */
+egt_status egt_get_device_original_indexes(egt_handle h, int32_t** idx);
+
+/* KNNInterface::GetMinExtent/GetMaxExtent, KNNInterface.cuh:24-25. */
+egt_status egt_get_extent(egt_handle h, float* min_extent, float* max_extent);
+
+/* KNNInterface::pointsRefCount/NeighborCount/GetNumberOfCards/GetPartitionSize, KNNInterface.cuh:9,19,26,28. */
+egt_status egt_get_counts(egt_handle h, int64_t* n_points, int32_t* k, int32_t* num_cards,
+                          int64_t* partition_size);
+
+/* Counters and stage timings (no reference counterpart). */
+egt_status egt_get_stats(egt_handle h, egt_stats* st);
+
+/* Tuning switches (no reference counterpart). */
+egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value);
+
+/* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA
+ * leaked (EGT_CUDA.cu:490-492). */
+egt_status egt_destroy(egt_handle h);
+
+/* HIP stream of the handle (hipStream_t), for callers that order their own work after the library's. */
+void* egt_get_stream(egt_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EGT_H */

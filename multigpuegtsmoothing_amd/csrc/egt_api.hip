@@ -1,0 +1,260 @@
+// extern "C" entry points of include/egt.h: argument checks, handle lifecycle, status codes.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include "egt_internal.h"
+
+namespace {
+thread_local std::string g_err;
+
+egt_status fail(egt_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+template <class F>
+egt_status guarded(F&& f) {
+    try {
+        f();
+        return EGT_OK;
+    } catch (const egt::ArgError& e) {
+        return fail(e.st, e.what);
+    } catch (const egt::HipError& e) {
+        return fail(e.err == hipErrorOutOfMemory ? EGT_ERR_OOM : EGT_ERR_HIP, e.what);
+    } catch (const std::bad_alloc&) {
+        return fail(EGT_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(EGT_ERR_STATE, e.what());
+    }
+}
+
+bool pow2_in_range(int b) { return b >= 1 && b <= 1024 && (b & (b - 1)) == 0; }
+
+egt_status check_common(int64_t n, int num_cards, int bucket_count, egt_handle* out) {
+    if (!out) return fail(EGT_ERR_INVALID_ARG, "null handle pointer");
+    if (n < 1) return fail(EGT_ERR_INVALID_ARG, "point count must be >= 1");
+    if (n >= (int64_t)1 << 31) return fail(EGT_ERR_UNSUPPORTED, "point count must fit in int32");
+    if (num_cards != 1) return fail(EGT_ERR_UNSUPPORTED, "num_cards must be 1: one GPU holds the whole cloud");
+    if (!pow2_in_range(bucket_count)) return fail(EGT_ERR_INVALID_ARG, "bucketCount must be a power of two in [1, 1024]");
+    return EGT_OK;
+}
+
+egt_status make_handle(const float* x, const float* y, const float* z, int64_t n, float eps, int bucket_count,
+                       bool device_input, int kind, egt_handle* out) {
+    if (!x || !y || !z) return fail(EGT_ERR_INVALID_ARG, "null coordinate array");
+    if (!(eps >= 0.0f) || !std::isfinite(eps)) return fail(EGT_ERR_INVALID_ARG, "eps must be finite and >= 0");
+    egt_ctx* h = new (std::nothrow) egt_ctx();
+    if (!h) return fail(EGT_ERR_OOM, "host allocation failed");
+    egt_status st = guarded([&] {
+        HIP_CHECK(hipGetDevice(&h->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->kind = kind;
+        h->n = n;
+        h->B = bucket_count;
+        h->eps = eps;
+        h->stats.n_points = n;
+        h->stats.bucket_count = bucket_count;
+        if (device_input) {
+            egt::build_grid(h, x, y, z);
+        } else {
+            float* tmp = h->scratch.get<float>(16, 3 * (size_t)n);
+            HIP_CHECK(hipMemcpyAsync(tmp, x, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + n, y, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            egt::build_grid(h, tmp, tmp + n, tmp + 2 * n);
+        }
+    });
+    if (st != EGT_OK) {
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return EGT_OK;
+}
+
+egt_status need(egt_handle h) {
+    if (!h) return fail(EGT_ERR_INVALID_ARG, "null handle");
+    return EGT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* egt_last_error_string(void) { return g_err.c_str(); }
+
+egt_status egt_grid_create(const float* x, const float* y, const float* z, int64_t n, int num_cards, float eps,
+                           int bucket_count, int max_division, egt_handle* out) {
+    egt_status st = check_common(n, num_cards, bucket_count, out);
+    if (st != EGT_OK) return st;
+    if (max_division < 1) return fail(EGT_ERR_INVALID_ARG, "maxDivision must be >= 1");
+    return make_handle(x, y, z, n, eps, bucket_count, false, 0, out);
+}
+
+egt_status egt_grid_create_device(const float* dx, const float* dy, const float* dz, int64_t n, int num_cards,
+                                  float eps, int bucket_count, int max_division, egt_handle* out) {
+    egt_status st = check_common(n, num_cards, bucket_count, out);
+    if (st != EGT_OK) return st;
+    if (max_division < 1) return fail(EGT_ERR_INVALID_ARG, "maxDivision must be >= 1");
+    return make_handle(dx, dy, dz, n, eps, bucket_count, true, 0, out);
+}
+
+egt_status egt_grid_knn_self(egt_handle h, int k, float thres) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "kNN search needs a point-cloud handle");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    return guarded([&] {
+        // GridKNNCuda.cu:2324-2325
+        float t = thres / (h->max_extent - h->min_extent);
+        t *= t;
+        egt::run_knn(h, k, t);
+        h->smoothed = false;
+    });
+}
+
+egt_status egt_mesh_create(const float* x, const float* y, const float* z, int64_t n, const uint32_t* tri,
+                           int64_t ntri, int num_cards, float eps, int bucket_count, egt_handle* out) {
+    egt_status st = check_common(n, num_cards, bucket_count, out);
+    if (st != EGT_OK) return st;
+    if (ntri < 0 || (ntri > 0 && !tri)) return fail(EGT_ERR_INVALID_ARG, "bad triangle array");
+    st = make_handle(x, y, z, n, eps, bucket_count, false, 1, out);
+    if (st != EGT_OK) return st;
+    egt_handle h = *out;
+    st = guarded([&] {
+        h->ntri = ntri;
+        h->tri.alloc(3 * (size_t)std::max<int64_t>(ntri, 1));
+        if (ntri > 0)
+            HIP_CHECK(hipMemcpyAsync(h->tri.p, tri, 3 * ntri * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(hipStreamSynchronize(h->stream));
+    });
+    if (st != EGT_OK) {
+        egt_destroy(h);
+        *out = nullptr;
+    }
+    return st;
+}
+
+egt_status egt_mesh_neighbors(egt_handle h, int k) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 1) return fail(EGT_ERR_STATE, "mesh neighbours need a mesh handle");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    return guarded([&] {
+        egt::run_mesh_adjacency(h, k);
+        h->smoothed = false;
+    });
+}
+
+egt_status egt_smooth(egt_handle h, int iterations, float lambda, float mu, float alpha) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!h->have_knn) return fail(EGT_ERR_STATE, "run the kNN / mesh neighbour step first");
+    if (iterations < 0) return fail(EGT_ERR_INVALID_ARG, "iterations must be >= 0");
+    if (!std::isfinite(lambda) || !std::isfinite(mu) || !std::isfinite(alpha))
+        return fail(EGT_ERR_INVALID_ARG, "lambda, mu and alpha must be finite");
+    return guarded([&] {
+        egt::run_egg(h, alpha);
+        egt::run_taubin(h, iterations, lambda, mu);
+        h->smoothed = true;
+    });
+}
+
+egt_status egt_normals(egt_handle h, float* nx, float* ny, float* nz) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!h->have_knn) return fail(EGT_ERR_STATE, "run the kNN / mesh neighbour step first");
+    if (!nx || !ny || !nz) return fail(EGT_ERR_INVALID_ARG, "null normal array");
+    return guarded([&] { egt::run_normals(h, nx, ny, nz); });
+}
+
+egt_status egt_get_output_points(egt_handle h, float* x, float* y, float* z) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!x || !y || !z) return fail(EGT_ERR_INVALID_ARG, "null output array");
+    return guarded([&] { egt::run_output(h, x, y, z); });
+}
+
+egt_status egt_get_knn_original(egt_handle h, int32_t* out) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!h->have_knn) return fail(EGT_ERR_STATE, "no neighbour rows yet");
+    if (!out) return fail(EGT_ERR_INVALID_ARG, "null output array");
+    return guarded([&] { egt::run_knn_original(h, out); });
+}
+
+egt_status egt_get_device_points(egt_handle h, float** x, float** y, float** z) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (x) *x = h->x.p;
+    if (y) *y = h->y.p;
+    if (z) *z = h->z.p;
+    return EGT_OK;
+}
+
+egt_status egt_get_device_knn(egt_handle h, int32_t** knn) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (knn) *knn = h->have_knn ? h->knn.p : nullptr;
+    return EGT_OK;
+}
+
+egt_status egt_get_device_original_indexes(egt_handle h, int32_t** idx) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (idx) *idx = h->perm.p;
+    return EGT_OK;
+}
+
+egt_status egt_get_extent(egt_handle h, float* mn, float* mx) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (mn) *mn = h->min_extent;
+    if (mx) *mx = h->max_extent;
+    return EGT_OK;
+}
+
+egt_status egt_get_counts(egt_handle h, int64_t* n, int32_t* k, int32_t* cards, int64_t* part) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (n) *n = h->n;
+    if (k) *k = h->k;
+    if (cards) *cards = 1;
+    if (part) *part = h->n;
+    return EGT_OK;
+}
+
+egt_status egt_get_stats(egt_handle h, egt_stats* st) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!st) return fail(EGT_ERR_INVALID_ARG, "null stats pointer");
+    *st = h->stats;
+    st->k = h->k;
+    return EGT_OK;
+}
+
+egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    switch (opt) {
+        case EGT_OPT_START_LEVEL:
+            if (value < -1 || value >= h->nlev) return fail(EGT_ERR_INVALID_ARG, "start level out of range");
+            h->start_level = (int)value;
+            return EGT_OK;
+        case EGT_OPT_LEARNED:
+            h->learned = value ? 1 : 0;
+            return EGT_OK;
+        case EGT_OPT_TAUBIN_GRAPH:
+            h->taubin_graph = value ? 1 : 0;
+            return EGT_OK;
+        case EGT_OPT_MAX_PTS_IN_CUBE:
+            if (value < 1 || value > INT32_MAX) return fail(EGT_ERR_INVALID_ARG, "max points in cube must be >= 1");
+            h->maxc = (int)value;
+            return EGT_OK;
+    }
+    return fail(EGT_ERR_INVALID_ARG, "unknown option");
+}
+
+egt_status egt_destroy(egt_handle h) {
+    if (!h) return EGT_OK;
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return EGT_OK;
+}
+
+void* egt_get_stream(egt_handle h) { return h ? (void*)h->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,345 @@
+// Grid construction (bounding cube, normalization, cell key, stable sort into sorted space), the
+// Morton-ordered search structure with per-level cell hash tables, and the output gathers.
+//
+// Reference: GridKNNCuda.cu:1023-1133 (InitStructure), :74-88 (cell key), :898-936 (getBuckets),
+// getOutputPoints.cu:9-63, GridKNNCuda.cu:1941-1968 (GetKNNIndexes).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "egt_device.h"
+#include "egt_internal.h"
+
+namespace egt {
+namespace {
+
+constexpr int BLK = 256;
+
+// Per-block min/max of x, y, z (thrust reduce with minimum/maximum, GridKNNCuda.cu:1042-1059) plus a
+// count of non-finite inputs.
+__global__ void __launch_bounds__(BLK) k_minmax_partial(const float* __restrict__ x, const float* __restrict__ y,
+                                                         const float* __restrict__ z, int64_t n,
+                                                         float* __restrict__ part, uint32_t* __restrict__ bad) {
+    float v[6] = {1e8f, -1e8f, 1e8f, -1e8f, 1e8f, -1e8f};
+    uint32_t nb = 0;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+        const float a = x[i], b = y[i], c = z[i];
+        nb += !(__builtin_isfinite(a) && __builtin_isfinite(b) && __builtin_isfinite(c));
+        v[0] = fminf(v[0], a); v[1] = fmaxf(v[1], a);
+        v[2] = fminf(v[2], b); v[3] = fmaxf(v[3], b);
+        v[4] = fminf(v[4], c); v[5] = fmaxf(v[5], c);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 6; c += 2) {
+            v[c] = fminf(v[c], __shfl_xor(v[c], d, 64));
+            v[c + 1] = fmaxf(v[c + 1], __shfl_xor(v[c + 1], d, 64));
+        }
+        nb += __shfl_xor(nb, d, 64);
+    }
+    __shared__ float s[BLK / 64][6];
+    __shared__ uint32_t sb[BLK / 64];
+    const int wid = threadIdx.x / 64;
+    if (__lane_id() == 0) {
+        for (int c = 0; c < 6; ++c) s[wid][c] = v[c];
+        sb[wid] = nb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLK / 64; ++w)
+            for (int c = 0; c < 6; c += 2) {
+                s[0][c] = fminf(s[0][c], s[w][c]);
+                s[0][c + 1] = fmaxf(s[0][c + 1], s[w][c + 1]);
+            }
+        uint32_t t = 0;
+        for (int w = 0; w < BLK / 64; ++w) t += sb[w];
+        for (int c = 0; c < 6; ++c) part[blockIdx.x * 6 + c] = s[0][c];
+        if (t) atomicAdd(bad, t);
+    }
+}
+
+// normalize_points GridKNNCuda.cu:840-849 fused with get_buckets_no_binary :74-88.
+__global__ void __launch_bounds__(BLK) k_normalize_key(const float* __restrict__ x, const float* __restrict__ y,
+                                                        const float* __restrict__ z, int64_t n, float mn, float d,
+                                                        float a, float ba, int B, int log2B, float* __restrict__ nx,
+                                                        float* __restrict__ ny, float* __restrict__ nz,
+                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const float xn = (x[i] - mn) / d, yn = (y[i] - mn) / d, zn = (z[i] - mn) / d;
+    nx[i] = xn;
+    ny[i] = yn;
+    nz[i] = zn;
+    const uint32_t cx = (uint32_t)cell_of(xn, a, ba, B), cy = (uint32_t)cell_of(yn, a, ba, B),
+                   cz = (uint32_t)cell_of(zn, a, ba, B);
+    key[i] = cx | (cy << log2B) | (cz << (2 * log2B));
+    idx[i] = (uint32_t)i;
+}
+
+// thrust::gather into sorted order, GridKNNCuda.cu:1114-1128.
+__global__ void __launch_bounds__(BLK) k_gather_sorted(const float* __restrict__ nx, const float* __restrict__ ny,
+                                                        const float* __restrict__ nz, const uint32_t* __restrict__ skey,
+                                                        const uint32_t* __restrict__ sidx, int64_t n,
+                                                        float* __restrict__ x, float* __restrict__ y,
+                                                        float* __restrict__ z, int32_t* __restrict__ perm,
+                                                        int32_t* __restrict__ key) {
+    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = sidx[s];
+    perm[s] = (int32_t)o;
+    key[s] = (int32_t)skey[s];
+    x[s] = nx[o];
+    y[s] = ny[o];
+    z[s] = nz[o];
+}
+
+__global__ void __launch_bounds__(BLK) k_morton_keys(const int32_t* __restrict__ key, int64_t n, int log2B,
+                                                      uint32_t* __restrict__ code, uint32_t* __restrict__ idx) {
+    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t k = (uint32_t)key[s], m = (1u << log2B) - 1u;
+    code[s] = morton_code(k & m, (k >> log2B) & m, k >> (2 * log2B));
+    idx[s] = (uint32_t)s;
+}
+
+__global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__ x, const float* __restrict__ y,
+                                                        const float* __restrict__ z, const int32_t* __restrict__ key,
+                                                        const uint32_t* __restrict__ order, int64_t n, int log2B,
+                                                        float4* __restrict__ mpts, uint32_t* __restrict__ mcell) {
+    const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (m >= n) return;
+    const uint32_t s = order[m];
+    mpts[m] = make_float4(x[s], y[s], z[s], __int_as_float((int)s));
+    const uint32_t k = (uint32_t)key[s], msk = (1u << log2B) - 1u;
+    mcell[m] = (k & msk) | (((k >> log2B) & msk) << 10) | ((k >> (2 * log2B)) << 20);
+}
+
+__global__ void __launch_bounds__(BLK) k_level_flags(const uint32_t* __restrict__ code, int64_t n, int shift,
+                                                      uint32_t* __restrict__ flag) {
+    const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (m >= n) return;
+    flag[m] = (m == 0 || (code[m] >> shift) != (code[m - 1] >> shift)) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(BLK) k_level_starts(const uint32_t* __restrict__ flag,
+                                                       const uint32_t* __restrict__ seg, int64_t n,
+                                                       uint32_t* __restrict__ starts) {
+    const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (m >= n) return;
+    if (flag[m]) starts[seg[m]] = (uint32_t)m;
+}
+
+__global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict__ starts, int64_t cells,
+                                                       int64_t n, const uint32_t* __restrict__ code, int shift,
+                                                       uint32_t* __restrict__ hkeys, uint2* __restrict__ hvals,
+                                                       uint64_t offset, uint32_t log2cap, int* __restrict__ maxc) {
+    const int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (c >= cells) return;
+    const uint32_t st = starts[c];
+    const uint32_t en = c + 1 < cells ? starts[c + 1] : (uint32_t)n;
+    const uint32_t cc = code[st] >> shift;
+    const uint32_t mask = (1u << log2cap) - 1u;
+    uint32_t s = hash_slot(cc, log2cap);
+    for (;;) {
+        const uint32_t prev = atomicCAS(&hkeys[offset + s], HASH_EMPTY, cc);
+        if (prev == HASH_EMPTY) {
+            hvals[offset + s] = make_uint2(st, en - st);
+            break;
+        }
+        s = (s + 1u) & mask;
+    }
+    atomicMax(maxc, (int)(en - st));
+}
+
+__global__ void __launch_bounds__(BLK) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLK) p[i] = v;
+}
+
+// produce_output_GKNN + rearrange_output, getOutputPoints.cu:9-30, into separate output buffers.
+__global__ void __launch_bounds__(BLK) k_output(const float* __restrict__ x, const float* __restrict__ y,
+                                                 const float* __restrict__ z, const int32_t* __restrict__ perm,
+                                                 int64_t n, float mn, float d, float* __restrict__ ox,
+                                                 float* __restrict__ oy, float* __restrict__ oz) {
+    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    const int32_t o = perm[s];
+    ox[o] = x[s] * d + mn;
+    oy[o] = y[s] * d + mn;
+    oz[o] = z[s] * d + mn;
+}
+
+// rearrangeReferenceNeighbors GridKNNCuda.cu:862-872 plus the host scatter :1959-1966, on the device.
+__global__ void __launch_bounds__(BLK) k_knn_original(const int32_t* __restrict__ knn,
+                                                       const int32_t* __restrict__ perm, int64_t n, int k,
+                                                       int32_t* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= n * k) return;
+    const int64_t s = t / k;
+    const int j = (int)(t - s * k);
+    const int32_t v = knn[t];
+    out[(int64_t)perm[s] * k + j] = v >= 0 ? perm[v] : -1;
+}
+
+int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+}  // namespace
+
+void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    Scratch& sc = h->scratch;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+
+    // bounding cube
+    const int nbp = std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(n, BLK)));
+    float* dpart = sc.get<float>(8, (size_t)nbp * 6 + 1);
+    uint32_t* dbad = reinterpret_cast<uint32_t*>(dpart + (size_t)nbp * 6);
+    HIP_CHECK(hipMemsetAsync(dbad, 0, sizeof(uint32_t), s));
+    k_minmax_partial<<<nbp, BLK, 0, s>>>(dx, dy, dz, n, dpart, dbad);
+    LAUNCH_CHECK();
+    std::vector<float> hp((size_t)nbp * 6 + 1);
+    HIP_CHECK(hipMemcpyAsync(hp.data(), dpart, hp.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    uint32_t bad;
+    std::memcpy(&bad, &hp[(size_t)nbp * 6], 4);
+    if (bad) throw ArgError{EGT_ERR_INVALID_ARG, "input coordinates contain NaN or Inf"};
+    float v[6] = {1e8f, -1e8f, 1e8f, -1e8f, 1e8f, -1e8f};
+    for (int b = 0; b < nbp; ++b)
+        for (int c = 0; c < 6; c += 2) {
+            v[c] = std::min(v[c], hp[(size_t)b * 6 + c]);
+            v[c + 1] = std::max(v[c + 1], hp[(size_t)b * 6 + c + 1]);
+        }
+    const float minx = v[0] - 1e-6f, maxx = v[1] + 1e-6f, miny = v[2] - 1e-6f, maxy = v[3] + 1e-6f;
+    const float minz = v[4] - 1e-6f, maxz = v[5] + 1e-6f;
+    const float mn = std::min(std::min(minx, miny), minz);
+    const float mx = std::max(std::max(maxx, maxy), maxz);
+    h->min_extent = mn;
+    h->max_extent = mx;
+    const float d = mx - mn;
+
+    // normalize + key + stable sort
+    const int log2B = ilog2(h->B);
+    const float a = -h->eps, b = 1.0f + h->eps, ba = b - a;
+    float* tx = sc.get<float>(9, n);
+    float* ty = sc.get<float>(10, n);
+    float* tz = sc.get<float>(11, n);
+    uint32_t* skey = sc.get<uint32_t>(12, n);
+    uint32_t* sidx = sc.get<uint32_t>(13, n);
+    k_normalize_key<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, n, mn, d, a, ba, h->B, log2B, tx, ty, tz, skey,
+                                                      sidx);
+    LAUNCH_CHECK();
+    radix_sort_pairs(skey, sidx, n, 3 * log2B, s, sc);
+    h->x.alloc(n);
+    h->y.alloc(n);
+    h->z.alloc(n);
+    h->perm.alloc(n);
+    h->key.alloc(n);
+    k_gather_sorted<<<ceil_div(n, BLK), BLK, 0, s>>>(tx, ty, tz, skey, sidx, n, h->x.p, h->y.p, h->z.p, h->perm.p,
+                                                      h->key.p);
+    LAUNCH_CHECK();
+    build_search_structure(h);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&h->stats.grid_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void build_search_structure(egt_ctx* h) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    Scratch& sc = h->scratch;
+    const int log2B = ilog2(h->B);
+    DBuf<uint32_t>& code = sc.bufs[12];
+    DBuf<uint32_t>& order = sc.bufs[13];
+    code.alloc(n);
+    order.alloc(n);
+    k_morton_keys<<<ceil_div(n, BLK), BLK, 0, s>>>(h->key.p, n, log2B, code.p, order.p);
+    LAUNCH_CHECK();
+    radix_sort_pairs(code.p, order.p, n, 3 * log2B, s, sc);
+    h->mpts.alloc(n);
+    h->mcell.alloc(n);
+    h->mcode.alloc(n);
+    k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, order.p, n, log2B, h->mpts.p,
+                                                      h->mcell.p);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(h->mcode.p, code.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+
+    // level tables: level l groups 2^l fine cells per axis (Morton code >> 3l)
+    h->nlev = log2B + 1;
+    DBuf<uint32_t>& flag = sc.bufs[9];
+    DBuf<uint32_t>& seg = sc.bufs[10];
+    DBuf<uint32_t>& starts = sc.bufs[11];
+    flag.alloc(n);
+    seg.alloc(n);
+    starts.alloc(n);
+    std::vector<uint32_t> cells(h->nlev);
+    uint64_t total = 0;
+    for (int l = 0; l < h->nlev; ++l) {
+        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, n, 3 * l, flag.p);
+        LAUNCH_CHECK();
+        exclusive_scan_u32(flag.p, seg.p, n, s, sc, &cells[l]);
+        uint32_t log2cap = 6;
+        while ((1ull << log2cap) < 2ull * cells[l]) ++log2cap;
+        h->lev[l].log2cap = log2cap;
+        h->lev[l].offset = total;
+        h->lev[l].cells = cells[l];
+        total += 1ull << log2cap;
+    }
+    h->hkeys.alloc(total);
+    h->hvals.alloc(total);
+    k_fill_u32<<<1024, BLK, 0, s>>>(h->hkeys.p, total, HASH_EMPTY);
+    LAUNCH_CHECK();
+    DBuf<uint32_t>& dmax = sc.bufs[14];
+    dmax.alloc(h->nlev);
+    HIP_CHECK(hipMemsetAsync(dmax.p, 0, h->nlev * sizeof(uint32_t), s));
+    for (int l = 0; l < h->nlev; ++l) {
+        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, n, 3 * l, flag.p);
+        LAUNCH_CHECK();
+        exclusive_scan_u32(flag.p, seg.p, n, s, sc, nullptr);
+        k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts.p);
+        LAUNCH_CHECK();
+        k_level_insert<<<std::max(1, ceil_div(cells[l], BLK)), BLK, 0, s>>>(
+            starts.p, cells[l], n, h->mcode.p, 3 * l, h->hkeys.p, h->hvals.p, h->lev[l].offset, h->lev[l].log2cap,
+            reinterpret_cast<int*>(dmax.p) + l);
+        LAUNCH_CHECK();
+    }
+    std::vector<int32_t> mx(h->nlev);
+    HIP_CHECK(hipMemcpyAsync(mx.data(), dmax.p, h->nlev * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int l = 0; l < h->nlev; ++l) h->lev[l].max_count = mx[l];
+}
+
+void run_output(egt_ctx* h, float* x, float* y, float* z) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    float* ox = h->scratch.get<float>(9, 3 * n);
+    k_output<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->perm.p, n, h->min_extent,
+                                              h->max_extent - h->min_extent, ox, ox + n, ox + 2 * n);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(x, ox, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(y, ox + n, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(z, ox + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void run_knn_original(egt_ctx* h, int32_t* out) {
+    const int64_t n = h->n, k = h->k;
+    hipStream_t s = h->stream;
+    int32_t* tmp = h->scratch.get<int32_t>(9, n * k);
+    k_knn_original<<<ceil_div(n * k, BLK), BLK, 0, s>>>(h->knn.p, h->perm.p, n, (int)k, tmp);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(out, tmp, n * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) { return morton_code(x, y, z); }
+
+}  // namespace egt

@@ -1,0 +1,159 @@
+// Internal declarations of the gfx950 library behind include/egt.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/egt.h"
+
+#define EGT_MAX_LEVELS 11
+
+namespace egt {
+
+struct HipError {
+    hipError_t err;
+    std::string what;
+};
+struct ArgError {
+    egt_status st;
+    std::string what;
+};
+
+#define HIP_CHECK(expr)                                                                         \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw ::egt::HipError{e_, std::string(#expr) + ": " + hipGetErrorString(e_)};       \
+    } while (0)
+
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+// Device allocation owned by a handle.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        free();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw HipError{e, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes"};
+        }
+        n = count;
+    }
+    void free() {
+        if (p) {
+            (void)hipDeviceSynchronize();  // never release memory an in-flight kernel may still touch
+            (void)hipFree(p);
+        }
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+};
+
+// One level of the cell pyramid: open-addressing hash from Morton cell code to (start, count) in the
+// Morton-ordered point array.
+struct LevelTable {
+    uint32_t log2cap = 0;
+    uint64_t offset = 0;  // slot offset in the shared table
+    int32_t max_count = 0;
+    int64_t cells = 0;
+};
+
+// Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code,
+// 16 host-input staging, 17-23 free.
+struct Scratch {
+    DBuf<uint32_t> bufs[24];
+    template <class T>
+    T* get(int i, size_t count) {
+        bufs[i].alloc((count * sizeof(T) + sizeof(uint32_t) - 1) / sizeof(uint32_t));
+        return reinterpret_cast<T*>(bufs[i].p);
+    }
+};
+
+}  // namespace egt
+
+struct egt_ctx {
+    int kind = 0;  // 0 cloud, 1 mesh
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int64_t n = 0;
+    int k = 0;
+    int B = 1024;
+    float eps = 1e-6f;
+    float min_extent = 0, max_extent = 0;
+    bool have_knn = false, smoothed = false;
+
+    // sorted space (reference layout)
+    egt::DBuf<float> x, y, z;
+    egt::DBuf<int32_t> perm;  // sorted -> original
+    egt::DBuf<int32_t> key;   // fine row-major cell key of each sorted point
+    egt::DBuf<int32_t> knn;   // n*k rows
+
+    // search structure (Morton order)
+    egt::DBuf<float4> mpts;      // x, y, z, bits(sorted index)
+    egt::DBuf<uint32_t> mcell;   // packed fine cell coords cx | cy<<10 | cz<<20
+    egt::DBuf<uint32_t> mcode;   // fine Morton code
+    egt::DBuf<uint32_t> hkeys;   // all level tables
+    egt::DBuf<uint2> hvals;
+    egt::LevelTable lev[EGT_MAX_LEVELS];
+    int nlev = 0;
+
+    // EGG CSR
+    egt::DBuf<int32_t> rowptr, col;
+    int64_t nnz = 0;
+
+    // mesh
+    egt::DBuf<uint32_t> tri;
+    int64_t ntri = 0;
+
+    // learned proposal
+    egt::DBuf<int8_t> qlevel;
+
+    // options
+    int start_level = -1;
+    int learned = 0;
+    int taubin_graph = 1;
+    int maxc = 100000;
+
+    egt_stats stats{};
+    egt::Scratch scratch;
+};
+
+namespace egt {
+// scan / sort / reduce utilities (egt_prims.hip)
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
+                        uint32_t* total_host);
+void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, Scratch& sc);
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
+
+// grid (egt_grid.hip)
+void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz);
+void build_search_structure(egt_ctx* h);
+
+// kNN (egt_knn.hip)
+void run_knn(egt_ctx* h, int k, float thres_n);
+
+// EGG + CSR (egt_egg.hip)
+void run_egg(egt_ctx* h, float alpha);
+void build_csr_from_rows(egt_ctx* h);
+
+// Taubin (egt_taubin.hip)
+void run_taubin(egt_ctx* h, int iters, float lambda, float mu);
+
+// mesh (egt_mesh.hip)
+void run_mesh_adjacency(egt_ctx* h, int k);
+
+// normals (egt_normals.hip)
+void run_normals(egt_ctx* h, float* nx_host, float* ny_host, float* nz_host);
+
+// output (egt_grid.hip)
+void run_output(egt_ctx* h, float* x, float* y, float* z);
+void run_knn_original(egt_ctx* h, int32_t* out);
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+}  // namespace egt

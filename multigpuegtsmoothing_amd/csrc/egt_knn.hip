@@ -1,0 +1,326 @@
+// Exact self kNN over the cell pyramid, with the reference's row-validity rule.
+//
+// Reference: GridStructure::GRIDCUDAKNNSELF_COMPACT, GridKNNCuda.cu:2322-2634, kernel
+// construct_neighbors_self_compact :647-743, insertion appendDistanceGrid :311-339, certificate :694-741,
+// invalidation :217-226, :287-309.
+//
+// One thread per query, queries walked in Morton order so a wave's 64 queries are spatial neighbours and
+// share cache lines of candidate tiles. The search box at pyramid level l is the 3x3x3 block of level-l
+// cells around the query's cell (cells of level l are contiguous ranges of the Morton-ordered point array,
+// found through a per-level hash). Each level scans only cells and points outside the previous box, keeps
+// a register-resident sorted top-K of 64-bit keys (distance bits << 32 | sorted index) and stops when the
+// k-th distance plus a 2e-6 margin (double) is below every interior face distance, so rows are the exact
+// k nearest neighbours ordered by (distance, sorted index). The reference's float certificate is then
+// replayed on that exact list, stage by stage (own cell, then ring 1 at bucket counts B, B/2, ..., 8), to
+// decide whether the reference would have blanked the row (cell with >= maxc points, or still searching
+// after the bucket-count-8 ring).
+#include <algorithm>
+#include "egt_device.h"
+#include "egt_internal.h"
+
+namespace egt {
+namespace {
+
+constexpr int KBLK = 256;
+constexpr uint64_t KEY_EMPTY = ~0ull;
+
+struct KnnParams {
+    const float4* __restrict__ mpts;
+    const uint32_t* __restrict__ mcell;
+    const uint32_t* __restrict__ hkeys;
+    const uint2* __restrict__ hvals;
+    const int32_t* __restrict__ key;       // sorted-space fine key (member cell coordinates)
+    const int8_t* __restrict__ qlevel;     // per-query start level (Morton order) or null
+    int32_t* __restrict__ knn;             // n*k rows, sorted space
+    float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
+    unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks
+    LevelDev lev[EGT_MAX_LEVELS];
+    int64_t n;
+    int k, B, log2B, nlev, start_level, stages, maxc;
+    uint32_t bigmask;  // levels holding a cell with >= maxc points
+    float a, ba, thres_n;
+};
+
+template <int K>
+__device__ inline void topk_insert(uint64_t (&L)[K], uint64_t key) {
+#pragma unroll
+    for (int i = K - 1; i > 0; --i) {
+        const bool lt_prev = key < L[i - 1];
+        const bool lt_cur = key < L[i];
+        L[i] = lt_prev ? L[i - 1] : (lt_cur ? key : L[i]);
+    }
+    L[0] = key < L[0] ? key : L[0];
+}
+
+// Reference certificate (GridKNNCuda.cu:694-741) at stage s: true = the reference keeps searching.
+__device__ inline bool ref_continue(const KnnParams& P, int s, float x, float y, float z, int fx, int fy, int fz, int c,
+                                    float kthd) {
+    const int lvl = s == 0 ? 0 : s - 1;
+    const int off = s == 0 ? 0 : 1;
+    const int Bl = P.B >> lvl;
+    const float h = P.ba / (float)Bl;
+    const float v[3] = {x, y, z};
+    const int ci[3] = {fx >> lvl, fy >> lvl, fz >> lvl};
+    bool cont = false;
+    if (c < P.k) {
+        const float r = __builtin_sqrtf(P.thres_n);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (ci[ax] - off >= 0 && r > __builtin_fmaf(-(float)(ci[ax] - off), h, v[ax] - P.a)) cont = true;
+            if (ci[ax] + off + 1 <= Bl && r > __builtin_fmaf((float)(ci[ax] + off + 1), h, P.a - v[ax])) cont = true;
+        }
+    } else {
+        const float r = __builtin_sqrtf(kthd);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (ci[ax] - off >= 0 && r >= __builtin_fmaf(-(float)(ci[ax] - off), h, v[ax] - P.a)) cont = true;
+            if (ci[ax] + off + 1 <= Bl && r >= __builtin_fmaf((float)(ci[ax] + off + 1), h, P.a - v[ax])) cont = true;
+        }
+    }
+    return cont;
+}
+
+__device__ inline uint32_t cell_count(const KnnParams& P, int lvl, int cx, int cy, int cz) {
+    const int Bl = P.B >> lvl;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= Bl || cy >= Bl || cz >= Bl) return 0;
+    return level_lookup(P.hkeys, P.hvals, P.lev[lvl], morton_code(cx, cy, cz)).y;
+}
+
+template <int K>
+__global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
+    const int64_t m = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    const bool active = m < P.n;
+    uint64_t cand = 0;
+    uint32_t invalid_row = 0, learned = 0, fallback = 0;
+    if (active) {
+        const float4 q = P.mpts[m];
+        const int32_t jq = __float_as_int(q.w);
+        const uint32_t qc = P.mcell[m];
+        const int fx = qc & 1023, fy = (qc >> 10) & 1023, fz = qc >> 20;
+        uint64_t L[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) L[i] = (i < K - P.k) ? 0ull : KEY_EMPTY;
+
+        int level = P.start_level;
+        if (P.qlevel) {
+            level = P.qlevel[m];
+            learned = 1;
+        }
+        level = level < 0 ? 0 : (level >= P.nlev ? P.nlev - 1 : level);
+        const int first_level = level;
+        int plo[3] = {P.B, P.B, P.B}, phi[3] = {-1, -1, -1};  // previous box (fine coords), empty
+        for (;;) {
+            const int Bl = P.B >> level;
+            const int cq[3] = {fx >> level, fy >> level, fz >> level};
+            for (int t = 0; t < 27; ++t) {
+                const int c3[3] = {cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1, cq[2] + (t / 9) - 1};
+                if (c3[0] < 0 || c3[1] < 0 || c3[2] < 0 || c3[0] >= Bl || c3[1] >= Bl || c3[2] >= Bl) continue;
+                bool inside = true, overlap = true;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int lo = c3[ax] << level, hi = ((c3[ax] + 1) << level) - 1;
+                    inside = inside && lo >= plo[ax] && hi <= phi[ax];
+                    overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
+                }
+                if (inside) continue;
+                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level], morton_code(c3[0], c3[1], c3[2]));
+                for (uint32_t p = r.x; p < r.x + r.y; ++p) {
+                    const float4 e = P.mpts[p];
+                    const int32_t j = __float_as_int(e.w);
+                    if (j == jq) continue;
+                    if (overlap) {
+                        const uint32_t cc = P.mcell[p];
+                        const int ex = cc & 1023, ey = (cc >> 10) & 1023, ez = cc >> 20;
+                        if (ex >= plo[0] && ex <= phi[0] && ey >= plo[1] && ey <= phi[1] && ez >= plo[2] &&
+                            ez <= phi[2])
+                            continue;
+                    }
+                    ++cand;
+                    const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
+                    if (d <= P.thres_n) {
+                        const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
+                        if (kk < L[K - 1]) topk_insert<K>(L, kk);
+                    }
+                }
+            }
+            if (level >= P.nlev - 1) break;
+            // exactness: every unscanned point is farther than the current bound
+            const bool full = L[K - 1] != KEY_EMPTY;
+            const float bound = full ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : P.thres_n;
+            const double R = __builtin_sqrt((double)bound) + 2e-6;
+            const double hl = (double)P.ba / (double)Bl;
+            const double qv[3] = {q.x, q.y, q.z};
+            bool exact = true;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                if (cq[ax] - 1 >= 1 && !(R < qv[ax] - ((double)P.a + (double)(cq[ax] - 1) * hl))) exact = false;
+                if (cq[ax] + 1 <= Bl - 2 && !(R < ((double)P.a + (double)(cq[ax] + 2) * hl) - qv[ax])) exact = false;
+            }
+            if (exact) break;
+            if (level == first_level && learned) fallback = 1;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                plo[ax] = max(0, (cq[ax] - 1) << level);
+                phi[ax] = min(P.B - 1, ((cq[ax] + 2) << level) - 1);
+            }
+            ++level;
+        }
+
+        // replay of the reference schedule on the exact list
+        int c = 0;
+        int lo[3] = {P.B, P.B, P.B}, hi[3] = {-1, -1, -1};
+        const uint32_t bm = (uint32_t)P.B - 1u;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (i >= K - P.k && L[i] != KEY_EMPTY) {
+                ++c;
+                const uint32_t kk = (uint32_t)P.key[(uint32_t)L[i]];
+                const int mx = kk & bm, my = (kk >> P.log2B) & bm, mz = kk >> (2 * P.log2B);
+                lo[0] = min(lo[0], mx); hi[0] = max(hi[0], mx);
+                lo[1] = min(lo[1], my); hi[1] = max(hi[1], my);
+                lo[2] = min(lo[2], mz); hi[2] = max(hi[2], mz);
+            }
+        }
+        const float kthd = c > 0 ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : 0.0f;
+        bool valid = false;
+        for (int s = 0; s <= P.stages; ++s) {
+            const int lvl = s == 0 ? 0 : s - 1;
+            const int off = s == 0 ? 0 : 1;
+            const int cx = fx >> lvl, cy = fy >> lvl, cz = fz >> lvl;
+            if ((P.bigmask >> lvl) & 1u) {
+                bool big = false;
+                if (s == 0) {
+                    big = cell_count(P, 0, cx, cy, cz) >= (uint32_t)P.maxc;
+                } else {
+                    for (int t = 0; t < 27 && !big; ++t) {
+                        if (t == 13) continue;
+                        big = cell_count(P, lvl, cx + (t % 3) - 1, cy + ((t / 3) % 3) - 1, cz + (t / 9) - 1) >=
+                              (uint32_t)P.maxc;
+                    }
+                }
+                if (big) break;
+            }
+            const bool inside = c == 0 || ((lo[0] >> lvl) >= cx - off && (hi[0] >> lvl) <= cx + off &&
+                                           (lo[1] >> lvl) >= cy - off && (hi[1] >> lvl) <= cy + off &&
+                                           (lo[2] >> lvl) >= cz - off && (hi[2] >> lvl) <= cz + off);
+            if (inside && !ref_continue(P, s, q.x, q.y, q.z, fx, fy, fz, c, kthd)) {
+                valid = true;
+                break;
+            }
+        }
+        int32_t* row = P.knn + (int64_t)jq * P.k;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (i >= K - P.k) {
+                const int32_t v = (valid && L[i] != KEY_EMPTY) ? (int32_t)(uint32_t)L[i] : -1;
+                row[i - (K - P.k)] = v;
+            }
+        }
+        P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
+        invalid_row = valid ? 0u : 1u;
+    }
+    // one atomic per wave
+    unsigned long long c0 = cand;
+    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        c0 += __shfl_xor(c0, d, 64);
+        c1 += __shfl_xor(c1, d, 64);
+        c2 += __shfl_xor(c2, d, 64);
+        c3 += __shfl_xor(c3, d, 64);
+    }
+    if (__lane_id() == 0) {
+        if (c0) atomicAdd(&P.counters[0], c0);
+        if (c1) atomicAdd(&P.counters[1], (unsigned long long)c1);
+        if (c2) atomicAdd(&P.counters[2], (unsigned long long)c2);
+        if (c3) atomicAdd(&P.counters[3], (unsigned long long)c3);
+    }
+}
+
+template <int K>
+void launch_knn(const KnnParams& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_knn<K>, dim3(ceil_div(P.n, KBLK)), dim3(KBLK), 0, s, P);
+}
+
+}  // namespace
+
+int auto_start_level(const egt_ctx* h, int k) {
+    // smallest level whose mean non-empty cell holds at least k/4 points (a 3x3x3 box around a surface
+    // point then holds ~2k candidates)
+    for (int l = 0; l < h->nlev; ++l) {
+        const double mean = (double)h->n / (double)std::max<int64_t>(1, h->lev[l].cells);
+        if (mean >= 0.25 * k) return l;
+    }
+    return h->nlev - 1;
+}
+
+void run_knn(egt_ctx* h, int k, float thres_n) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    KnnParams P{};
+    P.mpts = h->mpts.p;
+    P.mcell = h->mcell.p;
+    P.hkeys = h->hkeys.p;
+    P.hvals = h->hvals.p;
+    P.key = h->key.p;
+    P.qlevel = (h->learned && h->qlevel.p) ? h->qlevel.p : nullptr;
+    h->knn.alloc((size_t)n * k);
+    P.knn = h->knn.p;
+    P.kth = h->scratch.get<float>(15, n);
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 4));
+    HIP_CHECK(hipMemsetAsync(counters, 0, 4 * sizeof(uint64_t), s));
+    P.counters = counters;
+    P.bigmask = 0;
+    for (int l = 0; l < h->nlev; ++l) {
+        P.lev[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
+        if (h->lev[l].max_count >= h->maxc) P.bigmask |= 1u << l;
+    }
+    P.n = n;
+    P.k = k;
+    P.B = h->B;
+    int log2B = 0;
+    while ((1 << log2B) < h->B) ++log2B;
+    P.log2B = log2B;
+    P.nlev = h->nlev;
+    P.start_level = h->start_level >= 0 ? h->start_level : auto_start_level(h, k);
+    int stages = 0;
+    for (int b = h->B; b >= 8; b >>= 1) ++stages;
+    P.stages = std::max(stages, 1);
+    P.maxc = h->maxc;
+    P.a = -h->eps;
+    P.ba = (1.0f + h->eps) - P.a;
+    P.thres_n = thres_n;
+
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    const int K = k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : k <= 32 ? 32 : k <= 40 ? 40 : k <= 48 ? 48
+                : k <= 56 ? 56 : 64;
+    switch (K) {
+        case 8: launch_knn<8>(P, s); break;
+        case 16: launch_knn<16>(P, s); break;
+        case 24: launch_knn<24>(P, s); break;
+        case 32: launch_knn<32>(P, s); break;
+        case 40: launch_knn<40>(P, s); break;
+        case 48: launch_knn<48>(P, s); break;
+        case 56: launch_knn<56>(P, s); break;
+        default: launch_knn<64>(P, s); break;
+    }
+    LAUNCH_CHECK();
+    HIP_CHECK(hipEventRecord(e1, s));
+    uint64_t hc[4];
+    HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->stats.knn_candidates = (int64_t)hc[0];
+    h->stats.invalid_rows = (int64_t)hc[1];
+    h->stats.learned_queries = (int64_t)hc[2];
+    h->stats.learned_fallbacks = (int64_t)hc[3];
+    h->k = k;
+    h->have_knn = true;
+}
+
+}  // namespace egt

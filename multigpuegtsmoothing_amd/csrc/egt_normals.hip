@@ -1,0 +1,409 @@
+// PCA normals and their orientation over a minimum spanning forest of the neighbour graph.
+//
+// Reference: Normal_CUDA::GetNormals Normal_CUDA.cu:304-775 with calculateNormal :87-144, getEdgeNumber /
+// fillEdges :61-85, :146-175, mst() CUDA_MST.cu:423-460 (Boruvka), BFS bfs.cu:63-111 (sign propagation).
+//
+// 1. PCA: one thread per point; centroid of the point and its neighbours, covariance of the neighbours,
+//    eigenvector of the smallest eigenvalue by cyclic Jacobi (egt_device.h).
+// 2. Edges: every undirected edge {u < v} of the rows once, in (u, v) order: row u owns it when it lists v,
+//    otherwise row v does; per-u buckets come from a counting pass + prefix sum and are sorted by v.
+//    Weight max(0, 1 - |n_u . n_v|); edge key = (weight bits << 32) | edge id, a total order equal to
+//    (weight, u, v).
+// 3. Boruvka with parity: each component takes its minimum-key edge; mutual pairs keep the smaller id as
+//    root; hooks carry the parity bit par(u) ^ par(v) ^ (n_u . n_v < 0); pointer jumping on packed
+//    (target << 1 | parity) words resolves roots and parities. The forest is the unique minimum spanning
+//    forest under the key order.
+// 4. Orientation: each tree's root is u of its lightest edge; a vertex flips when its parity differs from
+//    the root's, which equals the reference's BFS rule "flip the child when n_parent . n_child < 0".
+#include <algorithm>
+#include "egt_device.h"
+#include "egt_internal.h"
+
+namespace egt {
+namespace {
+
+constexpr int NBLK = 256;
+constexpr uint64_t KEY_EMPTY = ~0ull;
+
+__global__ void __launch_bounds__(NBLK) k_pca(const float* __restrict__ x, const float* __restrict__ y,
+                                               const float* __restrict__ z, const int32_t* __restrict__ knn, int64_t n,
+                                               int k, float* __restrict__ nx, float* __restrict__ ny,
+                                               float* __restrict__ nz) {
+    const int64_t i = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t* row = knn + i * k;
+    float sx = x[i], sy = y[i], sz = z[i];
+    int counter = 1;
+    for (int j = 0; j < k; ++j) {
+        const int32_t nb = row[j];
+        if (nb >= 0) {
+            sx += x[nb];
+            sy += y[nb];
+            sz += z[nb];
+            ++counter;
+        }
+    }
+    const float cx = sx / (float)counter, cy = sy / (float)counter, cz = sz / (float)counter;
+    float a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
+    for (int j = 0; j < k; ++j) {
+        const int32_t nb = row[j];
+        if (nb < 0) continue;
+        const float dx = x[nb] - cx, dy = y[nb] - cy, dz = z[nb] - cz;
+        a00 = __builtin_fmaf(dx, dx, a00);
+        a01 = __builtin_fmaf(dx, dy, a01);
+        a02 = __builtin_fmaf(dx, dz, a02);
+        a11 = __builtin_fmaf(dy, dy, a11);
+        a12 = __builtin_fmaf(dy, dz, a12);
+        a22 = __builtin_fmaf(dz, dz, a22);
+    }
+    float vx, vy, vz;
+    eig3_smallest(a00, a01, a02, a11, a12, a22, vx, vy, vz);
+    const float nrm = __builtin_sqrtf(__builtin_fmaf(vz, vz, __builtin_fmaf(vy, vy, vx * vx)));
+    nx[i] = vx / nrm;
+    ny[i] = vy / nrm;
+    nz[i] = vz / nrm;
+}
+
+__device__ inline bool row_has(const int32_t* __restrict__ knn, int k, int32_t r, int32_t v) {
+    const int32_t* row = knn + (int64_t)r * k;
+    for (int j = 0; j < k; ++j)
+        if (row[j] == v) return true;
+    return false;
+}
+
+// pass 0: count per owner u; pass 1: write v into u's bucket
+template <int PASS>
+__global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ knn, int64_t n, int k,
+                                                 uint32_t* __restrict__ cnt, int32_t* __restrict__ ev) {
+    const int64_t i = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t* row = knn + i * k;
+    for (int j = 0; j < k; ++j) {
+        const int32_t nb = row[j];
+        if (nb < 0 || nb == i) continue;
+        int32_t u, v;
+        if (i < nb) {
+            u = (int32_t)i;
+            v = nb;
+        } else {
+            if (row_has(knn, k, nb, (int32_t)i)) continue;  // row nb owns it
+            u = nb;
+            v = (int32_t)i;
+        }
+        const uint32_t p = atomicAdd(&cnt[u], 1u);
+        if (PASS == 1) ev[p] = v;
+    }
+}
+
+__global__ void __launch_bounds__(NBLK) k_edge_sort(const uint32_t* __restrict__ off, int64_t n, uint32_t m,
+                                                     int32_t* __restrict__ ev, int32_t* __restrict__ eu) {
+    const int64_t u = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (u >= n) return;
+    const uint32_t b = off[u], e = u + 1 < n ? off[u + 1] : m;
+    for (uint32_t a = b + 1; a < e; ++a) {  // insertion sort of the bucket by v
+        const int32_t t = ev[a];
+        uint32_t c = a;
+        while (c > b && ev[c - 1] > t) {
+            ev[c] = ev[c - 1];
+            --c;
+        }
+        ev[c] = t;
+    }
+    for (uint32_t a = b; a < e; ++a) eu[a] = (int32_t)u;
+}
+
+__global__ void __launch_bounds__(NBLK) k_edge_attr(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
+                                                     uint32_t m, const float* __restrict__ nx,
+                                                     const float* __restrict__ ny, const float* __restrict__ nz,
+                                                     uint64_t* __restrict__ ekey, uint8_t* __restrict__ erel,
+                                                     float* __restrict__ ew) {
+    const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
+    if (e >= m) return;
+    const int32_t u = eu[e], v = ev[e];
+    const float dot = __builtin_fmaf(nz[u], nz[v], __builtin_fmaf(ny[u], ny[v], nx[u] * nx[v]));
+    const float w = fmaxf(0.0f, 1.0f - __builtin_fabsf(dot));
+    ew[e] = w;
+    ekey[e] = ((uint64_t)__float_as_uint(w) << 32) | e;
+    erel[e] = dot < 0.0f ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_init(int64_t n, int32_t* __restrict__ comp, uint8_t* __restrict__ par) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (v >= n) return;
+    comp[v] = (int32_t)v;
+    par[v] = 0;
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_reset(int64_t n, uint64_t* __restrict__ best, uint32_t* __restrict__ H) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (v >= n) return;
+    best[v] = KEY_EMPTY;
+    H[v] = (uint32_t)v << 1;
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
+                                                   const uint64_t* __restrict__ ekey, uint32_t m,
+                                                   const int32_t* __restrict__ comp, uint64_t* __restrict__ best,
+                                                   uint32_t* __restrict__ live) {
+    const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
+    if (e >= m) return;
+    const int32_t cu = comp[eu[e]], cv = comp[ev[e]];
+    if (cu == cv) return;
+    const uint64_t key = ekey[e];
+    atomicMin((unsigned long long*)&best[cu], (unsigned long long)key);
+    atomicMin((unsigned long long*)&best[cv], (unsigned long long)key);
+    if (live) *live = 1u;
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __restrict__ eu,
+                                                   const int32_t* __restrict__ ev, const uint8_t* __restrict__ erel,
+                                                   const int32_t* __restrict__ comp, const uint8_t* __restrict__ par,
+                                                   const uint64_t* __restrict__ best, uint32_t* __restrict__ H,
+                                                   uint8_t* __restrict__ msf, uint32_t* __restrict__ hooks) {
+    const int64_t c = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (c >= n || comp[c] != c) return;
+    const uint64_t b = best[c];
+    if (b == KEY_EMPTY) return;
+    const uint32_t e = (uint32_t)b;
+    const int32_t u = eu[e], v = ev[e];
+    const int32_t cu = comp[u], cv = comp[v];
+    const int32_t other = cu == (int32_t)c ? cv : cu;
+    if (best[other] == b && (int32_t)c < other) return;  // mutual pair: the smaller id stays root
+    H[c] = ((uint32_t)other << 1) | (uint32_t)(par[u] ^ par[v] ^ erel[e]);
+    msf[e] = 1;
+    atomicAdd(hooks, 1u);
+}
+
+// Follows hook pointers to the root. The hook graph is a forest (edge keys are a strict total order and
+// mutual pairs keep one root), so every walk ends; the walk is still capped at n steps and a cap hit is
+// reported through *overflow instead of spinning.
+__global__ void __launch_bounds__(NBLK) k_bv_jump(int64_t n, const int32_t* __restrict__ comp,
+                                                   uint32_t* __restrict__ H, uint32_t* __restrict__ overflow) {
+    const int64_t c = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (c >= n || comp[c] != c) return;
+    uint32_t w = H[c];
+    int64_t steps = 0;
+    for (;;) {
+        const uint32_t t = w >> 1;
+        const uint32_t w2 = __hip_atomic_load(&H[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((w2 >> 1) == t) break;
+        w = (w2 & ~1u) | ((w ^ w2) & 1u);
+        if (++steps > n) {
+            atomicAdd(overflow, 1u);
+            break;
+        }
+    }
+    __hip_atomic_store(&H[c], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_relabel(int64_t n, int32_t* __restrict__ comp, uint8_t* __restrict__ par,
+                                                      const uint32_t* __restrict__ H) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (v >= n) return;
+    const uint32_t w = H[comp[v]];
+    comp[v] = (int32_t)(w >> 1);
+    par[v] ^= (uint8_t)(w & 1u);
+}
+
+__global__ void __launch_bounds__(NBLK) k_tree_min(const uint8_t* __restrict__ msf, const uint64_t* __restrict__ ekey,
+                                                    const int32_t* __restrict__ eu, uint32_t m,
+                                                    const int32_t* __restrict__ comp, uint64_t* __restrict__ tmin) {
+    const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
+    if (e >= m || !msf[e]) return;
+    atomicMin((unsigned long long*)&tmin[comp[eu[e]]], (unsigned long long)ekey[e]);
+}
+
+__global__ void __launch_bounds__(NBLK) k_orient(int64_t n, const int32_t* __restrict__ comp,
+                                                  const uint8_t* __restrict__ par, const uint64_t* __restrict__ tmin,
+                                                  const int32_t* __restrict__ eu, float* __restrict__ nx,
+                                                  float* __restrict__ ny, float* __restrict__ nz) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (v >= n) return;
+    const uint64_t t = tmin[comp[v]];
+    if (t == KEY_EMPTY) return;
+    const int32_t root = eu[(uint32_t)t];
+    if (par[v] ^ par[root]) {
+        nx[v] = -nx[v];
+        ny[v] = -ny[v];
+        nz[v] = -nz[v];
+    }
+}
+
+// deterministic forest statistics: per-block partial sums, summed on the host in block order
+__global__ void __launch_bounds__(NBLK) k_msf_stats(const uint8_t* __restrict__ msf, const float* __restrict__ ew,
+                                                     uint32_t m, double* __restrict__ part_w,
+                                                     uint32_t* __restrict__ part_c) {
+    __shared__ double sw[NBLK];
+    __shared__ uint32_t scnt[NBLK];
+    double w = 0.0;
+    uint32_t c = 0;
+    for (uint32_t e = blockIdx.x * NBLK + threadIdx.x; e < m; e += gridDim.x * NBLK)
+        if (msf[e]) {
+            w += (double)ew[e];
+            ++c;
+        }
+    sw[threadIdx.x] = w;
+    scnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = NBLK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sw[threadIdx.x] += sw[threadIdx.x + s];
+            scnt[threadIdx.x] += scnt[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_w[blockIdx.x] = sw[0];
+        part_c[blockIdx.x] = scnt[0];
+    }
+}
+
+__global__ void __launch_bounds__(NBLK) k_count_roots(int64_t n, const int32_t* __restrict__ comp,
+                                                       const uint64_t* __restrict__ tmin, uint32_t* __restrict__ trees) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (v >= n) return;
+    if (comp[v] == v && tmin[v] != KEY_EMPTY) atomicAdd(trees, 1u);
+}
+
+__global__ void __launch_bounds__(NBLK) k_scatter_normals(const float* __restrict__ nx, const float* __restrict__ ny,
+                                                           const float* __restrict__ nz, const int32_t* __restrict__ perm,
+                                                           int64_t n, float* __restrict__ ox, float* __restrict__ oy,
+                                                           float* __restrict__ oz) {
+    const int64_t s = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (s >= n) return;
+    const int32_t o = perm[s];
+    ox[o] = nx[s];
+    oy[o] = ny[s];
+    oz[o] = nz[s];
+}
+
+template <class T>
+T* dalloc(DBuf<uint8_t>& b, size_t& off, size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = reinterpret_cast<T*>(b.p + off);
+    off += count * sizeof(T);
+    return p;
+}
+
+}  // namespace
+
+void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
+    const int64_t n = h->n;
+    const int k = h->k;
+    hipStream_t s = h->stream;
+    hipEvent_t ev0, ev1;
+    HIP_CHECK(hipEventCreate(&ev0));
+    HIP_CHECK(hipEventCreate(&ev1));
+    HIP_CHECK(hipEventRecord(ev0, s));
+    const dim3 gn(ceil_div(n, NBLK)), blk(NBLK);
+
+    float* nrm = h->scratch.get<float>(8, 6 * (size_t)n);
+    float *nx = nrm, *ny = nrm + n, *nz = nrm + 2 * n;
+    k_pca<<<gn, blk, 0, s>>>(h->x.p, h->y.p, h->z.p, h->knn.p, n, k, nx, ny, nz);
+    LAUNCH_CHECK();
+
+    // canonical undirected edge list
+    uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
+    uint32_t* off = h->scratch.get<uint32_t>(10, n);
+    HIP_CHECK(hipMemsetAsync(cnt, 0, n * sizeof(uint32_t), s));
+    k_edges<0><<<gn, blk, 0, s>>>(h->knn.p, n, k, cnt, nullptr);
+    LAUNCH_CHECK();
+    uint32_t m = 0;
+    exclusive_scan_u32(cnt, off, n, s, h->scratch, &m);
+    HIP_CHECK(hipMemcpyAsync(cnt, off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    DBuf<uint8_t> pool;
+    const size_t mm = std::max<uint32_t>(m, 1);
+    pool.alloc(mm * (4 + 4 + 8 + 1 + 1 + 4) + (size_t)n * (4 + 1 + 8 + 4 + 8) + 16 * 256 + 4096 * 12);
+    size_t po = 0;
+    int32_t* ev = dalloc<int32_t>(pool, po, mm);
+    int32_t* eu = dalloc<int32_t>(pool, po, mm);
+    uint64_t* ekey = dalloc<uint64_t>(pool, po, mm);
+    uint8_t* erel = dalloc<uint8_t>(pool, po, mm);
+    uint8_t* msf = dalloc<uint8_t>(pool, po, mm);
+    float* ew = dalloc<float>(pool, po, mm);
+    int32_t* comp = dalloc<int32_t>(pool, po, n);
+    uint8_t* par = dalloc<uint8_t>(pool, po, n);
+    uint64_t* best = dalloc<uint64_t>(pool, po, n);
+    uint32_t* H = dalloc<uint32_t>(pool, po, n);
+    uint64_t* tmin = dalloc<uint64_t>(pool, po, n);
+    uint32_t* flags = dalloc<uint32_t>(pool, po, 4);
+    double* part_w = dalloc<double>(pool, po, 1024);
+    uint32_t* part_c = dalloc<uint32_t>(pool, po, 1024);
+    if (po > pool.n) throw ArgError{EGT_ERR_STATE, "normals pool overflow"};
+
+    k_edges<1><<<gn, blk, 0, s>>>(h->knn.p, n, k, cnt, ev);
+    LAUNCH_CHECK();
+    k_edge_sort<<<gn, blk, 0, s>>>(off, n, m, ev, eu);
+    LAUNCH_CHECK();
+    const dim3 gm(std::max(1, ceil_div(m, NBLK)));
+    if (m > 0) {
+        k_edge_attr<<<gm, blk, 0, s>>>(eu, ev, m, nx, ny, nz, ekey, erel, ew);
+        LAUNCH_CHECK();
+    }
+    HIP_CHECK(hipMemsetAsync(msf, 0, mm, s));
+
+    // Boruvka with parity
+    k_bv_init<<<gn, blk, 0, s>>>(n, comp, par);
+    LAUNCH_CHECK();
+    for (int round = 0; round < 64 && m > 0; ++round) {
+        k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
+        HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
+        k_bv_best<<<gm, blk, 0, s>>>(eu, ev, ekey, m, comp, best, nullptr);
+        k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf, flags);
+        k_bv_jump<<<gn, blk, 0, s>>>(n, comp, H, flags + 1);
+        k_bv_relabel<<<gn, blk, 0, s>>>(n, comp, par, H);
+        LAUNCH_CHECK();
+        uint32_t hf[2] = {0, 0};
+        HIP_CHECK(hipMemcpyAsync(hf, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (hf[1]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
+        if (hf[0] == 0) break;
+    }
+
+    // orientation
+    k_bv_reset<<<gn, blk, 0, s>>>(n, tmin, H);
+    LAUNCH_CHECK();
+    if (m > 0) {
+        k_tree_min<<<gm, blk, 0, s>>>(msf, ekey, eu, m, comp, tmin);
+        LAUNCH_CHECK();
+    }
+    k_orient<<<gn, blk, 0, s>>>(n, comp, par, tmin, eu, nx, ny, nz);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipEventRecord(ev1, s));
+
+    // statistics
+    HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
+    k_count_roots<<<gn, blk, 0, s>>>(n, comp, tmin, flags);
+    const int sb = std::min(1024, std::max(1, ceil_div(m, NBLK)));
+    k_msf_stats<<<sb, blk, 0, s>>>(msf, ew, m, part_w, part_c);
+    LAUNCH_CHECK();
+    std::vector<double> hw(sb);
+    std::vector<uint32_t> hc(sb);
+    uint32_t trees = 0;
+    HIP_CHECK(hipMemcpyAsync(hw.data(), part_w, sb * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(hc.data(), part_c, sb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&trees, flags, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+
+    // original order
+    float* on = nrm + 3 * n;
+    k_scatter_normals<<<gn, blk, 0, s>>>(nx, ny, nz, h->perm.p, n, on, on + n, on + 2 * n);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(nxh, on, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(nyh, on + n, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(nzh, on + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipEventElapsedTime(&h->stats.normals_ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    double W = 0.0;
+    int64_t C = 0;
+    for (int b = 0; b < sb; ++b) {
+        W += hw[b];
+        C += hc[b];
+    }
+    h->stats.msf_weight = W;
+    h->stats.msf_edges = C;
+    h->stats.msf_trees = trees;
+}
+
+}  // namespace egt

@@ -1,0 +1,285 @@
+"""Python mirror of the reference's provider / consumer classes over the gfx950 C ABI (include/egt.h).
+
+Class and method names follow the reference so call sites read alike:
+
+    GridStructure(points, num_cards=1, eps=1e-6, bucketCount=1024, maxDivision=4)   GridKNNCuda.cuh:15-16
+        .GRIDCUDAKNNSELF_COMPACT(k, thres)                                            GridKNNCuda.cuh:22
+    GridMeshCuda(points, triangles, num_cards=1, eps=1e-6, bucketCount=1024)        GridMeshCuda1.cuh:16
+        .GetNeighborsFromTriangles(k)                                                 GridMeshCuda1.cuh:17
+    EGT_CUDA(provider).PerformSmoothing(iterations, lambda_=0.63, mu=-0.64, alpha=1e-8)  EGT_CUDA.cuh:13-14
+    Normal_CUDA(provider).GetNormals()                                                Normal_CUDA.cuh:12-13
+    getOutputPoints(provider)                                                         getOutputPoints.cuh:10
+
+Errors raise EGTError carrying the library's status code and message. The library is loaded from
+multigpuegtsmoothing_amd/libegt.so; importing this module without it raises immediately (there is no CPU
+fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libegt.so")
+
+EGT_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
+OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE = 0, 1, 2, 3
+
+
+class EGTError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"egt status {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_points", ctypes.c_int64), ("k", ctypes.c_int32), ("bucket_count", ctypes.c_int32),
+                ("invalid_rows", ctypes.c_int64), ("knn_candidates", ctypes.c_int64), ("egg_kept", ctypes.c_int64),
+                ("learned_queries", ctypes.c_int64), ("learned_fallbacks", ctypes.c_int64),
+                ("msf_edges", ctypes.c_int64), ("msf_trees", ctypes.c_int64), ("msf_weight", ctypes.c_double),
+                ("knn_ms", ctypes.c_float), ("egg_ms", ctypes.c_float), ("taubin_ms", ctypes.c_float),
+                ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+_F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_H = ctypes.c_void_p
+_PH = ctypes.POINTER(ctypes.c_void_p)
+_PF = ctypes.POINTER(ctypes.c_void_p)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (make -C "
+                          "multigpuegtsmoothing_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    i64, i32, f32, st = ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_int
+    sig = {
+        "egt_last_error_string": ([], ctypes.c_char_p),
+        "egt_grid_create": ([_F32P, _F32P, _F32P, i64, i32, f32, i32, i32, _PH], st),
+        "egt_grid_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, i64, i32, f32, i32, i32, _PH], st),
+        "egt_grid_knn_self": ([_H, i32, f32], st),
+        "egt_mesh_create": ([_F32P, _F32P, _F32P, i64, _U32P, i64, i32, f32, i32, _PH], st),
+        "egt_mesh_neighbors": ([_H, i32], st),
+        "egt_smooth": ([_H, i32, f32, f32, f32], st),
+        "egt_normals": ([_H, _F32P, _F32P, _F32P], st),
+        "egt_get_output_points": ([_H, _F32P, _F32P, _F32P], st),
+        "egt_get_knn_original": ([_H, _I32P], st),
+        "egt_get_device_points": ([_H, _PF, _PF, _PF], st),
+        "egt_get_device_knn": ([_H, _PF], st),
+        "egt_get_device_original_indexes": ([_H, _PF], st),
+        "egt_get_extent": ([_H, ctypes.POINTER(f32), ctypes.POINTER(f32)], st),
+        "egt_get_counts": ([_H, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                            ctypes.POINTER(i64)], st),
+        "egt_get_stats": ([_H, ctypes.POINTER(Stats)], st),
+        "egt_set_option": ([_H, i32, i64], st),
+        "egt_destroy": ([_H], st),
+        "egt_get_stream": ([_H], ctypes.c_void_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+_lib = _load()
+
+
+def lib():
+    return _lib
+
+
+def _check(status):
+    if status != EGT_OK:
+        raise EGTError(status, _lib.egt_last_error_string().decode())
+
+
+def _xyz(points):
+    if isinstance(points, (tuple, list)) and len(points) == 3:
+        x, y, z = points
+    else:
+        p = np.asarray(points, dtype=np.float32)
+        x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    return (np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32),
+            np.ascontiguousarray(z, np.float32))
+
+
+class _Provider:
+    """KNNInterface (KNNInterface.cuh:8-31) over one handle."""
+
+    def __init__(self):
+        self._h = None
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise EGTError(6, "handle destroyed")
+        return self._h
+
+    def close(self):
+        if self._h is not None:
+            _lib.egt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, opt, value):
+        _check(_lib.egt_set_option(self.handle, opt, int(value)))
+
+    # KNNInterface accessors
+    def GetNumberOfCards(self):
+        return 1
+
+    def pointsRefCount(self):
+        n = ctypes.c_int64()
+        _check(_lib.egt_get_counts(self.handle, ctypes.byref(n), None, None, None))
+        return n.value
+
+    def NeighborCount(self):
+        k = ctypes.c_int32()
+        _check(_lib.egt_get_counts(self.handle, None, ctypes.byref(k), None, None))
+        return k.value
+
+    def GetPartitionSize(self):
+        return self.pointsRefCount()
+
+    def GetMinExtent(self):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        _check(_lib.egt_get_extent(self.handle, ctypes.byref(mn), ctypes.byref(mx)))
+        return mn.value
+
+    def GetMaxExtent(self):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        _check(_lib.egt_get_extent(self.handle, ctypes.byref(mn), ctypes.byref(mx)))
+        return mx.value
+
+    def GetRefPointsDevice(self):
+        """Device pointers (int addresses) of the sorted, normalized x, y, z arrays (GetRefPointsX/Y/Z)."""
+        px, py, pz = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(_lib.egt_get_device_points(self.handle, ctypes.byref(px), ctypes.byref(py), ctypes.byref(pz)))
+        return px.value, py.value, pz.value
+
+    def GetKNNIndexesDevice(self):
+        p = ctypes.c_void_p()
+        _check(_lib.egt_get_device_knn(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def GetOriginalRefIndexesDevice(self):
+        p = ctypes.c_void_p()
+        _check(_lib.egt_get_device_original_indexes(self.handle, ctypes.byref(p)))
+        return p.value
+
+    def GetKNNIndexes(self):
+        """Neighbour rows in original index space, (n, k) int32 (GridKNNCuda.cu:1941-1968)."""
+        n, k = self.pointsRefCount(), self.NeighborCount()
+        out = np.empty(n * k, np.int32)
+        _check(_lib.egt_get_knn_original(self.handle, out))
+        return out.reshape(n, k)
+
+    def stats(self):
+        st = Stats()
+        _check(_lib.egt_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def stream(self):
+        return _lib.egt_get_stream(self.handle)
+
+
+class GridStructure(_Provider):
+    """Point-cloud provider (GridKNNCuda.cuh:15-16). points: (n, 3) array or (x, y, z); a torch CUDA tensor
+    triple is taken as device input."""
+
+    def __init__(self, points, num_cards=1, eps=1e-6, bucketCount=1024, maxDivision=4):
+        super().__init__()
+        h = ctypes.c_void_p()
+        if isinstance(points, (tuple, list)) and len(points) == 3 and hasattr(points[0], "data_ptr"):
+            x, y, z = points
+            n = x.numel()
+            self._keep = (x, y, z)
+            _check(_lib.egt_grid_create_device(x.data_ptr(), y.data_ptr(), z.data_ptr(), n, num_cards, eps,
+                                               bucketCount, maxDivision, ctypes.byref(h)))
+        else:
+            x, y, z = _xyz(points)
+            _check(_lib.egt_grid_create(x, y, z, x.size, num_cards, eps, bucketCount, maxDivision, ctypes.byref(h)))
+        self._h = h
+
+    def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
+        _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))
+
+
+class GridMeshCuda(_Provider):
+    """Mesh provider (GridMeshCuda1.cuh:16). triangles: (F, 3) uint32 original vertex indices."""
+
+    def __init__(self, points, triangles, num_cards=1, eps=1e-6, bucketCount=1024):
+        super().__init__()
+        x, y, z = _xyz(points)
+        tri = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1)
+        h = ctypes.c_void_p()
+        _check(_lib.egt_mesh_create(x, y, z, x.size, tri, tri.size // 3, num_cards, eps, bucketCount,
+                                    ctypes.byref(h)))
+        self._h = h
+
+    def GetNeighborsFromTriangles(self, k):
+        _check(_lib.egt_mesh_neighbors(self.handle, int(k)))
+
+
+class EGT_CUDA:
+    """EGG filter + Taubin smoother (EGT_CUDA.cuh:11-14)."""
+
+    def __init__(self, provider):
+        self.provider = provider
+
+    def PerformSmoothing(self, iterationCount, lambda_=0.63, mu=-0.64, alpha=1e-8):
+        _check(_lib.egt_smooth(self.provider.handle, int(iterationCount), float(lambda_), float(mu), float(alpha)))
+
+
+class Normal_CUDA:
+    """PCA normals + MST/BFS orientation (Normal_CUDA.cuh:10-13)."""
+
+    def __init__(self, provider):
+        self.provider = provider
+
+    def GetNormals(self):
+        n = self.provider.pointsRefCount()
+        nx, ny, nz = (np.empty(n, np.float32) for _ in range(3))
+        _check(_lib.egt_normals(self.provider.handle, nx, ny, nz))
+        return nx, ny, nz
+
+
+def getOutputPoints(provider):
+    """De-normalized points in original order (getOutputPoints.cu:34-63)."""
+    n = provider.pointsRefCount()
+    x, y, z = (np.empty(n, np.float32) for _ in range(3))
+    _check(_lib.egt_get_output_points(provider.handle, x, y, z))
+    return x, y, z
+
+
+def device_array(ptr, count, dtype):
+    """Copy a device buffer handed out by an accessor into a numpy array (tests / debugging)."""
+    import torch  # plumbing only
+
+    torch_dtype = {np.float32: torch.float32, np.int32: torch.int32}[np.dtype(dtype).type]
+    t = torch.empty(count, dtype=torch_dtype, device="cuda")
+    nbytes = count * np.dtype(dtype).itemsize
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise EGTError(3, f"hipMemcpy failed ({rc})")
+    return t.cpu().numpy()
