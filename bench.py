@@ -1,0 +1,147 @@
+"""Benchmark: the grid-kNN -> EGG -> Taubin -> normals path on one MI355X.
+
+Workload (one step): 4,000,000-point torus (R=1, r=0.3, density 1-10x along the major angle, noise 0.003,
+seed 2), point-cloud mode, k=40, alpha=0.65, 300 Taubin iterations (lambda 0.5, mu -0.53), PCA normals with
+MST orientation. Coordinates are resident in HBM when the timed region starts; a step builds the grid from
+them, runs the kNN, the EGG filter + Taubin loop and the normals (normals are returned to host memory, as
+the reference's GetNormals does).
+
+Prints one JSON line. value = end-to-end points/s; the per-stage rates (kNN queries/s, Taubin
+point-iterations/s) and the roofline of the dominant kernel ride along. Usage:
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "k=40 kNN queries/s + Taubin point-iters/s; learned-proposal vs plain, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=4_000_000)
+    p.add_argument("--k", type=int, default=40)
+    p.add_argument("--iters", type=int, default=300)
+    p.add_argument("--no-normals", action="store_true")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample", type=int, default=250_000)
+    return p.parse_args()
+
+
+def gpu_step(E, torch, dev_xyz, args, stream_sync):
+    g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+    g.GRIDCUDAKNNSELF_COMPACT(args.k, 1e8)
+    E.EGT_CUDA(g).PerformSmoothing(args.iters, 0.5, -0.53, 0.65)
+    if not args.no_normals:
+        E.Normal_CUDA(g).GetNormals()
+    st = g.stats()
+    g.close()
+    return st
+
+
+def cpu_baseline(args):
+    from multigpuegtsmoothing_amd import datasets as D
+    from oracle import oracle as O
+
+    n = args.cpu_sample
+    pts = D.torus(n, seed=2)
+    t0 = time.perf_counter()
+    O.pipeline(*pts, args.k, args.iters, 0.5, -0.53, 0.65, normals=not args.no_normals)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "points/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"oracle (host C++/OpenMP restatement) on a {n}-point torus from the same generator, "
+                      f"same k/alpha/{args.iters} iterations/normals; {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (no multi-GPU path)"}))
+        sys.exit(2)
+    import torch
+
+    from multigpuegtsmoothing_amd import datasets as D
+    from multigpuegtsmoothing_amd import egt as E
+
+    torch.cuda.set_device(0)
+    x, y, z = D.torus(args.n, seed=2)
+    dev = tuple(torch.from_numpy(a).cuda() for a in (x, y, z))
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        gpu_step(E, torch, dev, args, None)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(gpu_step(E, torch, dev, args, None))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    def med(key):
+        return float(np.median([s[key] for s in stats]))
+
+    knn_ms, egg_ms, taub_ms, nrm_ms, grid_ms = (med(k) for k in ("knn_ms", "egg_ms", "taubin_ms", "normals_ms",
+                                                                 "grid_ms"))
+    n, k = args.n, args.k
+    nnz = stats[-1]["egg_kept"]
+    passes = 2 * args.iters
+    # Taubin pass, algorithmic bytes per launch: own xyz read 12 + write 12 + row ptr 4 per point, 4 B per
+    # kept neighbour index
+    taub_bytes = 28 * n + 4 * nnz
+    taub_pass_ms = taub_ms / passes if passes else float("nan")
+    # kNN, algorithmic bytes: query xyz 12 + 4k output per query
+    knn_bytes = n * (12 + 4 * k)
+    stage = {"grid": grid_ms, "knn": knn_ms, "egg_csr": egg_ms, "taubin": taub_ms, "normals": nrm_ms}
+    dominant = max(stage, key=stage.get)
+    if dominant == "taubin":
+        ach = taub_bytes / (taub_pass_ms * 1e-3) / 1e9
+        roof = {"kernel": "k_taubin_pass", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": taub_bytes,
+                "avg_launch_ms": taub_pass_ms}
+    else:
+        ach = knn_bytes / (knn_ms * 1e-3) / 1e9
+        roof = {"kernel": "k_knn", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": knn_bytes, "avg_launch_ms": knn_ms}
+    out = {
+        "metric": METRIC,
+        "value": n / dt,
+        "unit": "points/s (end-to-end grid+kNN+EGG+Taubin+normals)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"torus {n} pts (R=1, r=0.3, density 1-10x, sigma 0.003, seed 2), cloud mode, "
+                               f"k={k}, alpha=0.65, {args.iters} Taubin its (lambda 0.5, mu -0.53)"
+                               + ("" if args.no_normals else ", PCA normals + MST orientation"),
+                   "n_points": n, "k": k, "iterations": args.iters, "path": "plain"},
+        "knn_queries_per_s": n / (knn_ms * 1e-3),
+        "taubin_point_iters_per_s": n * args.iters / (taub_ms * 1e-3) if taub_ms > 0 else None,
+        "stage_ms": stage,
+        "egg_kept_per_point": nnz / n,
+        "knn_candidates_per_query": stats[-1]["knn_candidates"] / n,
+        "invalid_rows": stats[-1]["invalid_rows"],
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -109,7 +109,9 @@ egt_status egt_grid_knn_self(egt_handle h, int k, float thres) {
         // GridKNNCuda.cu:2324-2325
         float t = thres / (h->max_extent - h->min_extent);
         t *= t;
+        if (h->search_stale) egt::refresh_search_structure(h);
         egt::run_knn(h, k, t);
+        h->csr_valid = false;
         h->smoothed = false;
     });
 }
@@ -143,6 +145,7 @@ egt_status egt_mesh_neighbors(egt_handle h, int k) {
     if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
     return guarded([&] {
         egt::run_mesh_adjacency(h, k);
+        h->csr_valid = false;
         h->smoothed = false;
     });
 }

@@ -1,12 +1,14 @@
-// Elliptic Gabriel graph filter (in place on the neighbour rows) and the CSR adjacency it feeds to Taubin.
+// Elliptic Gabriel graph filter (in place on the sorted-space neighbour rows) and the Morton-order CSR
+// adjacency it feeds to Taubin and the normals.
 //
 // Reference: calculateEGG_GKNN EGT_CUDA.cu:172-189, isEllipicGabrielNeighbor_GKNN :136-170, frame helpers
 // :55-134, batch driver :505-530.
 //
-// One thread per point; the point's neighbour coordinates are staged in LDS as [slot][lane] so every slot
-// read is bank-conflict free. Slot j (ranked by distance) is dropped when a valid slot m < j lies strictly
-// inside the ellipsoid with semi-axis |pq_j|/2 along pq_j and alpha*|pq_j|/2 across it. Tests for slot j only
-// read the original rows of slots m < j, so the result equals the reference's in-place downward sweep.
+// One thread per point, threads walking the points in Morton order so the neighbour coordinates a wave
+// gathers sit in a few cache lines. The point's neighbour coordinates are staged in LDS as [slot][lane]
+// (bank-conflict free). Slot j (ranked by distance) is dropped when a valid slot m < j lies strictly inside
+// the ellipsoid with semi-axis |pq_j|/2 along pq_j and alpha*|pq_j|/2 across it. Tests for slot j only read
+// the original entries of slots m < j, so the result equals the reference's in-place downward sweep.
 #include "egt_device.h"
 #include "egt_internal.h"
 
@@ -25,26 +27,36 @@ __device__ inline float rowt(float a1, float a2, float a3, float t, float x, flo
     return __builtin_fmaf(a3, z, __builtin_fmaf(a2, y, a1 * x)) + t;
 }
 
+// mpts: current points in Morton order (w = sorted index). knn: sorted-space rows, updated in place.
+// mrow: per Morton position, the k slots mapped to Morton positions (-1 for empty). keepm: kept-slot mask.
 template <int K>
-__global__ void __launch_bounds__(EBLK) k_egg(const float* __restrict__ xs, const float* __restrict__ ys,
-                                               const float* __restrict__ zs, int32_t* __restrict__ knn, int64_t n,
-                                               int k, float alpha, uint32_t* __restrict__ kept) {
+__global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, const int32_t* __restrict__ minv,
+                                               int32_t* __restrict__ knn, int64_t n, int k, float alpha,
+                                               int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
+                                               uint32_t* __restrict__ kept) {
     __shared__ float sx[K][EBLK], sy[K][EBLK], sz[K][EBLK];
     const int tid = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * EBLK + tid;
-    if (i >= n) return;
-    int32_t* row = knn + i * k;
+    const int64_t m = (int64_t)blockIdx.x * EBLK + tid;
+    if (m >= n) return;
+    const float4 p = mpts[m];
+    const int32_t s = __float_as_int(p.w);
+    int32_t* row = knn + (int64_t)s * k;
+    int32_t* mr = mrow + m * k;
     uint64_t valid = 0;
     for (int j = 0; j < k; ++j) {
         const int32_t nb = row[j];
+        int32_t nm = -1;
         if (nb >= 0) {
             valid |= 1ull << j;
-            sx[j][tid] = xs[nb];
-            sy[j][tid] = ys[nb];
-            sz[j][tid] = zs[nb];
+            nm = minv[nb];
+            const float4 c = mpts[nm];
+            sx[j][tid] = c.x;
+            sy[j][tid] = c.y;
+            sz[j][tid] = c.z;
         }
+        mr[j] = nm;
     }
-    const float x = xs[i], y = ys[i], z = zs[i];
+    const float x = p.x, y = p.y, z = p.z;
     const float aa = alpha * alpha;
     uint64_t keep = valid;
     for (int j = k - 1; j >= 0; --j) {
@@ -96,30 +108,58 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float* __restrict__ xs, cons
         dropped &= dropped - 1ull;
         row[j] = -1;
     }
-    kept[i] = (uint32_t)__popcll(keep);
+    keepm[m] = keep;
+    kept[m] = (uint32_t)__popcll(keep);
 }
 
-__global__ void __launch_bounds__(256) k_row_counts(const int32_t* __restrict__ knn, int64_t n, int k,
-                                                    uint32_t* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint32_t c = 0;
-    for (int j = 0; j < k; ++j) c += knn[i * k + j] >= 0;
-    cnt[i] = c;
-}
-
-__global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ knn, int64_t n, int k,
-                                                  const uint32_t* __restrict__ off, int32_t* __restrict__ rowptr,
-                                                  int32_t* __restrict__ col, uint32_t total) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint32_t o = off[i];
-    rowptr[i] = (int32_t)o;
-    if (i == n - 1) rowptr[n] = (int32_t)total;
-    for (int j = 0; j < k; ++j) {
-        const int32_t v = knn[i * k + j];
-        if (v >= 0) col[o++] = v;
+__global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mrow, const uint64_t* __restrict__ keepm,
+                                                  int64_t n, int k, const uint32_t* __restrict__ off,
+                                                  int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                                                  uint32_t total) {
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= n) return;
+    uint32_t o = off[m];
+    rowptr[m] = (int32_t)o;
+    if (m == n - 1) rowptr[n] = (int32_t)total;
+    uint64_t keep = keepm[m];
+    const int32_t* mr = mrow + m * k;
+    while (keep) {
+        const int j = __builtin_ctzll(keep);
+        keep &= keep - 1ull;
+        col[o++] = mr[j];
     }
+}
+
+// Rows -> Morton-order slots without filtering (normals requested straight after the neighbour search).
+__global__ void __launch_bounds__(256) k_rows_to_mrow(const int32_t* __restrict__ morder,
+                                                      const int32_t* __restrict__ minv,
+                                                      const int32_t* __restrict__ knn, int64_t n, int k,
+                                                      int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
+                                                      uint32_t* __restrict__ kept) {
+    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (m >= n) return;
+    const int32_t* row = knn + (int64_t)morder[m] * k;
+    uint64_t valid = 0;
+    for (int j = 0; j < k; ++j) {
+        const int32_t nb = row[j];
+        mrow[m * k + j] = nb >= 0 ? minv[nb] : -1;
+        if (nb >= 0) valid |= 1ull << j;
+    }
+    keepm[m] = valid;
+    kept[m] = (uint32_t)__popcll(valid);
+}
+
+void finish_csr(egt_ctx* h, const int32_t* mrow, const uint64_t* keepm, const uint32_t* kept) {
+    const int64_t n = h->n;
+    uint32_t* off = h->scratch.get<uint32_t>(10, n);
+    uint32_t total = 0;
+    exclusive_scan_u32(kept, off, n, h->stream, h->scratch, &total);
+    h->rowptr.alloc(n + 1);
+    h->col.alloc(std::max<uint32_t>(total, 1));
+    k_csr_fill<<<ceil_div(n, 256), 256, 0, h->stream>>>(mrow, keepm, n, h->k, off, h->rowptr.p, h->col.p, total);
+    LAUNCH_CHECK();
+    h->nnz = total;
+    h->csr_valid = true;
 }
 
 }  // namespace
@@ -127,19 +167,13 @@ __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ kn
 void build_csr_from_rows(egt_ctx* h) {
     const int64_t n = h->n;
     const int k = h->k;
-    hipStream_t s = h->stream;
-    uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
-    uint32_t* off = h->scratch.get<uint32_t>(10, n);
-    k_row_counts<<<ceil_div(n, 256), 256, 0, s>>>(h->knn.p, n, k, cnt);
+    uint32_t* kept = h->scratch.get<uint32_t>(9, n);
+    uint64_t* keepm = h->scratch.get<uint64_t>(17, n);
+    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)n * k);
+    k_rows_to_mrow<<<ceil_div(n, 256), 256, 0, h->stream>>>(h->morder.p, h->minv.p, h->knn.p, n, k, mrow, keepm,
+                                                            kept);
     LAUNCH_CHECK();
-    uint32_t total = 0;
-    exclusive_scan_u32(cnt, off, n, s, h->scratch, &total);
-    h->rowptr.alloc(n + 1);
-    h->col.alloc(std::max<uint32_t>(total, 1));
-    k_csr_fill<<<ceil_div(n, 256), 256, 0, s>>>(h->knn.p, n, k, off, h->rowptr.p, h->col.p, total);
-    LAUNCH_CHECK();
-    h->nnz = total;
-    h->stats.egg_kept = total;
+    finish_csr(h, mrow, keepm, kept);
 }
 
 void run_egg(egt_ctx* h, float alpha) {
@@ -147,18 +181,23 @@ void run_egg(egt_ctx* h, float alpha) {
     const int k = h->k;
     hipStream_t s = h->stream;
     uint32_t* kept = h->scratch.get<uint32_t>(9, n);
+    uint64_t* keepm = h->scratch.get<uint64_t>(17, n);
+    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)n * k);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
     const dim3 grid(ceil_div(n, EBLK)), blk(EBLK);
-    if (k <= 16) hipLaunchKernelGGL(k_egg<16>, grid, blk, 0, s, h->x.p, h->y.p, h->z.p, h->knn.p, n, k, alpha, kept);
-    else if (k <= 32) hipLaunchKernelGGL(k_egg<32>, grid, blk, 0, s, h->x.p, h->y.p, h->z.p, h->knn.p, n, k, alpha, kept);
-    else if (k <= 40) hipLaunchKernelGGL(k_egg<40>, grid, blk, 0, s, h->x.p, h->y.p, h->z.p, h->knn.p, n, k, alpha, kept);
-    else hipLaunchKernelGGL(k_egg<64>, grid, blk, 0, s, h->x.p, h->y.p, h->z.p, h->knn.p, n, k, alpha, kept);
+#define EGG_ARGS h->mpts.p, h->minv.p, h->knn.p, n, k, alpha, mrow, keepm, kept
+    if (k <= 16) hipLaunchKernelGGL(k_egg<16>, grid, blk, 0, s, EGG_ARGS);
+    else if (k <= 32) hipLaunchKernelGGL(k_egg<32>, grid, blk, 0, s, EGG_ARGS);
+    else if (k <= 40) hipLaunchKernelGGL(k_egg<40>, grid, blk, 0, s, EGG_ARGS);
+    else hipLaunchKernelGGL(k_egg<64>, grid, blk, 0, s, EGG_ARGS);
+#undef EGG_ARGS
     LAUNCH_CHECK();
+    finish_csr(h, mrow, keepm, kept);
+    h->stats.egg_kept = h->nnz;
     HIP_CHECK(hipEventRecord(e1, s));
-    build_csr_from_rows(h);
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&h->stats.egg_ms, e0, e1));
     (void)hipEventDestroy(e0);

@@ -105,10 +105,13 @@ __global__ void __launch_bounds__(BLK) k_morton_keys(const int32_t* __restrict__
 __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ z, const int32_t* __restrict__ key,
                                                         const uint32_t* __restrict__ order, int64_t n, int log2B,
-                                                        float4* __restrict__ mpts, uint32_t* __restrict__ mcell) {
+                                                        float4* __restrict__ mpts, uint32_t* __restrict__ mcell,
+                                                        int32_t* __restrict__ morder, int32_t* __restrict__ minv) {
     const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (m >= n) return;
     const uint32_t s = order[m];
+    morder[m] = (int32_t)s;
+    minv[s] = (int32_t)m;
     mpts[m] = make_float4(x[s], y[s], z[s], __int_as_float((int)s));
     const uint32_t k = (uint32_t)key[s], msk = (1u << log2B) - 1u;
     mcell[m] = (k & msk) | (((k >> log2B) & msk) << 10) | ((k >> (2 * log2B)) << 20);
@@ -267,8 +270,10 @@ void build_search_structure(egt_ctx* h) {
     h->mpts.alloc(n);
     h->mcell.alloc(n);
     h->mcode.alloc(n);
+    h->morder.alloc(n);
+    h->minv.alloc(n);
     k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, order.p, n, log2B, h->mpts.p,
-                                                      h->mcell.p);
+                                                      h->mcell.p, h->morder.p, h->minv.p);
     LAUNCH_CHECK();
     HIP_CHECK(hipMemcpyAsync(h->mcode.p, code.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
 
@@ -315,6 +320,29 @@ void build_search_structure(egt_ctx* h) {
     HIP_CHECK(hipMemcpyAsync(mx.data(), dmax.p, h->nlev * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     for (int l = 0; l < h->nlev; ++l) h->lev[l].max_count = mx[l];
+}
+
+namespace {
+__global__ void __launch_bounds__(BLK) k_rekey(const float* __restrict__ x, const float* __restrict__ y,
+                                                const float* __restrict__ z, int64_t n, float a, float ba, int B,
+                                                int log2B, int32_t* __restrict__ key) {
+    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t cx = (uint32_t)cell_of(x[s], a, ba, B), cy = (uint32_t)cell_of(y[s], a, ba, B),
+                   cz = (uint32_t)cell_of(z[s], a, ba, B);
+    key[s] = (int32_t)(cx | (cy << log2B) | (cz << (2 * log2B)));
+}
+}  // namespace
+
+// After smoothing moved the points: recompute each point's fine cell and rebuild the Morton search
+// structure (the sorted-space order itself is kept).
+void refresh_search_structure(egt_ctx* h) {
+    const float a = -h->eps, ba = (1.0f + h->eps) - a;
+    k_rekey<<<ceil_div(h->n, BLK), BLK, 0, h->stream>>>(h->x.p, h->y.p, h->z.p, h->n, a, ba, h->B, ilog2(h->B),
+                                                        h->key.p);
+    LAUNCH_CHECK();
+    build_search_structure(h);
+    h->search_stale = false;
 }
 
 void run_output(egt_ctx* h, float* x, float* y, float* z) {

@@ -98,14 +98,18 @@ struct egt_ctx {
     egt::DBuf<float4> mpts;      // x, y, z, bits(sorted index)
     egt::DBuf<uint32_t> mcell;   // packed fine cell coords cx | cy<<10 | cz<<20
     egt::DBuf<uint32_t> mcode;   // fine Morton code
+    egt::DBuf<int32_t> morder;   // Morton position -> sorted index
+    egt::DBuf<int32_t> minv;     // sorted index -> Morton position
+    bool search_stale = false;   // points moved since the search structure was built
     egt::DBuf<uint32_t> hkeys;   // all level tables
     egt::DBuf<uint2> hvals;
     egt::LevelTable lev[EGT_MAX_LEVELS];
     int nlev = 0;
 
-    // EGG CSR
+    // EGG adjacency as CSR in Morton order (rows and column indices are Morton positions)
     egt::DBuf<int32_t> rowptr, col;
     int64_t nnz = 0;
+    bool csr_valid = false;  // CSR matches the current neighbour rows
 
     // mesh
     egt::DBuf<uint32_t> tri;
@@ -134,6 +138,7 @@ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
 // grid (egt_grid.hip)
 void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz);
 void build_search_structure(egt_ctx* h);
+void refresh_search_structure(egt_ctx* h);
 
 // kNN (egt_knn.hip)
 void run_knn(egt_ctx* h, int k, float thres_n);

@@ -25,30 +25,30 @@ namespace {
 constexpr int NBLK = 256;
 constexpr uint64_t KEY_EMPTY = ~0ull;
 
-__global__ void __launch_bounds__(NBLK) k_pca(const float* __restrict__ x, const float* __restrict__ y,
-                                               const float* __restrict__ z, const int32_t* __restrict__ knn, int64_t n,
-                                               int k, float* __restrict__ nx, float* __restrict__ ny,
-                                               float* __restrict__ nz) {
-    const int64_t i = (int64_t)blockIdx.x * NBLK + threadIdx.x;
-    if (i >= n) return;
-    const int32_t* row = knn + i * k;
-    float sx = x[i], sy = y[i], sz = z[i];
+// One thread per Morton position; neighbours from the Morton-order CSR (slot order of the rows), points
+// from the Morton float4 copy. Writes the normal of sorted point s = mpts[m].w.
+__global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ col, int64_t n, float* __restrict__ nx,
+                                               float* __restrict__ ny, float* __restrict__ nz) {
+    const int64_t m = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (m >= n) return;
+    const float4 p = mpts[m];
+    const int32_t i = __float_as_int(p.w);
+    const int32_t e0 = rowptr[m], e1 = rowptr[m + 1];
+    float sx = p.x, sy = p.y, sz = p.z;
     int counter = 1;
-    for (int j = 0; j < k; ++j) {
-        const int32_t nb = row[j];
-        if (nb >= 0) {
-            sx += x[nb];
-            sy += y[nb];
-            sz += z[nb];
-            ++counter;
-        }
+    for (int32_t e = e0; e < e1; ++e) {
+        const float4 q = mpts[col[e]];
+        sx += q.x;
+        sy += q.y;
+        sz += q.z;
+        ++counter;
     }
     const float cx = sx / (float)counter, cy = sy / (float)counter, cz = sz / (float)counter;
     float a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
-    for (int j = 0; j < k; ++j) {
-        const int32_t nb = row[j];
-        if (nb < 0) continue;
-        const float dx = x[nb] - cx, dy = y[nb] - cy, dz = z[nb] - cz;
+    for (int32_t e = e0; e < e1; ++e) {
+        const float4 q = mpts[col[e]];
+        const float dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
         a00 = __builtin_fmaf(dx, dx, a00);
         a01 = __builtin_fmaf(dx, dy, a01);
         a02 = __builtin_fmaf(dx, dz, a02);
@@ -141,18 +141,41 @@ __global__ void __launch_bounds__(NBLK) k_bv_reset(int64_t n, uint64_t* __restri
     H[v] = (uint32_t)v << 1;
 }
 
+// 64-bit atomicMin on base[idx]; when every active lane of the wave targets the same address (one giant
+// component late in Boruvka) the wave reduces first and issues a single atomic. Call from all lanes.
+__device__ inline void wave_atomic_min(uint64_t* base, int32_t idx, uint64_t val, bool active) {
+    const uint64_t amask = __ballot(active);
+    if (!amask) return;
+    const int leader = __ffsll((unsigned long long)amask) - 1;
+    const int32_t lidx = __shfl(idx, leader, 64);
+    if (__ballot(active && idx != lidx) == 0) {
+        unsigned long long v = active ? (unsigned long long)val : ~0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(v, d, 64);
+            v = o < v ? o : v;
+        }
+        if (__lane_id() == leader) atomicMin((unsigned long long*)&base[lidx], v);
+    } else if (active) {
+        atomicMin((unsigned long long*)&base[idx], (unsigned long long)val);
+    }
+}
+
 __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
                                                    const uint64_t* __restrict__ ekey, uint32_t m,
-                                                   const int32_t* __restrict__ comp, uint64_t* __restrict__ best,
-                                                   uint32_t* __restrict__ live) {
+                                                   const int32_t* __restrict__ comp, uint64_t* __restrict__ best) {
     const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
-    if (e >= m) return;
-    const int32_t cu = comp[eu[e]], cv = comp[ev[e]];
-    if (cu == cv) return;
-    const uint64_t key = ekey[e];
-    atomicMin((unsigned long long*)&best[cu], (unsigned long long)key);
-    atomicMin((unsigned long long*)&best[cv], (unsigned long long)key);
-    if (live) *live = 1u;
+    int32_t cu = 0, cv = 0;
+    uint64_t key = 0;
+    bool act = e < m;
+    if (act) {
+        cu = comp[eu[e]];
+        cv = comp[ev[e]];
+        act = cu != cv;
+        key = ekey[e];
+    }
+    wave_atomic_min(best, cu, key, act);
+    wave_atomic_min(best, cv, key, act);
 }
 
 __global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __restrict__ eu,
@@ -209,8 +232,8 @@ __global__ void __launch_bounds__(NBLK) k_tree_min(const uint8_t* __restrict__ m
                                                     const int32_t* __restrict__ eu, uint32_t m,
                                                     const int32_t* __restrict__ comp, uint64_t* __restrict__ tmin) {
     const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
-    if (e >= m || !msf[e]) return;
-    atomicMin((unsigned long long*)&tmin[comp[eu[e]]], (unsigned long long)ekey[e]);
+    const bool act = e < m && msf[e];
+    wave_atomic_min(tmin, act ? comp[eu[e]] : 0, act ? ekey[e] : 0ull, act);
 }
 
 __global__ void __launch_bounds__(NBLK) k_orient(int64_t n, const int32_t* __restrict__ comp,
@@ -299,7 +322,8 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
 
     float* nrm = h->scratch.get<float>(8, 6 * (size_t)n);
     float *nx = nrm, *ny = nrm + n, *nz = nrm + 2 * n;
-    k_pca<<<gn, blk, 0, s>>>(h->x.p, h->y.p, h->z.p, h->knn.p, n, k, nx, ny, nz);
+    if (!h->csr_valid) build_csr_from_rows(h);
+    k_pca<<<gn, blk, 0, s>>>(h->mpts.p, h->rowptr.p, h->col.p, n, nx, ny, nz);
     LAUNCH_CHECK();
 
     // canonical undirected edge list
@@ -348,7 +372,7 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     for (int round = 0; round < 64 && m > 0; ++round) {
         k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
         HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
-        k_bv_best<<<gm, blk, 0, s>>>(eu, ev, ekey, m, comp, best, nullptr);
+        k_bv_best<<<gm, blk, 0, s>>>(eu, ev, ekey, m, comp, best);
         k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf, flags);
         k_bv_jump<<<gn, blk, 0, s>>>(n, comp, H, flags + 1);
         k_bv_relabel<<<gn, blk, 0, s>>>(n, comp, par, H);

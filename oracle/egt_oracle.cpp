@@ -172,6 +172,8 @@ extern "C" {
 
 int orc_version() { return 1; }
 
+int orc_num_threads() { return omp_get_max_threads(); }
+
 /* egt_expf: exp(x) for the Taubin weights. Cody-Waite argument reduction x = n*ln2 + r with ln2 split
  * into a 24-bit head and a tail, |r| <= ln2/2, degree-7 Taylor polynomial of exp(r) in Horner form, and
  * exact scaling by 2^n. */

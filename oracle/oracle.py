@@ -38,6 +38,7 @@ def lib():
         L.orc_pca_normals.argtypes = [F32P, F32P, F32P, I32P, i64, i32, F32P, F32P, F32P]
         L.orc_orient_normals.argtypes = [I32P, i64, i32, F32P, F32P, F32P, I64P,
                                          ctypes.POINTER(ctypes.c_double)]
+        L.orc_num_threads.restype = ctypes.c_int
         L.egt_expf.argtypes = [f32]
         L.egt_expf.restype = f32
         L.orc_eig3.argtypes = [F32P, F32P, F32P]
@@ -152,6 +153,10 @@ def orient_normals(knn, nx, ny, nz):
     w = ctypes.c_double(0.0)
     lib().orc_orient_normals(np.ascontiguousarray(knn, np.int32).ravel(), n, k, nx, ny, nz, stats, ctypes.byref(w))
     return nx, ny, nz, stats, w.value
+
+
+def num_threads():
+    return int(lib().orc_num_threads())
 
 
 def expf(x):
