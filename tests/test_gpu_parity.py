@@ -26,7 +26,7 @@ def _cloud(E, pts, k, bucket=1024, maxc=None, start_level=None, learned=None):
     return g
 
 
-@pytest.mark.parametrize("case", ["sphere20k_k40", "torus30k_k16", "sphere5k_k64", "lattice_k26", "dups_k8"])
+@pytest.mark.parametrize("case", ["sphere20k_k40", "torus30k_k16", "sphere5k_k64", "lattice_k18", "dups_k8"])
 def test_knn_rows_bit_exact(egt_mod, oracle_mod, case):
     E, O = egt_mod, oracle_mod
     if case == "sphere20k_k40":
@@ -35,8 +35,8 @@ def test_knn_rows_bit_exact(egt_mod, oracle_mod, case):
         pts, k = D.torus(30_000, seed=12), 16
     elif case == "sphere5k_k64":
         pts, k = D.sphere(5_000, seed=13), 64
-    elif case == "lattice_k26":
-        pts, k = D.tiny_lattice(8), 26
+    elif case == "lattice_k18":
+        pts, k = D.tiny_lattice(12), 18
     else:
         x, y, z = D.sphere(2_000, seed=14)
         pts, k = (np.concatenate([x, x[:300]]), np.concatenate([y, y[:300]]), np.concatenate([z, z[:300]])), 8

@@ -1,0 +1,90 @@
+"""Pins the oracle's PCA normals and MST orientation (reference Normal_CUDA.cu, EigenSolver.cuh,
+CUDA_MST.cu, bfs.cu) against numpy / scipy."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components, minimum_spanning_tree
+
+from multigpuegtsmoothing_amd import datasets as D
+
+
+def test_eig3_matches_numpy(oracle_mod):
+    rng = np.random.default_rng(41)
+    for _ in range(400):
+        A = rng.standard_normal((3, 3))
+        A = (A @ A.T).astype(np.float32)
+        a6 = np.float32([A[0, 0], A[0, 1], A[0, 2], A[1, 1], A[1, 2], A[2, 2]])
+        idx, ev, V = oracle_mod.eig3(a6)
+        w, U = np.linalg.eigh(A.astype(np.float64))
+        np.testing.assert_allclose(np.sort(ev), w, atol=2e-5 * np.abs(w).max())
+        assert abs(ev[idx] - w[0]) <= 2e-5 * np.abs(w).max()
+        if w[1] - w[0] > 1e-2 * np.abs(w).max():
+            v = V[idx] / np.linalg.norm(V[idx])
+            assert abs(abs(v @ U[:, 0]) - 1.0) < 1e-4
+
+
+def test_zero_matrix_gives_x_axis(oracle_mod):
+    idx, ev, V = oracle_mod.eig3(np.zeros(6, np.float32))
+    assert idx == 0 and np.allclose(V[0], [1, 0, 0])
+
+
+def test_plane_normals(oracle_mod):
+    rng = np.random.default_rng(42)
+    n = 4000
+    P = np.stack([rng.uniform(0, 1, n), rng.uniform(0, 1, n), 0.3 + 1e-4 * rng.standard_normal(n)], 1)
+    g = oracle_mod.Grid(*(P.astype(np.float32).T))
+    knn, _, _, _ = g.knn(20)
+    nx, ny, nz = oracle_mod.pca_normals(g.xs, g.ys, g.zs, knn)
+    assert np.mean(np.abs(nz) > 0.99) > 0.99
+
+
+def test_sphere_normals_radial_and_oriented(oracle_mod):
+    res = oracle_mod.pipeline(*D.sphere(20_000, seed=43), 24, 5, 0.5, -0.53, 0.65, normals=True)
+    g = res["grid"]
+    sx, sy, sz = res["smoothed_sorted"]
+    nx, ny, nz = res["normals_sorted"]
+    c = np.array([sx.mean(), sy.mean(), sz.mean()])
+    r = np.stack([sx - c[0], sy - c[1], sz - c[2]], 1)
+    r /= np.linalg.norm(r, axis=1, keepdims=True)
+    dot = r[:, 0] * nx + r[:, 1] * ny + r[:, 2] * nz
+    assert np.mean(np.abs(dot) > 0.95) > 0.99
+    # one component: every normal on the same side (all outward or all inward)
+    frac = np.mean(dot > 0)
+    assert frac > 0.999 or frac < 0.001
+    assert res["orient_stats"][2] == 1
+
+
+def test_msf_weight_matches_scipy(oracle_mod):
+    res = oracle_mod.pipeline(*D.torus(8_000, seed=44), 16, 3, 0.5, -0.53, 0.65, normals=True)
+    rows = res["egg"]
+    nx, ny, nz = res["pca_sorted"]
+    n = rows.shape[0]
+    i = np.repeat(np.arange(n), rows.shape[1])
+    j = rows.ravel()
+    ok = (j >= 0) & (j != i)
+    uv = np.unique(np.minimum(i[ok], j[ok]).astype(np.int64) * n + np.maximum(i[ok], j[ok]))
+    u, v = uv // n, uv % n
+    dot = np.abs(nx[u].astype(np.float64) * nx[v] + ny[u].astype(np.float64) * ny[v] + nz[u].astype(np.float64) * nz[v])
+    w = np.maximum(0.0, 1.0 - dot) + 1e-9  # scipy drops explicit zeros
+    G = sp.coo_matrix((w, (u, v)), shape=(n, n)).tocsr()
+    T = minimum_spanning_tree(G)
+    ncomp, _ = connected_components(G, directed=False)
+    st = res["orient_stats"]
+    assert st[0] == len(u)
+    assert st[1] == T.nnz == n - ncomp
+    assert abs((T.sum() - 1e-9 * T.nnz) - res["msf_weight"]) <= 1e-4 * max(1.0, res["msf_weight"])
+
+
+def test_orientation_consistent_on_two_spheres(oracle_mod):
+    a = np.stack(D.sphere(6_000, seed=45), 1)
+    b = np.stack(D.sphere(6_000, seed=46), 1) * 0.5 + np.float32([4.0, 0.0, 0.0])
+    p = np.concatenate([a, b]).astype(np.float32)
+    res = oracle_mod.pipeline(p[:, 0], p[:, 1], p[:, 2], 16, 3, 0.5, -0.53, 0.65, normals=True)
+    assert res["orient_stats"][2] == 2
+    nx, ny, nz = res["normals"]
+    for sl, c in [(slice(0, 6000), np.zeros(3)), (slice(6000, 12000), np.float32([4.0, 0, 0]))]:
+        r = p[sl] - c
+        r /= np.linalg.norm(r, axis=1, keepdims=True)
+        dot = r[:, 0] * nx[sl] + r[:, 1] * ny[sl] + r[:, 2] * nz[sl]
+        frac = np.mean(dot > 0)
+        assert frac > 0.999 or frac < 0.001
