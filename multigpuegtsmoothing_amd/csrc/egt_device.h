@@ -58,6 +58,25 @@ __device__ inline float egt_expf(float x) {
     return p * __uint_as_float((uint32_t)(ni + 127) << 23);
 }
 
+// egt_expf for the Taubin weight argument x = -|q - p|^2 <= 0: on [-80, 0] the range checks of egt_expf
+// never fire and 2^n is a normal number, so the same reduction and polynomial followed by an exact ldexp
+// give identical results; other arguments go through egt_expf.
+__device__ inline float egt_expf_taubin(float x) {
+    if (!(x >= -80.0f)) return egt_expf(x);
+    const float n = __builtin_rintf(x * 1.44269502163f);
+    float r = __builtin_fmaf(-n, 0.693145751953125f, x);
+    r = __builtin_fmaf(-n, 1.428606765330187e-06f, r);
+    float p = 1.98412698e-4f;
+    p = __builtin_fmaf(p, r, 1.38888889e-3f);
+    p = __builtin_fmaf(p, r, 8.33333333e-3f);
+    p = __builtin_fmaf(p, r, 4.16666667e-2f);
+    p = __builtin_fmaf(p, r, 1.66666667e-1f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return __builtin_ldexpf(p, (int)n);
+}
+
 // Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5); returns the
 // index of the smallest eigenvalue and its eigenvector (vx, vy, vz).
 __device__ inline void eig3_smallest(float a00, float a01, float a02, float a11, float a12, float a22, float& vx,

@@ -110,6 +110,7 @@ struct egt_ctx {
     egt::DBuf<int32_t> rowptr, col;
     int64_t nnz = 0;
     bool csr_valid = false;  // CSR matches the current neighbour rows
+    int64_t taubin_fallback_tiles = 0;
 
     // mesh
     egt::DBuf<uint32_t> tri;

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(TBLK) k_taubin_pass(const float4* __restrict__
         for (int u = 0; u < CH; ++u) {
             if (c[u] >= 0) {
                 const float dist = sqd(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z);
-                const float w = egt_expf(-dist);
+                const float w = egt_expf_taubin(-dist);
                 cgx = __builtin_fmaf(w, q[u].x - p.x, cgx);
                 cgy = __builtin_fmaf(w, q[u].y - p.y, cgy);
                 cgz = __builtin_fmaf(w, q[u].z - p.z, cgz);
@@ -59,6 +59,176 @@ __global__ void __launch_bounds__(TBLK) k_taubin_pass(const float4* __restrict__
     }
     if (sum == 0.0f) sum = 1e-8f;
     out[i] = make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// LDS-tiled pass. A tile is TILE consecutive Morton positions; its 1-hop halo (neighbours outside the tile,
+// deduplicated and sorted) is listed once per smoothing call, and every CSR column is rewritten as a 16-bit
+// local index (< TILE: inside the tile; TILE + h: halo entry h). A pass then stages tile + halo coordinates
+// in LDS and reads all neighbours from LDS. Tiles whose halo exceeds HALO_MAX use global gathers.
+constexpr int TILE = 1024;
+constexpr int HALO_MAX = 1024;
+constexpr int PBLK = 256;          // tile preparation
+constexpr int TBLK_TILE = TILE;    // tiled pass: one thread per point
+constexpr int SORT_CAP = 8192;
+
+__device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += PBLK) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                     int64_t n, uint16_t* __restrict__ col16,
+                                                     int32_t* __restrict__ halo, int32_t* __restrict__ hcount) {
+    __shared__ uint32_t raw[SORT_CAP];
+    __shared__ uint32_t uniq[SORT_CAP];
+    __shared__ uint32_t cnt, wsum[PBLK / 64];
+    const int64_t t = blockIdx.x;
+    const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
+    const int32_t eb = rowptr[base], ee = rowptr[end];
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
+        const int32_t c = col[e];
+        if (c < base || c >= end) {
+            const uint32_t p = atomicAdd(&cnt, 1u);
+            if (p < SORT_CAP) raw[p] = (uint32_t)c;
+        }
+    }
+    __syncthreads();
+    const uint32_t total = cnt;
+    if (total > SORT_CAP) {
+        if (threadIdx.x == 0) hcount[t] = -1;
+        return;
+    }
+    int n2 = 1;
+    while (n2 < (int)total) n2 <<= 1;
+    for (int i = total + threadIdx.x; i < n2; i += PBLK) raw[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    block_bitonic_sort(raw, n2);
+    // unique: each thread owns a contiguous chunk
+    const int per = (n2 + PBLK - 1) / PBLK;
+    const int c0 = threadIdx.x * per, c1 = min(c0 + per, (int)total);
+    uint32_t mine = 0;
+    for (int i = c0; i < c1; ++i) mine += (i == 0 || raw[i] != raw[i - 1]);
+    uint32_t v = mine;
+    const int lane = __lane_id(), wid = threadIdx.x / 64;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    uint32_t off = v - mine, H = 0;
+    for (int w = 0; w < PBLK / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        H += wsum[w];
+    }
+    for (int i = c0; i < c1; ++i)
+        if (i == 0 || raw[i] != raw[i - 1]) uniq[off++] = raw[i];
+    __syncthreads();
+    if (H > HALO_MAX) {
+        if (threadIdx.x == 0) hcount[t] = -1;
+        return;
+    }
+    for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * HALO_MAX + h] = (int32_t)uniq[h];
+    for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
+        const uint32_t c = (uint32_t)col[e];
+        uint16_t loc;
+        if (c >= (uint64_t)base && c < (uint64_t)end) {
+            loc = (uint16_t)(c - base);
+        } else {
+            uint32_t lo = 0, hi = H;  // lower_bound in uniq
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (uniq[mid] < c) lo = mid + 1;
+                else hi = mid;
+            }
+            loc = (uint16_t)(TILE + lo);
+        }
+        col16[e] = loc;
+    }
+    if (threadIdx.x == 0) hcount[t] = (int32_t)H;
+}
+
+// Neighbour loop over LDS coordinates: CH local columns are fetched before use so a row costs about one
+// global latency; accumulation stays in slot order.
+__device__ inline void taubin_point(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ c16,
+                                    int32_t e0, int32_t e1, float& cgx, float& cgy, float& cgz, float& sum) {
+    constexpr int CH = 16;
+    for (int32_t e = e0; e < e1; e += CH) {
+        uint32_t c[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) c[u] = e + u < e1 ? (uint32_t)c16[e + u] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (c[u] != 0xFFFFFFFFu) {
+                const float4 q = src[c[u]];
+                const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
+                const float w = egt_expf_taubin(-dist);
+                cgx = __builtin_fmaf(w, q.x - p.x, cgx);
+                cgy = __builtin_fmaf(w, q.y - p.y, cgy);
+                cgz = __builtin_fmaf(w, q.z - p.z, cgz);
+                sum += w;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restrict__ in, float4* __restrict__ out,
+                                                       const int32_t* __restrict__ rowptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const uint16_t* __restrict__ col16,
+                                                       const int32_t* __restrict__ halo,
+                                                       const int32_t* __restrict__ hcount, int64_t n, float scale,
+                                                       int ntiles) {
+    extern __shared__ float4 lds[];
+    __shared__ int32_t srow[TILE + 1];
+    const int64_t t = xcd_block(ntiles);
+    if (t >= ntiles) return;
+    const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
+    const int32_t H = hcount[t];
+    for (int i = threadIdx.x; i <= end - base; i += TBLK_TILE) srow[i] = rowptr[base + i];
+    if (H >= 0) {
+        for (int i = threadIdx.x; i < end - base; i += TBLK_TILE) lds[i] = in[base + i];
+        for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * HALO_MAX + h]];
+    }
+    __syncthreads();
+    for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
+        const float4 p = H >= 0 ? lds[i - base] : in[i];
+        float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
+        const int32_t e0 = srow[i - base], e1 = srow[i - base + 1];
+        if (H >= 0) {
+            taubin_point(p, lds, col16, e0, e1, cgx, cgy, cgz, sum);
+        } else {
+            for (int32_t e = e0; e < e1; ++e) {
+                const float4 q = in[col[e]];
+                const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
+                const float w = egt_expf_taubin(-dist);
+                cgx = __builtin_fmaf(w, q.x - p.x, cgx);
+                cgy = __builtin_fmaf(w, q.y - p.y, cgy);
+                cgz = __builtin_fmaf(w, q.z - p.z, cgz);
+                sum += w;
+            }
+        }
+        if (sum == 0.0f) sum = 1e-8f;
+        out[i] = make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+    }
 }
 
 // Writes the smoothed points back to the sorted-space SoA arrays and refreshes the Morton copy.
@@ -82,21 +252,43 @@ void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     hipStream_t s = h->stream;
     float4* A = h->scratch.get<float4>(12, n);
     float4* Bf = h->scratch.get<float4>(13, n);
-    const int nb = ceil_div(n, TBLK);
-    const dim3 grid(ceil_div(nb, NXCD) * NXCD), blk(TBLK);
     HIP_CHECK(hipMemcpyAsync(A, h->mpts.p, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
+
+    // tile preparation (once per call): halo lists and 16-bit local columns
+    const int ntiles = ceil_div(n, TILE);
+    uint16_t* col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
+    int32_t* halo = h->scratch.get<int32_t>(20, (size_t)ntiles * HALO_MAX);
+    int32_t* hcount = h->scratch.get<int32_t>(21, ntiles);
+    k_tile_prep<<<ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, col16, halo, hcount);
+    LAUNCH_CHECK();
+    std::vector<int32_t> hc(ntiles);
+    HIP_CHECK(hipMemcpyAsync(hc.data(), hcount, ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    int maxH = 0;
+    int64_t fallback_tiles = 0;
+    for (int32_t v : hc) {
+        if (v < 0) ++fallback_tiles;
+        else maxH = std::max(maxH, v);
+    }
+    h->taubin_fallback_tiles = fallback_tiles;
+    const size_t lds_bytes = (size_t)(TILE + maxH) * sizeof(float4);
+    const dim3 tgrid(ceil_div(ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
+    auto pass = [&](const float4* in, float4* out, float scale) {
+        k_taubin_tile<<<tgrid, tblk, lds_bytes, s>>>(in, out, h->rowptr.p, h->col.p, col16, halo, hcount, n, scale,
+                                                     ntiles);
+    };
     if (iters > 0) {
         if (h->taubin_graph) {
             hipGraph_t g;
             hipGraphExec_t ge;
             HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int it = 0; it < iters; ++it) {
-                k_taubin_pass<<<grid, blk, 0, s>>>(A, Bf, h->rowptr.p, h->col.p, n, lambda, nb);
-                k_taubin_pass<<<grid, blk, 0, s>>>(Bf, A, h->rowptr.p, h->col.p, n, mu, nb);
+                pass(A, Bf, lambda);
+                pass(Bf, A, mu);
             }
             HIP_CHECK(hipStreamEndCapture(s, &g));
             HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -106,8 +298,8 @@ void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
             (void)hipGraphDestroy(g);
         } else {
             for (int it = 0; it < iters; ++it) {
-                k_taubin_pass<<<grid, blk, 0, s>>>(A, Bf, h->rowptr.p, h->col.p, n, lambda, nb);
-                k_taubin_pass<<<grid, blk, 0, s>>>(Bf, A, h->rowptr.p, h->col.p, n, mu, nb);
+                pass(A, Bf, lambda);
+                pass(Bf, A, mu);
             }
             LAUNCH_CHECK();
         }

@@ -99,6 +99,22 @@ def test_egg_and_taubin_bit_exact(egt_mod, oracle_mod):
     assert g.stats()["egg_kept"] == int((ref["egg"] >= 0).sum())
 
 
+def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
+    """A uniform 3-D cloud at k=64 gives tiles whose halo exceeds the LDS budget: exercises the
+    global-gather path of the tiled Taubin pass next to LDS tiles."""
+    E, O = egt_mod, oracle_mod
+    rng = np.random.default_rng(42)
+    p = rng.uniform(0, 1, (40_000, 3)).astype(np.float32)
+    pts = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy())
+    k, iters = 64, 4
+    g = _cloud(E, pts, k)
+    E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, 0.65)
+    out = E.getOutputPoints(g)
+    ref = O.pipeline(*pts, k, iters, 0.5, -0.53, 0.65, normals=False)
+    for a, b in zip(out, ref["output"]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_taubin_vs_fp64(egt_mod, oracle_mod):
     """C1 settings at 100k points: FP32 GPU coordinates within 1e-5 of the bbox diagonal of the FP64 run."""
     E, O = egt_mod, oracle_mod
