@@ -8,7 +8,8 @@
 // gathers sit in a few cache lines. The point's neighbour coordinates are staged in LDS as [slot][lane]
 // (bank-conflict free). Slot j (ranked by distance) is dropped when a valid slot m < j lies strictly inside
 // the ellipsoid with semi-axis |pq_j|/2 along pq_j and alpha*|pq_j|/2 across it. Tests for slot j only read
-// the original entries of slots m < j, so the result equals the reference's in-place downward sweep.
+// the original entries of slots m < j, so the result equals the reference's in-place downward sweep. A
+// conservative bounding-ball test skips candidates that cannot be inside before the exact frame test.
 #include "egt_device.h"
 #include "egt_internal.h"
 
@@ -42,19 +43,24 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
     const int32_t s = __float_as_int(p.w);
     int32_t* row = knn + (int64_t)s * k;
     int32_t* mr = mrow + m * k;
+    // staging: the row, then every Morton position, then every coordinate are issued back to back, so the
+    // three dependent loads per slot cost three latencies per point rather than three per slot
     uint64_t valid = 0;
-    for (int j = 0; j < k; ++j) {
-        const int32_t nb = row[j];
-        int32_t nm = -1;
-        if (nb >= 0) {
+    int32_t nbv[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) nbv[j] = j < k ? row[j] : -1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) nbv[j] = nbv[j] >= 0 ? minv[nbv[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (nbv[j] >= 0) {
             valid |= 1ull << j;
-            nm = minv[nb];
-            const float4 c = mpts[nm];
+            const float4 c = mpts[nbv[j]];
             sx[j][tid] = c.x;
             sy[j][tid] = c.y;
             sz[j][tid] = c.z;
         }
-        mr[j] = nm;
+        if (j < k) mr[j] = nbv[j];
     }
     const float x = p.x, y = p.y, z = p.z;
     const float aa = alpha * alpha;
@@ -87,11 +93,17 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
         const float ty = -rowt(bx, by, bz, 0.0f, xo, yo, zo);
         const float tz = -rowt(cx, cy, cz, 0.0f, xo, yo, zo);
         const float dd = d * d;
+        // The ellipsoid lies in the ball of radius d*max(1, alpha) around the midpoint. A candidate outside
+        // that ball padded by a relative 1e-4 and an absolute 4e-6 (well above the ~1e-6 rounding of the frame
+        // transform on unit-cube coordinates) cannot pass the exact test below, so it is skipped.
+        const float rb = d * fmaxf(1.0f, alpha) * 1.0001f + 4e-6f;
+        const float rb2 = rb * rb;
         uint64_t mset = valid & ((1ull << j) - 1ull);
         while (mset) {
             const int mm = __builtin_ctzll(mset);
             mset &= mset - 1ull;
             const float px = sx[mm][tid], py = sy[mm][tid], pz = sz[mm][tid];
+            if (sqd(px, py, pz, xo, yo, zo) >= rb2) continue;
             const float ex = rowt(ax, ay, az, tx, px, py, pz);
             const float ey = rowt(bx, by, bz, ty, px, py, pz);
             const float ez = rowt(cx, cy, cz, tz, px, py, pz);

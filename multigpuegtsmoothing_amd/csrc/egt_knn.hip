@@ -124,22 +124,37 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                 }
                 if (inside) continue;
                 const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level], morton_code(c3[0], c3[1], c3[2]));
-                for (uint32_t p = r.x; p < r.x + r.y; ++p) {
-                    const float4 e = P.mpts[p];
-                    const int32_t j = __float_as_int(e.w);
-                    if (j == jq) continue;
-                    if (overlap) {
-                        const uint32_t cc = P.mcell[p];
-                        const int ex = cc & 1023, ey = (cc >> 10) & 1023, ez = cc >> 20;
-                        if (ex >= plo[0] && ex <= phi[0] && ey >= plo[1] && ey <= phi[1] && ez >= plo[2] &&
-                            ez <= phi[2])
-                            continue;
+                const uint32_t pend = r.x + r.y;
+                // candidates are fetched CH at a time before any is used (one memory latency per chunk); the
+                // top-K under the (distance, index) order does not depend on the offering order
+                constexpr int CH = 4;
+                for (uint32_t p0 = r.x; p0 < pend; p0 += CH) {
+                    float4 ev[CH];
+                    uint32_t cc[CH];
+#pragma unroll
+                    for (int u = 0; u < CH; ++u) {
+                        const uint32_t p = min(p0 + u, pend - 1);
+                        ev[u] = P.mpts[p];
+                        cc[u] = overlap ? P.mcell[p] : 0u;
                     }
-                    ++cand;
-                    const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
-                    if (d <= P.thres_n) {
-                        const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
-                        if (kk < L[K - 1]) topk_insert<K>(L, kk);
+#pragma unroll
+                    for (int u = 0; u < CH; ++u) {
+                        if (p0 + u >= pend) continue;
+                        const float4 e = ev[u];
+                        const int32_t j = __float_as_int(e.w);
+                        if (j == jq) continue;
+                        if (overlap) {
+                            const int ex = cc[u] & 1023, ey = (cc[u] >> 10) & 1023, ez = cc[u] >> 20;
+                            if (ex >= plo[0] && ex <= phi[0] && ey >= plo[1] && ey <= phi[1] && ez >= plo[2] &&
+                                ez <= phi[2])
+                                continue;
+                        }
+                        ++cand;
+                        const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
+                        if (d <= P.thres_n) {
+                            const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
+                            if (kk < L[K - 1]) topk_insert<K>(L, kk);
+                        }
                     }
                 }
             }

@@ -64,31 +64,31 @@ __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, c
     nz[i] = vz / nrm;
 }
 
-__device__ inline bool row_has(const int32_t* __restrict__ knn, int k, int32_t r, int32_t v) {
-    const int32_t* row = knn + (int64_t)r * k;
-    for (int j = 0; j < k; ++j)
-        if (row[j] == v) return true;
-    return false;
-}
-
 // pass 0: count per owner u; pass 1: write v into u's bucket
+// Walks the Morton-order CSR (the same rows as the sorted-space neighbour lists, holes removed); the
+// ownership test scans the neighbour's compact CSR row.
 template <int PASS>
-__global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ knn, int64_t n, int k,
+__global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                                 const int32_t* __restrict__ morder, int64_t n,
                                                  uint32_t* __restrict__ cnt, int32_t* __restrict__ ev) {
-    const int64_t i = (int64_t)blockIdx.x * NBLK + threadIdx.x;
-    if (i >= n) return;
-    const int32_t* row = knn + i * k;
-    for (int j = 0; j < k; ++j) {
-        const int32_t nb = row[j];
-        if (nb < 0 || nb == i) continue;
+    const int64_t m = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    if (m >= n) return;
+    const int32_t i = morder[m];
+    const int32_t e0 = rowptr[m], e1 = rowptr[m + 1];
+    for (int32_t e = e0; e < e1; ++e) {
+        const int32_t m2 = col[e];
+        const int32_t nb = morder[m2];
+        if (nb == i) continue;
         int32_t u, v;
         if (i < nb) {
-            u = (int32_t)i;
+            u = i;
             v = nb;
         } else {
-            if (row_has(knn, k, nb, (int32_t)i)) continue;  // row nb owns it
+            bool listed = false;  // does row nb list i? then row nb owns the edge
+            for (int32_t f = rowptr[m2], f1 = rowptr[m2 + 1]; f < f1; ++f) listed = listed || col[f] == (int32_t)m;
+            if (listed) continue;
             u = nb;
-            v = (int32_t)i;
+            v = i;
         }
         const uint32_t p = atomicAdd(&cnt[u], 1u);
         if (PASS == 1) ev[p] = v;
@@ -161,14 +161,65 @@ __device__ inline void wave_atomic_min(uint64_t* base, int32_t idx, uint64_t val
     }
 }
 
+// Block-level variant: when all active lanes of the block target one address, one atomic per block.
+__device__ inline void block_atomic_min(uint64_t* base, int32_t idx, uint64_t val, bool active) {
+    __shared__ int32_t widx[NBLK / 64];
+    __shared__ unsigned long long wval[NBLK / 64];
+    __shared__ int wstate[NBLK / 64];  // 0 idle, 1 uniform, 2 mixed
+    const int wid = threadIdx.x / 64;
+    const uint64_t amask = __ballot(active);
+    int state = 0;
+    int32_t lidx = 0;
+    unsigned long long v = ~0ull;
+    if (amask) {
+        const int leader = __ffsll((unsigned long long)amask) - 1;
+        lidx = __shfl(idx, leader, 64);
+        state = __ballot(active && idx != lidx) == 0 ? 1 : 2;
+        v = active ? (unsigned long long)val : ~0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(v, d, 64);
+            v = o < v ? o : v;
+        }
+    }
+    if (__lane_id() == 0) {
+        widx[wid] = lidx;
+        wval[wid] = v;
+        wstate[wid] = state;
+    }
+    __syncthreads();
+    if (state == 2 && active) atomicMin((unsigned long long*)&base[idx], (unsigned long long)val);
+    if (threadIdx.x == 0) {
+        bool all_same = true;
+        int32_t bidx = -1;
+        unsigned long long bv = ~0ull;
+        for (int w = 0; w < NBLK / 64; ++w) {
+            if (wstate[w] != 1) continue;
+            if (bidx < 0) bidx = widx[w];
+            if (widx[w] != bidx) all_same = false;
+            bv = wval[w] < bv ? wval[w] : bv;
+        }
+        if (bidx >= 0) {
+            if (all_same) {
+                atomicMin((unsigned long long*)&base[bidx], bv);
+            } else {
+                for (int w = 0; w < NBLK / 64; ++w)
+                    if (wstate[w] == 1) atomicMin((unsigned long long*)&base[widx[w]], wval[w]);
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
-                                                   const uint64_t* __restrict__ ekey, uint32_t m,
-                                                   const int32_t* __restrict__ comp, uint64_t* __restrict__ best) {
-    const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
+                                                   const uint64_t* __restrict__ ekey, const uint32_t* __restrict__ alive,
+                                                   uint32_t m, const int32_t* __restrict__ comp,
+                                                   uint64_t* __restrict__ best) {
+    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
     int32_t cu = 0, cv = 0;
     uint64_t key = 0;
-    bool act = e < m;
+    bool act = t < m;
     if (act) {
+        const uint32_t e = alive ? alive[t] : t;
         cu = comp[eu[e]];
         cv = comp[ev[e]];
         act = cu != cv;
@@ -176,6 +227,24 @@ __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu
     }
     wave_atomic_min(best, cu, key, act);
     wave_atomic_min(best, cv, key, act);
+}
+
+// Edges still joining two components after this round's merge (the rest can never be chosen again).
+__global__ void __launch_bounds__(NBLK) k_bv_alive(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
+                                                    const uint32_t* __restrict__ alive, uint32_t m,
+                                                    const int32_t* __restrict__ comp, uint32_t* __restrict__ flag) {
+    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
+    if (t >= m) return;
+    const uint32_t e = alive ? alive[t] : t;
+    flag[t] = comp[eu[e]] != comp[ev[e]] ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(NBLK) k_bv_compact(const uint32_t* __restrict__ alive, uint32_t m,
+                                                      const uint32_t* __restrict__ flag,
+                                                      const uint32_t* __restrict__ pos, uint32_t* __restrict__ out) {
+    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
+    if (t >= m || !flag[t]) return;
+    out[pos[t]] = alive ? alive[t] : t;
 }
 
 __global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __restrict__ eu,
@@ -228,12 +297,12 @@ __global__ void __launch_bounds__(NBLK) k_bv_relabel(int64_t n, int32_t* __restr
     par[v] ^= (uint8_t)(w & 1u);
 }
 
-__global__ void __launch_bounds__(NBLK) k_tree_min(const uint8_t* __restrict__ msf, const uint64_t* __restrict__ ekey,
-                                                    const int32_t* __restrict__ eu, uint32_t m,
-                                                    const int32_t* __restrict__ comp, uint64_t* __restrict__ tmin) {
-    const uint32_t e = blockIdx.x * NBLK + threadIdx.x;
-    const bool act = e < m && msf[e];
-    wave_atomic_min(tmin, act ? comp[eu[e]] : 0, act ? ekey[e] : 0ull, act);
+// Lightest edge of each tree = minimum over its vertices of their lightest incident edge (round-1 best).
+__global__ void __launch_bounds__(NBLK) k_tree_min(int64_t n, const int32_t* __restrict__ comp,
+                                                    const uint64_t* __restrict__ best1, uint64_t* __restrict__ tmin) {
+    const int64_t v = (int64_t)blockIdx.x * NBLK + threadIdx.x;
+    const bool act = v < n && best1[v] != KEY_EMPTY;
+    block_atomic_min(tmin, act ? comp[v] : 0, act ? best1[v] : 0ull, act);
 }
 
 __global__ void __launch_bounds__(NBLK) k_orient(int64_t n, const int32_t* __restrict__ comp,
@@ -330,14 +399,14 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
     uint32_t* off = h->scratch.get<uint32_t>(10, n);
     HIP_CHECK(hipMemsetAsync(cnt, 0, n * sizeof(uint32_t), s));
-    k_edges<0><<<gn, blk, 0, s>>>(h->knn.p, n, k, cnt, nullptr);
+    k_edges<0><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, nullptr);
     LAUNCH_CHECK();
     uint32_t m = 0;
     exclusive_scan_u32(cnt, off, n, s, h->scratch, &m);
     HIP_CHECK(hipMemcpyAsync(cnt, off, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     DBuf<uint8_t> pool;
     const size_t mm = std::max<uint32_t>(m, 1);
-    pool.alloc(mm * (4 + 4 + 8 + 1 + 1 + 4) + (size_t)n * (4 + 1 + 8 + 4 + 8) + 16 * 256 + 4096 * 12);
+    pool.alloc(mm * (4 + 4 + 8 + 1 + 1 + 4 + 16) + (size_t)n * (4 + 1 + 8 + 4 + 8 + 8) + 24 * 256 + 4096 * 12);
     size_t po = 0;
     int32_t* ev = dalloc<int32_t>(pool, po, mm);
     int32_t* eu = dalloc<int32_t>(pool, po, mm);
@@ -350,12 +419,17 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     uint64_t* best = dalloc<uint64_t>(pool, po, n);
     uint32_t* H = dalloc<uint32_t>(pool, po, n);
     uint64_t* tmin = dalloc<uint64_t>(pool, po, n);
+    uint64_t* best1 = dalloc<uint64_t>(pool, po, n);
+    uint32_t* alive_a = dalloc<uint32_t>(pool, po, mm);
+    uint32_t* alive_b = dalloc<uint32_t>(pool, po, mm);
+    uint32_t* aflag = dalloc<uint32_t>(pool, po, mm);
+    uint32_t* apos = dalloc<uint32_t>(pool, po, mm);
     uint32_t* flags = dalloc<uint32_t>(pool, po, 4);
     double* part_w = dalloc<double>(pool, po, 1024);
     uint32_t* part_c = dalloc<uint32_t>(pool, po, 1024);
     if (po > pool.n) throw ArgError{EGT_ERR_STATE, "normals pool overflow"};
 
-    k_edges<1><<<gn, blk, 0, s>>>(h->knn.p, n, k, cnt, ev);
+    k_edges<1><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, ev);
     LAUNCH_CHECK();
     k_edge_sort<<<gn, blk, 0, s>>>(off, n, m, ev, eu);
     LAUNCH_CHECK();
@@ -369,26 +443,37 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     // Boruvka with parity
     k_bv_init<<<gn, blk, 0, s>>>(n, comp, par);
     LAUNCH_CHECK();
-    for (int round = 0; round < 64 && m > 0; ++round) {
+    uint32_t* alive = nullptr;  // null = all edges
+    uint32_t malive = m;
+    for (int round = 0; round < 64 && malive > 0; ++round) {
+        const dim3 ga(std::max(1, ceil_div(malive, NBLK)));
         k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
         HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
-        k_bv_best<<<gm, blk, 0, s>>>(eu, ev, ekey, m, comp, best);
+        k_bv_best<<<ga, blk, 0, s>>>(eu, ev, ekey, alive, malive, comp, best);
+        if (round == 0) HIP_CHECK(hipMemcpyAsync(best1, best, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
         k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf, flags);
         k_bv_jump<<<gn, blk, 0, s>>>(n, comp, H, flags + 1);
         k_bv_relabel<<<gn, blk, 0, s>>>(n, comp, par, H);
+        k_bv_alive<<<ga, blk, 0, s>>>(eu, ev, alive, malive, comp, aflag);
         LAUNCH_CHECK();
         uint32_t hf[2] = {0, 0};
         HIP_CHECK(hipMemcpyAsync(hf, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
+        uint32_t next = 0;
+        exclusive_scan_u32(aflag, apos, malive, s, h->scratch, &next);  // synchronizes the stream
         if (hf[1]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
-        if (hf[0] == 0) break;
+        if (hf[0] == 0 || next == 0) break;
+        uint32_t* out = alive == alive_a ? alive_b : alive_a;
+        k_bv_compact<<<ga, blk, 0, s>>>(alive, malive, aflag, apos, out);
+        LAUNCH_CHECK();
+        alive = out;
+        malive = next;
     }
 
     // orientation
     k_bv_reset<<<gn, blk, 0, s>>>(n, tmin, H);
     LAUNCH_CHECK();
     if (m > 0) {
-        k_tree_min<<<gm, blk, 0, s>>>(msf, ekey, eu, m, comp, tmin);
+        k_tree_min<<<gn, blk, 0, s>>>(n, comp, best1, tmin);
         LAUNCH_CHECK();
     }
     k_orient<<<gn, blk, 0, s>>>(n, comp, par, tmin, eu, nx, ny, nz);
