@@ -50,6 +50,7 @@ typedef struct {
     int64_t msf_trees;         /* trees (components with at least one edge) */
     double msf_weight;         /* total forest weight */
     float knn_ms, egg_ms, taubin_ms, normals_ms, grid_ms; /* device time of the last call of each stage */
+    float propose_ms;          /* device time of the last learned proposal (features + MLP + level choice) */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */
@@ -132,6 +133,49 @@ egt_status egt_destroy(egt_handle h);
 
 /* HIP stream of the handle (hipStream_t), for callers that order their own work after the library's. */
 void* egt_get_stream(egt_handle h);
+
+/* ---- Learned search-radius proposal (no reference counterpart; the reference starts every query at its
+ * own cell, GridKNNCuda.cu:2409-2474). A 28 -> 64 -> 64 -> 1 ReLU MLP maps the log-normalised occupancy of
+ * the 3x3x3 cells around a query (at the level the plain search would start from) plus log2(k) to
+ * log2(k-th neighbour distance / cell edge); the kNN then starts each query at the smallest level whose
+ * box reaches that radius plus a safety margin. Its exactness test expands past any proposal that is too
+ * small, so neighbour rows are identical to the plain search's. Forward and backward run on BF16 MFMA with
+ * FP32 accumulation; Adam (beta1 0.9, beta2 0.999, eps 1e-8) updates FP32 master weights.
+ * Parameter vector (FP32, egt_model_param_count() entries): W1[32][64] (input-major; rows 28..31 are
+ * padding and kept at zero), b1[64], W2[64][64], b2[64], W3[64], b3.
+ * Feature rows passed in or out are egt_model_feature_count() = 28 floats each. */
+typedef struct egt_model_s* egt_model;
+
+int32_t egt_model_param_count(void);
+int32_t egt_model_feature_count(void);
+/* New model on the current device, weights drawn He-uniform from seed. */
+egt_status egt_model_create(uint64_t seed, egt_model* out);
+egt_status egt_model_destroy(egt_model m);
+egt_status egt_model_get_params(egt_model m, float* params, int64_t count);
+/* Sets the weights and clears the Adam state. */
+egt_status egt_model_set_params(egt_model m, const float* params, int64_t count);
+/* Forward and gradient of the mean squared error over one batch (host arrays): y[n], grad[param_count]
+ * and loss are each optional. */
+egt_status egt_model_eval(egt_model m, const float* features, const float* targets, int64_t n, float* y,
+                          float* grad, float* loss);
+/* One Adam step on a host batch; grad (optional) receives the gradient it used. */
+egt_status egt_model_adam_step(egt_model m, const float* features, const float* targets, int64_t n, float lr,
+                               float* grad, float* loss);
+/* Trains on a cloud: runs the plain exact search with k on h (its neighbour rows become that search's),
+ * takes up to max_samples valid queries (0 = all), runs epochs of Adam over batches of batch samples, and
+ * sets the safety margin to the given quantile of the training residuals. loss = final training MSE. */
+egt_status egt_model_train(egt_model m, egt_handle h, int k, int epochs, int batch, int64_t max_samples, float lr,
+                           float quantile, float* loss);
+egt_status egt_model_get_safety(egt_model m, float* safety);
+egt_status egt_model_set_safety(egt_model m, float safety);
+/* Proposes a start level for every query of h for neighbour count k and switches h to learned starts
+ * (EGT_OPT_LEARNED = 1); the next egt_grid_knn_self uses them. */
+egt_status egt_model_propose(egt_model m, egt_handle h, int k);
+/* Diagnostics: feature rows (n x 28, sorted order), targets of the handle's last search with this k (NaN
+ * where invalid or absent) and the reference level; each output optional. */
+egt_status egt_model_features(egt_handle h, int k, float* features, float* targets, int32_t* ref_level);
+/* Diagnostics: the proposed start level of every point (sorted order). */
+egt_status egt_model_levels(egt_handle h, int8_t* levels);
 
 #ifdef __cplusplus
 }

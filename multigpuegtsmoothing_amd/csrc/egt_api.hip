@@ -260,4 +260,110 @@ egt_status egt_destroy(egt_handle h) {
 
 void* egt_get_stream(egt_handle h) { return h ? (void*)h->stream : nullptr; }
 
+int32_t egt_model_param_count(void) { return egt::mlp_param_count(); }
+int32_t egt_model_feature_count(void) { return egt::mlp_feature_count(); }
+
+egt_status egt_model_create(uint64_t seed, egt_model* out) {
+    if (!out) return fail(EGT_ERR_INVALID_ARG, "null model pointer");
+    egt_model_s* m = new (std::nothrow) egt_model_s();
+    if (!m) return fail(EGT_ERR_OOM, "host allocation failed");
+    egt_status st = guarded([&] { egt::mlp_init(m, seed); });
+    if (st != EGT_OK) {
+        if (m->stream) (void)hipStreamDestroy(m->stream);
+        delete m;
+        return st;
+    }
+    *out = m;
+    return EGT_OK;
+}
+
+egt_status egt_model_destroy(egt_model m) {
+    if (!m) return EGT_OK;
+    if (m->stream) {
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipStreamDestroy(m->stream);
+    }
+    delete m;
+    return EGT_OK;
+}
+
+egt_status egt_model_get_params(egt_model m, float* params, int64_t count) {
+    if (!m || !params) return fail(EGT_ERR_INVALID_ARG, "null model or parameter array");
+    if (count != egt::mlp_param_count()) return fail(EGT_ERR_INVALID_ARG, "count must be egt_model_param_count()");
+    return guarded([&] { egt::mlp_get_params(m, params); });
+}
+
+egt_status egt_model_set_params(egt_model m, const float* params, int64_t count) {
+    if (!m || !params) return fail(EGT_ERR_INVALID_ARG, "null model or parameter array");
+    if (count != egt::mlp_param_count()) return fail(EGT_ERR_INVALID_ARG, "count must be egt_model_param_count()");
+    return guarded([&] { egt::mlp_set_params(m, params); });
+}
+
+egt_status egt_model_eval(egt_model m, const float* features, const float* targets, int64_t n, float* y, float* grad,
+                          float* loss) {
+    if (!m || !features || !targets) return fail(EGT_ERR_INVALID_ARG, "null model, features or targets");
+    if (n < 1 || n > ((int64_t)1 << 26)) return fail(EGT_ERR_INVALID_ARG, "batch size must be in [1, 2^26]");
+    return guarded([&] { egt::mlp_eval(m, features, targets, n, y, grad, loss); });
+}
+
+egt_status egt_model_adam_step(egt_model m, const float* features, const float* targets, int64_t n, float lr,
+                               float* grad, float* loss) {
+    if (!m || !features || !targets) return fail(EGT_ERR_INVALID_ARG, "null model, features or targets");
+    if (n < 1 || n > ((int64_t)1 << 26)) return fail(EGT_ERR_INVALID_ARG, "batch size must be in [1, 2^26]");
+    if (!(lr > 0.0f) || !std::isfinite(lr)) return fail(EGT_ERR_INVALID_ARG, "learning rate must be > 0");
+    return guarded([&] { egt::mlp_adam_step(m, features, targets, n, lr, grad, loss); });
+}
+
+egt_status egt_model_train(egt_model m, egt_handle h, int k, int epochs, int batch, int64_t max_samples, float lr,
+                           float quantile, float* loss) {
+    if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
+    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "training needs a cloud handle");
+    if (k < 1 || k > 64 || k >= h->n) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, min(64, n - 1)]");
+    if (epochs < 0 || batch < 32) return fail(EGT_ERR_INVALID_ARG, "epochs must be >= 0 and batch >= 32");
+    if (max_samples < 0) return fail(EGT_ERR_INVALID_ARG, "max_samples must be >= 0");
+    if (!(lr > 0.0f) || !std::isfinite(lr)) return fail(EGT_ERR_INVALID_ARG, "learning rate must be > 0");
+    if (!(quantile >= 0.0f && quantile <= 1.0f)) return fail(EGT_ERR_INVALID_ARG, "quantile must be in [0, 1]");
+    return guarded([&] {
+        h->csr_valid = false;
+        h->smoothed = false;
+        egt::mlp_train(m, h, k, epochs, batch, max_samples, lr, quantile, loss);
+    });
+}
+
+egt_status egt_model_get_safety(egt_model m, float* safety) {
+    if (!m || !safety) return fail(EGT_ERR_INVALID_ARG, "null model or output");
+    *safety = m->safety;
+    return EGT_OK;
+}
+
+egt_status egt_model_set_safety(egt_model m, float safety) {
+    if (!m) return fail(EGT_ERR_INVALID_ARG, "null model");
+    if (!std::isfinite(safety)) return fail(EGT_ERR_INVALID_ARG, "safety must be finite");
+    m->safety = safety;
+    return EGT_OK;
+}
+
+egt_status egt_model_propose(egt_model m, egt_handle h, int k) {
+    if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
+    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "proposals apply to cloud handles");
+    if (k < 1 || k > 64) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, 64]");
+    return guarded([&] { egt::mlp_propose(m, h, k); });
+}
+
+egt_status egt_model_features(egt_handle h, int k, float* features, float* targets, int32_t* ref_level) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (k < 1 || k > 64) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, 64]");
+    return guarded([&] {
+        int r = 0;
+        egt::mlp_features(h, k, features, targets, &r);
+        if (ref_level) *ref_level = r;
+    });
+}
+
+egt_status egt_model_levels(egt_handle h, int8_t* levels) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!levels) return fail(EGT_ERR_INVALID_ARG, "null output");
+    return guarded([&] { egt::mlp_levels(h, levels); });
+}
+
 }  // extern "C"

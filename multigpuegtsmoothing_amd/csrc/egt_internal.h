@@ -64,10 +64,10 @@ struct LevelTable {
     int64_t cells = 0;
 };
 
-// Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code,
-// 16 host-input staging, 17-23 free.
+// Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code (15 = k-th
+// distance of the last search), 16 host-input staging, 17-21 EGG / Taubin, 24-26 learned proposal.
 struct Scratch {
-    DBuf<uint32_t> bufs[24];
+    DBuf<uint32_t> bufs[28];
     template <class T>
     T* get(int i, size_t count) {
         bufs[i].alloc((count * sizeof(T) + sizeof(uint32_t) - 1) / sizeof(uint32_t));
@@ -129,6 +129,20 @@ struct egt_ctx {
     egt::Scratch scratch;
 };
 
+// Learned proposal model (egt_mlp.hip): FP32 master weights, Adam moments, gradient partials, training set.
+struct egt_model_s {
+    egt::DBuf<float> P, M, V, grad, part, ploss, Y;
+    egt::DBuf<float> X, T;       // host-supplied batch
+    egt::DBuf<uint32_t> sel;     // training sample order
+    int step = 0;
+    int last_parts = 0;
+    float safety = 0.0f;
+    int64_t train_samples = 0;
+    uint64_t rng = 0x243F6A8885A308D3ull;
+    hipStream_t stream = nullptr;
+    int device = 0;
+};
+
 namespace egt {
 // scan / sort / reduce utilities (egt_prims.hip)
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
@@ -143,6 +157,21 @@ void refresh_search_structure(egt_ctx* h);
 
 // kNN (egt_knn.hip)
 void run_knn(egt_ctx* h, int k, float thres_n);
+int auto_start_level(const egt_ctx* h, int k);
+
+// learned proposal (egt_mlp.hip)
+int mlp_param_count();
+int mlp_feature_count();
+void mlp_init(egt_model_s* m, uint64_t seed);
+void mlp_set_params(egt_model_s* m, const float* params);
+void mlp_get_params(egt_model_s* m, float* params);
+void mlp_eval(egt_model_s* m, const float* X, const float* T, int64_t n, float* y, float* grad, float* loss);
+void mlp_adam_step(egt_model_s* m, const float* X, const float* T, int64_t n, float lr, float* grad, float* loss);
+void mlp_train(egt_model_s* m, egt_ctx* h, int k, int epochs, int batch, int64_t max_samples, float lr,
+               float quantile, float* loss);
+void mlp_propose(egt_model_s* m, egt_ctx* h, int k);
+void mlp_features(egt_ctx* h, int k, float* X, float* T, int* ref_level);
+void mlp_levels(egt_ctx* h, int8_t* out);
 
 // EGG + CSR (egt_egg.hip)
 void run_egg(egt_ctx* h, float alpha);

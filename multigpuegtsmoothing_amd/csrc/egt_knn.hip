@@ -112,6 +112,23 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
         for (;;) {
             const int Bl = P.B >> level;
             const int cq[3] = {fx >> level, fy >> level, fz >> level};
+            // Early exit (exact): the box of level l is what the reference's stage l + 1 scans. Once the box
+            // of level l - 1 is scanned, the list answers every earlier stage exactly as the reference's own
+            // list would. If stage l + 1 does not exist (l >= stages) or would meet a cell of >= maxc points
+            // (the reference then invalidates the query), nothing beyond the scanned box can change the
+            // outcome: the replay below either stops at an earlier stage with this list or blanks the row.
+            if (level > first_level) {
+                if (level >= P.stages) break;
+                if ((P.bigmask >> level) & 1u) {
+                    bool big = false;
+                    for (int t = 0; t < 27 && !big; ++t) {
+                        if (t == 13) continue;
+                        big = cell_count(P, level, cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1,
+                                         cq[2] + (t / 9) - 1) >= (uint32_t)P.maxc;
+                    }
+                    if (big) break;
+                }
+            }
             for (int t = 0; t < 27; ++t) {
                 const int c3[3] = {cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1, cq[2] + (t / 9) - 1};
                 if (c3[0] < 0 || c3[1] < 0 || c3[2] < 0 || c3[0] >= Bl || c3[1] >= Bl || c3[2] >= Bl) continue;

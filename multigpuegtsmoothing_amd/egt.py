@@ -39,7 +39,7 @@ class Stats(ctypes.Structure):
                 ("learned_queries", ctypes.c_int64), ("learned_fallbacks", ctypes.c_int64),
                 ("msf_edges", ctypes.c_int64), ("msf_trees", ctypes.c_int64), ("msf_weight", ctypes.c_double),
                 ("knn_ms", ctypes.c_float), ("egg_ms", ctypes.c_float), ("taubin_ms", ctypes.c_float),
-                ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float)]
+                ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("propose_ms", ctypes.c_float)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -49,6 +49,21 @@ _F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _I32P = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _U32P = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _H = ctypes.c_void_p
+
+
+class _OptionalF32:
+    """ctypes argument type: a C-contiguous float32 array or None (NULL)."""
+
+    @classmethod
+    def from_param(cls, obj):
+        if obj is None:
+            return None
+        if not (isinstance(obj, np.ndarray) and obj.dtype == np.float32 and obj.flags["C_CONTIGUOUS"]):
+            raise TypeError("expected a C-contiguous float32 array or None")
+        return obj.ctypes.data_as(ctypes.c_void_p)
+
+
+_F32N = _OptionalF32
 _PH = ctypes.POINTER(ctypes.c_void_p)
 _PF = ctypes.POINTER(ctypes.c_void_p)
 
@@ -80,6 +95,20 @@ def _load():
         "egt_set_option": ([_H, i32, i64], st),
         "egt_destroy": ([_H], st),
         "egt_get_stream": ([_H], ctypes.c_void_p),
+        "egt_model_param_count": ([], ctypes.c_int32),
+        "egt_model_feature_count": ([], ctypes.c_int32),
+        "egt_model_create": ([ctypes.c_uint64, _PH], st),
+        "egt_model_destroy": ([_H], st),
+        "egt_model_get_params": ([_H, _F32P, i64], st),
+        "egt_model_set_params": ([_H, _F32P, i64], st),
+        "egt_model_eval": ([_H, _F32P, _F32P, i64, _F32N, _F32N, ctypes.POINTER(f32)], st),
+        "egt_model_adam_step": ([_H, _F32P, _F32P, i64, f32, _F32N, ctypes.POINTER(f32)], st),
+        "egt_model_train": ([_H, _H, i32, i32, i32, i64, f32, f32, ctypes.POINTER(f32)], st),
+        "egt_model_get_safety": ([_H, ctypes.POINTER(f32)], st),
+        "egt_model_set_safety": ([_H, f32], st),
+        "egt_model_propose": ([_H, _H, i32], st),
+        "egt_model_features": ([_H, i32, _F32N, _F32N, ctypes.POINTER(ctypes.c_int32)], st),
+        "egt_model_levels": ([_H, np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")], st),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -268,6 +297,104 @@ def getOutputPoints(provider):
     x, y, z = (np.empty(n, np.float32) for _ in range(3))
     _check(_lib.egt_get_output_points(provider.handle, x, y, z))
     return x, y, z
+
+
+class RadiusModel:
+    """Learned search-radius proposal (no reference counterpart): MLP 28 -> 64 -> 64 -> 1 trained on the GPU
+    by the library's BF16-MFMA forward / backward and Adam kernels. A proposal only moves where each query's
+    search starts; the exactness test keeps neighbour rows identical to the plain search."""
+
+    def __init__(self, seed=1):
+        h = ctypes.c_void_p()
+        _check(_lib.egt_model_create(seed, ctypes.byref(h)))
+        self._h = h.value
+
+    @staticmethod
+    def param_count():
+        return int(_lib.egt_model_param_count())
+
+    @staticmethod
+    def feature_count():
+        return int(_lib.egt_model_feature_count())
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise EGTError(6, "model already destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.egt_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.empty(self.param_count(), np.float32)
+        _check(_lib.egt_model_get_params(self.handle, p, p.size))
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _check(_lib.egt_model_set_params(self.handle, p, p.size))
+
+    def eval(self, features, targets):
+        """Forward outputs, gradient of the batch MSE and the loss for host features (n x 28) and targets."""
+        X = np.ascontiguousarray(features, np.float32)
+        T = np.ascontiguousarray(targets, np.float32)
+        y = np.empty(len(T), np.float32)
+        g = np.empty(self.param_count(), np.float32)
+        loss = ctypes.c_float()
+        _check(_lib.egt_model_eval(self.handle, X, T, len(T), y, g, ctypes.byref(loss)))
+        return y, g, loss.value
+
+    def adam_step(self, features, targets, lr):
+        X = np.ascontiguousarray(features, np.float32)
+        T = np.ascontiguousarray(targets, np.float32)
+        g = np.empty(self.param_count(), np.float32)
+        loss = ctypes.c_float()
+        _check(_lib.egt_model_adam_step(self.handle, X, T, len(T), lr, g, ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, provider, k, epochs=4, batch=4096, max_samples=2_000_000, lr=2e-3, quantile=0.9):
+        loss = ctypes.c_float()
+        _check(_lib.egt_model_train(self.handle, provider.handle, k, epochs, batch, max_samples, lr, quantile,
+                                    ctypes.byref(loss)))
+        return loss.value
+
+    @property
+    def safety(self):
+        v = ctypes.c_float()
+        _check(_lib.egt_model_get_safety(self.handle, ctypes.byref(v)))
+        return v.value
+
+    @safety.setter
+    def safety(self, value):
+        _check(_lib.egt_model_set_safety(self.handle, value))
+
+    def propose(self, provider, k):
+        _check(_lib.egt_model_propose(self.handle, provider.handle, k))
+
+
+def model_features(provider, k):
+    """(features n x 28, targets n, reference level) of a provider, sorted order (diagnostics)."""
+    n = provider.pointsRefCount()
+    X = np.empty((n, RadiusModel.feature_count()), np.float32)
+    T = np.empty(n, np.float32)
+    ref = ctypes.c_int32()
+    _check(_lib.egt_model_features(provider.handle, k, X, T, ctypes.byref(ref)))
+    return X, T, ref.value
+
+
+def model_levels(provider):
+    out = np.empty(provider.pointsRefCount(), np.int8)
+    _check(_lib.egt_model_levels(provider.handle, out))
+    return out
 
 
 def device_array(ptr, count, dtype):
