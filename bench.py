@@ -36,11 +36,30 @@ def parse():
     p.add_argument("--no-normals", action="store_true")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=250_000)
+    p.add_argument("--no-learned", action="store_true", help="skip the learned-proposal vs plain comparison")
+    p.add_argument("--no-graph", action="store_true", help="launch the Taubin passes one by one (profiling)")
     return p.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_taubin_tile_torus4M_k40_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per Taubin pass from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled
+    by the same counters on a copy kernel of known size), or None when that summary is absent."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def gpu_step(E, torch, dev_xyz, args, stream_sync):
     g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+    if args.no_graph:
+        g.set_option(E.OPT_TAUBIN_GRAPH, 0)
     g.GRIDCUDAKNNSELF_COMPACT(args.k, 1e8)
     E.EGT_CUDA(g).PerformSmoothing(args.iters, 0.5, -0.53, 0.65)
     if not args.no_normals:
@@ -48,6 +67,45 @@ def gpu_step(E, torch, dev_xyz, args, stream_sync):
     st = g.stats()
     g.close()
     return st
+
+
+def learned_vs_plain(E, dev_xyz, args):
+    """kNN stage of the same workload with the plain start (the query's own cell, as the reference) and
+    with the learned proposal (model trained on the GPU on a torus drawn with a different seed). Rows must
+    be identical; the learned time includes the proposal kernel."""
+    from multigpuegtsmoothing_amd import datasets as D
+
+    k = args.k
+    model = E.RadiusModel(seed=11)
+    train = E.GridStructure(D.torus(2_000_000, seed=1002), 1, 1e-6, 1024, 4)
+    t0 = time.perf_counter()
+    mse = model.train(train, k, epochs=6, batch=4096, max_samples=2_000_000, lr=2e-3, quantile=0.9)
+    train_s = time.perf_counter() - t0
+    train.close()
+    res = {}
+    rows = {}
+    for mode in ("plain", "learned"):
+        g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+        best = None
+        for _ in range(3):
+            prop = 0.0
+            if mode == "learned":
+                model.propose(g, k)
+                prop = g.stats()["propose_ms"]
+            g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
+            st = g.stats()
+            t = st["knn_ms"] + prop
+            best = t if best is None else min(best, t)
+        res[mode] = {"knn_ms": best, "candidates_per_query": st["knn_candidates"] / args.n,
+                     "cells_per_query": st["knn_cells"] / args.n,
+                     "fallback_rate": st["learned_fallbacks"] / args.n if mode == "learned" else None}
+        rows[mode] = g.GetKNNIndexes()
+        g.close()
+    model.close()
+    return {"plain": res["plain"], "learned": res["learned"],
+            "knn_speedup": res["plain"]["knn_ms"] / res["learned"]["knn_ms"],
+            "bit_exact": bool(np.array_equal(rows["plain"], rows["learned"])),
+            "train": {"cloud": "torus 2M pts, seed 1002", "seconds": train_s, "mse": mse}}
 
 
 def cpu_baseline(args):
@@ -106,8 +164,8 @@ def main():
     dominant = max(stage, key=stage.get)
     if dominant == "taubin":
         ach = taub_bytes / (taub_pass_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_taubin_pass", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": taub_bytes,
+        roof = {"kernel": "k_taubin_tile", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(), "bytes_per_launch": taub_bytes,
                 "avg_launch_ms": taub_pass_ms}
     else:
         ach = knn_bytes / (knn_ms * 1e-3) / 1e9
@@ -138,6 +196,8 @@ def main():
         "invalid_rows": stats[-1]["invalid_rows"],
         "roofline": roof,
     }
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain(E, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)

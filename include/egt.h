@@ -51,14 +51,18 @@ typedef struct {
     double msf_weight;         /* total forest weight */
     float knn_ms, egg_ms, taubin_ms, normals_ms, grid_ms; /* device time of the last call of each stage */
     float propose_ms;          /* device time of the last learned proposal (features + MLP + level choice) */
+    int64_t knn_cells;         /* cells looked up by the kNN search */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */
 typedef enum {
-    EGT_OPT_START_LEVEL = 0,     /* kNN: pyramid level of the first search box; -1 = automatic (default) */
+    EGT_OPT_START_LEVEL = 0,     /* kNN: pyramid level of the first search box; 0 = the query's own cell, as
+                                    in the reference (default); -1 = from the mean cell occupancy */
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
-    EGT_OPT_MAX_PTS_IN_CUBE = 3  /* kNN validity rule: cell population that invalidates a query (1e5) */
+    EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
+    EGT_OPT_KNN_PERSISTENT = 4   /* kNN: 1 = one persistent work-queue launch per neighbour-count class
+                                    (default for batch handles), 0 = one workgroup per 256 queries */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -127,12 +131,39 @@ egt_status egt_get_stats(egt_handle h, egt_stats* st);
 /* Tuning switches (no reference counterpart). */
 egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value);
 
+/* Autotuner (no reference counterpart): times the plain search with neighbour count k and radius thres
+ * (input units, as egt_grid_knn_self) on an evenly spread sample of queries for each candidate start level
+ * (the cell size of the first search box), and sets EGT_OPT_START_LEVEL to the fastest. Rows do not
+ * depend on the choice. trial_ms (optional, 11 floats) receives each level's sample time in ms, -1 where
+ * untried. The handle's neighbour rows are not valid afterwards: run egt_grid_knn_self. */
+egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms);
+
 /* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA
  * leaked (EGT_CUDA.cu:490-492). */
 egt_status egt_destroy(egt_handle h);
 
 /* HIP stream of the handle (hipStream_t), for callers that order their own work after the library's. */
 void* egt_get_stream(egt_handle h);
+
+/* ---- Batches of independent clouds (no reference counterpart: the reference handles one cloud per
+ * provider). x, y, z hold the clouds one after another (host, each in its own original order); cloud c is
+ * points offsets[c] .. offsets[c + 1] - 1 (nclouds + 1 offsets, offsets[0] = 0). Every cloud is normalized,
+ * sorted and searched exactly as if it were its own GridStructure, so results equal separate runs, but each
+ * stage runs over all clouds at once: the kNN as one persistent work-queue launch per neighbour-count class,
+ * the EGG filter as one launch per class, Taubin as one graph over all clouds.
+ * On a batch handle egt_smooth, egt_normals and egt_get_output_points work as on one cloud and use the
+ * concatenated order of the input; egt_grid_knn_self and egt_get_knn_original are replaced below. */
+egt_status egt_batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
+                            float eps, int bucket_count, egt_handle* out);
+/* GRIDCUDAKNNSELF_COMPACT for every cloud: k[c] neighbours (1..64) for cloud c, radius thres in the input
+ * units of each cloud (GridKNNCuda.cu:2322-2634 per cloud). */
+egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres);
+/* GetKNNIndexes for one cloud (GridKNNCuda.cu:1941-1968): n_c x k_c int32 in that cloud's own original
+ * order and index space, -1 holes. */
+egt_status egt_batch_get_knn_original(egt_handle h, int32_t cloud, int32_t* out);
+/* Point count, neighbour count of the last search and bounding cube of one cloud. */
+egt_status egt_batch_cloud_info(egt_handle h, int32_t cloud, int64_t* n, int32_t* k, float* min_extent,
+                                float* max_extent);
 
 /* ---- Learned search-radius proposal (no reference counterpart; the reference starts every query at its
  * own cell, GridKNNCuda.cu:2409-2474). A 28 -> 64 -> 64 -> 1 ReLU MLP maps the log-normalised occupancy of

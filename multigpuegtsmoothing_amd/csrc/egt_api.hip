@@ -101,6 +101,7 @@ egt_status egt_grid_create_device(const float* dx, const float* dy, const float*
 
 egt_status egt_grid_knn_self(egt_handle h, int k, float thres) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind == 2) return fail(EGT_ERR_STATE, "a batch handle searches with egt_batch_knn");
     if (h->kind != 0) return fail(EGT_ERR_STATE, "kNN search needs a point-cloud handle");
     if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
     if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
@@ -178,6 +179,7 @@ egt_status egt_get_output_points(egt_handle h, float* x, float* y, float* z) {
 
 egt_status egt_get_knn_original(egt_handle h, int32_t* out) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind == 2) return fail(EGT_ERR_STATE, "a batch handle exports rows per cloud (egt_batch_get_knn_original)");
     if (!h->have_knn) return fail(EGT_ERR_STATE, "no neighbour rows yet");
     if (!out) return fail(EGT_ERR_INVALID_ARG, "null output array");
     return guarded([&] { egt::run_knn_original(h, out); });
@@ -244,8 +246,116 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
             if (value < 1 || value > INT32_MAX) return fail(EGT_ERR_INVALID_ARG, "max points in cube must be >= 1");
             h->maxc = (int)value;
             return EGT_OK;
+        case EGT_OPT_KNN_PERSISTENT:
+            h->knn_persistent = value ? 1 : 0;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
+}
+
+egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "the autotuner needs a one-cloud point-cloud handle");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    return guarded([&] {
+        float t = thres / (h->max_extent - h->min_extent);
+        t *= t;
+        if (h->search_stale) egt::refresh_search_structure(h);
+        float ms[EGT_MAX_LEVELS];
+        const int best = egt::autotune_start_level(h, k, t, ms);
+        h->csr_valid = false;
+        if (start_level) *start_level = best;
+        if (trial_ms)
+            for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = ms[l];
+    });
+}
+
+egt_status egt_batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
+                            float eps, int bucket_count, egt_handle* out) {
+    if (!out) return fail(EGT_ERR_INVALID_ARG, "null handle pointer");
+    if (!x || !y || !z || !offsets) return fail(EGT_ERR_INVALID_ARG, "null coordinate or offset array");
+    if (nclouds < 1 || nclouds > (1 << 20)) return fail(EGT_ERR_INVALID_ARG, "cloud count must be in [1, 2^20]");
+    if (offsets[0] != 0) return fail(EGT_ERR_INVALID_ARG, "offsets[0] must be 0");
+    for (int32_t c = 0; c < nclouds; ++c)
+        if (offsets[c + 1] <= offsets[c]) return fail(EGT_ERR_INVALID_ARG, "every cloud needs at least one point");
+    const int64_t n = offsets[nclouds];
+    if (n >= (int64_t)1 << 31) return fail(EGT_ERR_UNSUPPORTED, "total point count must fit in int32");
+    if (!pow2_in_range(bucket_count)) return fail(EGT_ERR_INVALID_ARG, "bucketCount must be a power of two in [1, 1024]");
+    if (!(eps >= 0.0f) || !std::isfinite(eps)) return fail(EGT_ERR_INVALID_ARG, "eps must be finite and >= 0");
+    egt_ctx* h = new (std::nothrow) egt_ctx();
+    if (!h) return fail(EGT_ERR_OOM, "host allocation failed");
+    egt_status st = guarded([&] {
+        HIP_CHECK(hipGetDevice(&h->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->kind = 2;
+        h->n = n;
+        h->B = bucket_count;
+        h->eps = eps;
+        h->knn_persistent = 1;
+        h->stats.n_points = n;
+        h->stats.bucket_count = bucket_count;
+        float* tmp = h->scratch.get<float>(16, 3 * (size_t)n);
+        HIP_CHECK(hipMemcpyAsync(tmp, x, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(hipMemcpyAsync(tmp + n, y, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        HIP_CHECK(hipMemcpyAsync(tmp + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        egt::build_grid_batch(h, tmp, tmp + n, tmp + 2 * n, offsets, nclouds);
+    });
+    if (st != EGT_OK) {
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return EGT_OK;
+}
+
+egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 2) return fail(EGT_ERR_STATE, "egt_batch_knn needs a batch handle");
+    if (!k) return fail(EGT_ERR_INVALID_ARG, "null k array");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    for (int c = 0; c < h->nclouds; ++c) {
+        if (k[c] < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+        if (k[c] > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    }
+    return guarded([&] {
+        if (h->search_stale) egt::refresh_search_structure(h);
+        int kmax = 0;
+        for (int c = 0; c < h->nclouds; ++c) {
+            egt::CloudDev& cd = h->clouds[c];
+            cd.k = k[c];
+            float t = thres / cd.extent;  // GridKNNCuda.cu:2324-2325, per cloud
+            cd.thres_n = t * t;
+            kmax = std::max(kmax, k[c]);
+        }
+        egt::run_knn_clouds(h);
+        h->k = kmax;
+        h->csr_valid = false;
+        h->smoothed = false;
+    });
+}
+
+egt_status egt_batch_get_knn_original(egt_handle h, int32_t cloud, int32_t* out) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 2) return fail(EGT_ERR_STATE, "egt_batch_get_knn_original needs a batch handle");
+    if (cloud < 0 || cloud >= h->nclouds) return fail(EGT_ERR_INVALID_ARG, "cloud index out of range");
+    if (!h->have_knn) return fail(EGT_ERR_STATE, "no neighbour rows yet");
+    if (!out) return fail(EGT_ERR_INVALID_ARG, "null output array");
+    return guarded([&] { egt::run_knn_original_cloud(h, cloud, out); });
+}
+
+egt_status egt_batch_cloud_info(egt_handle h, int32_t cloud, int64_t* n, int32_t* k, float* min_extent,
+                                float* max_extent) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (cloud < 0 || cloud >= h->nclouds) return fail(EGT_ERR_INVALID_ARG, "cloud index out of range");
+    const egt::CloudDev& c = h->clouds[cloud];
+    if (n) *n = c.n;
+    if (k) *k = h->have_knn ? c.k : 0;
+    if (min_extent) *min_extent = c.min_extent;
+    if (max_extent) *max_extent = c.max_extent;
+    return EGT_OK;
 }
 
 egt_status egt_destroy(egt_handle h) {

@@ -118,11 +118,17 @@ __device__ inline void eig3_smallest(float a00, float a01, float a02, float a11,
     vz = V[2][idx];
 }
 
-// Fibonacci hashing into a power-of-two table.
-__host__ __device__ inline uint32_t hash_slot(uint32_t code, uint32_t log2cap) {
-    return (code * 2654435761u) >> (32u - log2cap);
+// Cell-table key: cloud index in the high word, level Morton code of the cell in the low word (one table
+// per level holds the cells of every cloud of a batch; a single cloud is cloud 0).
+__host__ __device__ inline uint64_t cell_key(uint32_t cloud, uint32_t code) {
+    return ((uint64_t)cloud << 32) | code;
 }
-constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+
+// Fibonacci hashing of a 64-bit key into a power-of-two table.
+__host__ __device__ inline uint32_t hash_slot(uint64_t key, uint32_t log2cap) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64u - log2cap));
+}
+constexpr uint64_t HASH_EMPTY = ~0ull;
 
 struct LevelDev {
     uint64_t offset;
@@ -131,13 +137,13 @@ struct LevelDev {
 };
 
 // Looks up a level cell; returns (start, count) or count 0 when empty.
-__device__ inline uint2 level_lookup(const uint32_t* __restrict__ hkeys, const uint2* __restrict__ hvals,
-                                     const LevelDev& L, uint32_t code) {
+__device__ inline uint2 level_lookup(const uint64_t* __restrict__ hkeys, const uint2* __restrict__ hvals,
+                                     const LevelDev& L, uint64_t key) {
     const uint32_t mask = (1u << L.log2cap) - 1u;
-    uint32_t s = hash_slot(code, L.log2cap);
+    uint32_t s = hash_slot(key, L.log2cap);
     for (uint32_t probe = 0; probe <= mask; ++probe) {  // tables are at most half full
-        const uint32_t kk = hkeys[L.offset + s];
-        if (kk == code) return hvals[L.offset + s];
+        const uint64_t kk = hkeys[L.offset + s];
+        if (kk == key) return hvals[L.offset + s];
         if (kk == HASH_EMPTY) break;
         s = (s + 1u) & mask;
     }

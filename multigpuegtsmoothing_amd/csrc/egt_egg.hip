@@ -30,19 +30,25 @@ __device__ inline float rowt(float a1, float a2, float a3, float t, float x, flo
 
 // mpts: current points in Morton order (w = sorted index). knn: sorted-space rows, updated in place.
 // mrow: per Morton position, the k slots mapped to Morton positions (-1 for empty). keepm: kept-slot mask.
+// One workgroup per 64-point chunk of one cloud: chunks[b] is the chunk's first Morton position (null: 64 b).
 template <int K>
 __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, const int32_t* __restrict__ minv,
-                                               int32_t* __restrict__ knn, int64_t n, int k, float alpha,
-                                               int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
+                                               int32_t* __restrict__ knn, const CloudDev* __restrict__ clouds,
+                                               const int32_t* __restrict__ mcloud, const int64_t* __restrict__ chunks,
+                                               float alpha, int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
                                                uint32_t* __restrict__ kept) {
     __shared__ float sx[K][EBLK], sy[K][EBLK], sz[K][EBLK];
     const int tid = threadIdx.x;
-    const int64_t m = (int64_t)blockIdx.x * EBLK + tid;
-    if (m >= n) return;
+    const int64_t start = chunks ? chunks[blockIdx.x] : (int64_t)blockIdx.x * EBLK;
+    const CloudDev cd = clouds[mcloud ? mcloud[start] : 0];
+    const int k = cd.k;
+    const int64_t m = start + tid;
+    if (m >= cd.base + cd.n) return;
     const float4 p = mpts[m];
     const int32_t s = __float_as_int(p.w);
-    int32_t* row = knn + (int64_t)s * k;
-    int32_t* mr = mrow + m * k;
+    int32_t* row = knn + cd.row_off + (int64_t)(s - cd.base) * k;
+    int32_t* mr = mrow + cd.row_off + (m - cd.base) * k;
+    const int32_t base = (int32_t)cd.base;
     // staging: the row, then every Morton position, then every coordinate are issued back to back, so the
     // three dependent loads per slot cost three latencies per point rather than three per slot
     uint64_t valid = 0;
@@ -50,7 +56,7 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
 #pragma unroll
     for (int j = 0; j < K; ++j) nbv[j] = j < k ? row[j] : -1;
 #pragma unroll
-    for (int j = 0; j < K; ++j) nbv[j] = nbv[j] >= 0 ? minv[nbv[j]] : -1;
+    for (int j = 0; j < K; ++j) nbv[j] = nbv[j] >= 0 ? minv[nbv[j] + base] : -1;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (nbv[j] >= 0) {
@@ -125,7 +131,8 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
 }
 
 __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mrow, const uint64_t* __restrict__ keepm,
-                                                  int64_t n, int k, const uint32_t* __restrict__ off,
+                                                  int64_t n, const CloudDev* __restrict__ clouds,
+                                                  const int32_t* __restrict__ mcloud, const uint32_t* __restrict__ off,
                                                   int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
                                                   uint32_t total) {
     const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -134,7 +141,8 @@ __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mr
     rowptr[m] = (int32_t)o;
     if (m == n - 1) rowptr[n] = (int32_t)total;
     uint64_t keep = keepm[m];
-    const int32_t* mr = mrow + m * k;
+    const CloudDev& cd = clouds[mcloud ? mcloud[m] : 0];
+    const int32_t* mr = mrow + cd.row_off + (m - cd.base) * cd.k;
     while (keep) {
         const int j = __builtin_ctzll(keep);
         keep &= keep - 1ull;
@@ -145,16 +153,20 @@ __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mr
 // Rows -> Morton-order slots without filtering (normals requested straight after the neighbour search).
 __global__ void __launch_bounds__(256) k_rows_to_mrow(const int32_t* __restrict__ morder,
                                                       const int32_t* __restrict__ minv,
-                                                      const int32_t* __restrict__ knn, int64_t n, int k,
-                                                      int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
-                                                      uint32_t* __restrict__ kept) {
+                                                      const int32_t* __restrict__ knn, int64_t n,
+                                                      const CloudDev* __restrict__ clouds,
+                                                      const int32_t* __restrict__ mcloud, int32_t* __restrict__ mrow,
+                                                      uint64_t* __restrict__ keepm, uint32_t* __restrict__ kept) {
     const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (m >= n) return;
-    const int32_t* row = knn + (int64_t)morder[m] * k;
+    const CloudDev& cd = clouds[mcloud ? mcloud[m] : 0];
+    const int k = cd.k;
+    const int32_t* row = knn + cd.row_off + (int64_t)(morder[m] - cd.base) * k;
+    int32_t* mr = mrow + cd.row_off + (m - cd.base) * k;
     uint64_t valid = 0;
     for (int j = 0; j < k; ++j) {
         const int32_t nb = row[j];
-        mrow[m * k + j] = nb >= 0 ? minv[nb] : -1;
+        mr[j] = nb >= 0 ? minv[nb + cd.base] : -1;
         if (nb >= 0) valid |= 1ull << j;
     }
     keepm[m] = valid;
@@ -168,7 +180,9 @@ void finish_csr(egt_ctx* h, const int32_t* mrow, const uint64_t* keepm, const ui
     exclusive_scan_u32(kept, off, n, h->stream, h->scratch, &total);
     h->rowptr.alloc(n + 1);
     h->col.alloc(std::max<uint32_t>(total, 1));
-    k_csr_fill<<<ceil_div(n, 256), 256, 0, h->stream>>>(mrow, keepm, n, h->k, off, h->rowptr.p, h->col.p, total);
+    k_csr_fill<<<ceil_div(n, 256), 256, 0, h->stream>>>(mrow, keepm, n, h->dclouds.p,
+                                                         h->nclouds > 1 ? h->mcloud.p : nullptr, off, h->rowptr.p,
+                                                         h->col.p, total);
     LAUNCH_CHECK();
     h->nnz = total;
     h->csr_valid = true;
@@ -176,13 +190,20 @@ void finish_csr(egt_ctx* h, const int32_t* mrow, const uint64_t* keepm, const ui
 
 }  // namespace
 
+int64_t row_elems(const egt_ctx* h) {
+    int64_t t = 0;
+    for (const CloudDev& c : h->clouds) t += c.n * c.k;
+    return t;
+}
+
 void build_csr_from_rows(egt_ctx* h) {
     const int64_t n = h->n;
-    const int k = h->k;
+    sync_single_cloud(h);
     uint32_t* kept = h->scratch.get<uint32_t>(9, n);
     uint64_t* keepm = h->scratch.get<uint64_t>(17, n);
-    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)n * k);
-    k_rows_to_mrow<<<ceil_div(n, 256), 256, 0, h->stream>>>(h->morder.p, h->minv.p, h->knn.p, n, k, mrow, keepm,
+    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)std::max<int64_t>(row_elems(h), 1));
+    k_rows_to_mrow<<<ceil_div(n, 256), 256, 0, h->stream>>>(h->morder.p, h->minv.p, h->knn.p, n, h->dclouds.p,
+                                                            h->nclouds > 1 ? h->mcloud.p : nullptr, mrow, keepm,
                                                             kept);
     LAUNCH_CHECK();
     finish_csr(h, mrow, keepm, kept);
@@ -190,23 +211,30 @@ void build_csr_from_rows(egt_ctx* h) {
 
 void run_egg(egt_ctx* h, float alpha) {
     const int64_t n = h->n;
-    const int k = h->k;
     hipStream_t s = h->stream;
+    sync_single_cloud(h);
     uint32_t* kept = h->scratch.get<uint32_t>(9, n);
     uint64_t* keepm = h->scratch.get<uint64_t>(17, n);
-    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)n * k);
+    int32_t* mrow = h->scratch.get<int32_t>(18, (size_t)std::max<int64_t>(row_elems(h), 1));
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
-    const dim3 grid(ceil_div(n, EBLK)), blk(EBLK);
-#define EGG_ARGS h->mpts.p, h->minv.p, h->knn.p, n, k, alpha, mrow, keepm, kept
-    if (k <= 16) hipLaunchKernelGGL(k_egg<16>, grid, blk, 0, s, EGG_ARGS);
-    else if (k <= 32) hipLaunchKernelGGL(k_egg<32>, grid, blk, 0, s, EGG_ARGS);
-    else if (k <= 40) hipLaunchKernelGGL(k_egg<40>, grid, blk, 0, s, EGG_ARGS);
-    else hipLaunchKernelGGL(k_egg<64>, grid, blk, 0, s, EGG_ARGS);
+    // one launch per neighbour-count class over its chunk list (one cloud: every chunk, class of k)
+    std::vector<WorkClass> classes = h->work_classes;
+    if (h->nclouds == 1) classes.assign(1, WorkClass{h->clouds[0].k, 0, (n + EBLK - 1) / EBLK});
+    const int32_t* mcl = h->nclouds > 1 ? h->mcloud.p : nullptr;
+    for (const WorkClass& wc : classes) {
+        const int64_t* chunks = h->nclouds > 1 ? h->work_lists.p + wc.first : nullptr;
+        const dim3 grid((unsigned)wc.count), blk(EBLK);
+#define EGG_ARGS h->mpts.p, h->minv.p, h->knn.p, h->dclouds.p, mcl, chunks, alpha, mrow, keepm, kept
+        if (wc.K <= 16) hipLaunchKernelGGL(k_egg<16>, grid, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 32) hipLaunchKernelGGL(k_egg<32>, grid, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 40) hipLaunchKernelGGL(k_egg<40>, grid, blk, 0, s, EGG_ARGS);
+        else hipLaunchKernelGGL(k_egg<64>, grid, blk, 0, s, EGG_ARGS);
 #undef EGG_ARGS
-    LAUNCH_CHECK();
+        LAUNCH_CHECK();
+    }
     finish_csr(h, mrow, keepm, kept);
     h->stats.egg_kept = h->nnz;
     HIP_CHECK(hipEventRecord(e1, s));

@@ -58,14 +58,22 @@ __global__ void __launch_bounds__(BLK) k_minmax_partial(const float* __restrict_
     }
 }
 
-// normalize_points GridKNNCuda.cu:840-849 fused with get_buckets_no_binary :74-88.
+// normalize_points GridKNNCuda.cu:840-849 fused with get_buckets_no_binary :74-88. With icloud (batch) each
+// point uses its own cloud's cube; key_in (batch) keeps an unsorted copy of the keys.
 __global__ void __launch_bounds__(BLK) k_normalize_key(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ z, int64_t n, float mn, float d,
-                                                        float a, float ba, int B, int log2B, float* __restrict__ nx,
-                                                        float* __restrict__ ny, float* __restrict__ nz,
-                                                        uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
+                                                        const int32_t* __restrict__ icloud,
+                                                        const CloudDev* __restrict__ clouds, float a, float ba, int B,
+                                                        int log2B, float* __restrict__ nx, float* __restrict__ ny,
+                                                        float* __restrict__ nz, uint32_t* __restrict__ key,
+                                                        uint32_t* __restrict__ key_in, uint32_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
+    if (icloud) {
+        const CloudDev& c = clouds[icloud[i]];
+        mn = c.min_extent;
+        d = c.extent;
+    }
     const float xn = (x[i] - mn) / d, yn = (y[i] - mn) / d, zn = (z[i] - mn) / d;
     nx[i] = xn;
     ny[i] = yn;
@@ -73,21 +81,105 @@ __global__ void __launch_bounds__(BLK) k_normalize_key(const float* __restrict__
     const uint32_t cx = (uint32_t)cell_of(xn, a, ba, B), cy = (uint32_t)cell_of(yn, a, ba, B),
                    cz = (uint32_t)cell_of(zn, a, ba, B);
     key[i] = cx | (cy << log2B) | (cz << (2 * log2B));
+    if (key_in) key_in[i] = key[i];
     idx[i] = (uint32_t)i;
+}
+
+// Cloud of every input point (binary search over the input offsets).
+__global__ void __launch_bounds__(BLK) k_cloud_of(const int64_t* __restrict__ off, int nclouds, int64_t n,
+                                                   int32_t* __restrict__ cloud) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    int lo = 0, hi = nclouds - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    cloud[i] = lo;
+}
+
+// Order-preserving float <-> int map, so float min/max can use integer atomics.
+__device__ inline int f2o(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+
+// Per-cloud min/max of x, y, z (fold starting at +-1e8, as the reference's thrust reductions
+// GridKNNCuda.cu:1042-1059) plus a count of non-finite inputs. A wave whose 64 points share one cloud
+// reduces first and issues one atomic per value.
+__global__ void __launch_bounds__(BLK) k_cloud_minmax(const float* __restrict__ x, const float* __restrict__ y,
+                                                       const float* __restrict__ z, const int32_t* __restrict__ icloud,
+                                                       int64_t n, int* __restrict__ mm, uint32_t* __restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const bool act = i < n;
+    const int c = act ? icloud[i] : -1;
+    float v[6] = {1e8f, -1e8f, 1e8f, -1e8f, 1e8f, -1e8f};
+    uint32_t nb = 0;
+    if (act) {
+        const float p[3] = {x[i], y[i], z[i]};
+        nb = !(__builtin_isfinite(p[0]) && __builtin_isfinite(p[1]) && __builtin_isfinite(p[2]));
+        for (int ax = 0; ax < 3; ++ax) {
+            v[2 * ax] = fminf(v[2 * ax], p[ax]);
+            v[2 * ax + 1] = fmaxf(v[2 * ax + 1], p[ax]);
+        }
+    }
+    const int c0 = __shfl(c, 0, 64);
+    const bool uniform = __ballot(c == c0) == ~0ull;
+    if (uniform) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+#pragma unroll
+            for (int j = 0; j < 6; j += 2) {
+                v[j] = fminf(v[j], __shfl_xor(v[j], d, 64));
+                v[j + 1] = fmaxf(v[j + 1], __shfl_xor(v[j + 1], d, 64));
+            }
+            nb += __shfl_xor(nb, d, 64);
+        }
+        if (__lane_id() == 0) {
+            for (int j = 0; j < 6; j += 2) {
+                atomicMin(&mm[c0 * 6 + j], f2o(v[j]));
+                atomicMax(&mm[c0 * 6 + j + 1], f2o(v[j + 1]));
+            }
+            if (nb) atomicAdd(bad, nb);
+        }
+    } else if (act) {
+        for (int j = 0; j < 6; j += 2) {
+            atomicMin(&mm[c * 6 + j], f2o(v[j]));
+            atomicMax(&mm[c * 6 + j + 1], f2o(v[j + 1]));
+        }
+        if (nb) atomicAdd(bad, nb);
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_minmax_init(int* __restrict__ mm, int nclouds) {
+    const int i = blockIdx.x * BLK + threadIdx.x;
+    if (i < nclouds * 6) mm[i] = f2o((i & 1) ? -1e8f : 1e8f);
+}
+
+// Cloud of each element of an index list (second, stable sort pass by cloud).
+__global__ void __launch_bounds__(BLK) k_gather_cloud(const int32_t* __restrict__ cloud_of,
+                                                       const uint32_t* __restrict__ idx, int64_t n,
+                                                       uint32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)cloud_of[idx[i]];
 }
 
 // thrust::gather into sorted order, GridKNNCuda.cu:1114-1128.
 __global__ void __launch_bounds__(BLK) k_gather_sorted(const float* __restrict__ nx, const float* __restrict__ ny,
                                                         const float* __restrict__ nz, const uint32_t* __restrict__ skey,
+                                                        const uint32_t* __restrict__ key_in,
+                                                        const int32_t* __restrict__ icloud,
                                                         const uint32_t* __restrict__ sidx, int64_t n,
                                                         float* __restrict__ x, float* __restrict__ y,
                                                         float* __restrict__ z, int32_t* __restrict__ perm,
-                                                        int32_t* __restrict__ key) {
+                                                        int32_t* __restrict__ key, int32_t* __restrict__ scloud) {
     const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (s >= n) return;
     const uint32_t o = sidx[s];
     perm[s] = (int32_t)o;
-    key[s] = (int32_t)skey[s];
+    key[s] = (int32_t)(key_in ? key_in[o] : skey[s]);
+    if (scloud) scloud[s] = icloud[o];
     x[s] = nx[o];
     y[s] = ny[o];
     z[s] = nz[o];
@@ -104,8 +196,10 @@ __global__ void __launch_bounds__(BLK) k_morton_keys(const int32_t* __restrict__
 
 __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ z, const int32_t* __restrict__ key,
+                                                        const int32_t* __restrict__ scloud,
                                                         const uint32_t* __restrict__ order, int64_t n, int log2B,
                                                         float4* __restrict__ mpts, uint32_t* __restrict__ mcell,
+                                                        uint32_t* __restrict__ mcode, int32_t* __restrict__ mcloud,
                                                         int32_t* __restrict__ morder, int32_t* __restrict__ minv) {
     const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (m >= n) return;
@@ -114,14 +208,20 @@ __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__
     minv[s] = (int32_t)m;
     mpts[m] = make_float4(x[s], y[s], z[s], __int_as_float((int)s));
     const uint32_t k = (uint32_t)key[s], msk = (1u << log2B) - 1u;
-    mcell[m] = (k & msk) | (((k >> log2B) & msk) << 10) | ((k >> (2 * log2B)) << 20);
+    const uint32_t cx = k & msk, cy = (k >> log2B) & msk, cz = k >> (2 * log2B);
+    mcell[m] = cx | (cy << 10) | (cz << 20);
+    mcode[m] = morton_code(cx, cy, cz);
+    if (mcloud) mcloud[m] = scloud[s];
 }
 
-__global__ void __launch_bounds__(BLK) k_level_flags(const uint32_t* __restrict__ code, int64_t n, int shift,
+__global__ void __launch_bounds__(BLK) k_level_flags(const uint32_t* __restrict__ code,
+                                                      const int32_t* __restrict__ mcloud, int64_t n, int shift,
                                                       uint32_t* __restrict__ flag) {
     const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (m >= n) return;
-    flag[m] = (m == 0 || (code[m] >> shift) != (code[m - 1] >> shift)) ? 1u : 0u;
+    bool first = m == 0 || (code[m] >> shift) != (code[m - 1] >> shift);
+    if (mcloud && m > 0) first = first || mcloud[m] != mcloud[m - 1];
+    flag[m] = first ? 1u : 0u;
 }
 
 __global__ void __launch_bounds__(BLK) k_level_starts(const uint32_t* __restrict__ flag,
@@ -133,18 +233,21 @@ __global__ void __launch_bounds__(BLK) k_level_starts(const uint32_t* __restrict
 }
 
 __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict__ starts, int64_t cells,
-                                                       int64_t n, const uint32_t* __restrict__ code, int shift,
-                                                       uint32_t* __restrict__ hkeys, uint2* __restrict__ hvals,
-                                                       uint64_t offset, uint32_t log2cap, int* __restrict__ maxc) {
+                                                       int64_t n, const uint32_t* __restrict__ code,
+                                                       const int32_t* __restrict__ mcloud, int shift,
+                                                       unsigned long long* __restrict__ hkeys,
+                                                       uint2* __restrict__ hvals, uint64_t offset, uint32_t log2cap,
+                                                       int* __restrict__ maxc) {
     const int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (c >= cells) return;
     const uint32_t st = starts[c];
     const uint32_t en = c + 1 < cells ? starts[c + 1] : (uint32_t)n;
-    const uint32_t cc = code[st] >> shift;
+    const unsigned long long cc = cell_key(mcloud ? (uint32_t)mcloud[st] : 0u, code[st] >> shift);
     const uint32_t mask = (1u << log2cap) - 1u;
     uint32_t s = hash_slot(cc, log2cap);
+    // a table has at least twice as many slots as cells, so an empty slot is always found
     for (;;) {
-        const uint32_t prev = atomicCAS(&hkeys[offset + s], HASH_EMPTY, cc);
+        const unsigned long long prev = atomicCAS(&hkeys[offset + s], (unsigned long long)HASH_EMPTY, cc);
         if (prev == HASH_EMPTY) {
             hvals[offset + s] = make_uint2(st, en - st);
             break;
@@ -154,17 +257,24 @@ __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict
     atomicMax(maxc, (int)(en - st));
 }
 
-__global__ void __launch_bounds__(BLK) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+__global__ void __launch_bounds__(BLK) k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLK) p[i] = v;
 }
 
 // produce_output_GKNN + rearrange_output, getOutputPoints.cu:9-30, into separate output buffers.
 __global__ void __launch_bounds__(BLK) k_output(const float* __restrict__ x, const float* __restrict__ y,
                                                  const float* __restrict__ z, const int32_t* __restrict__ perm,
-                                                 int64_t n, float mn, float d, float* __restrict__ ox,
-                                                 float* __restrict__ oy, float* __restrict__ oz) {
+                                                 const int32_t* __restrict__ scloud,
+                                                 const CloudDev* __restrict__ clouds, int64_t n, float mn, float d,
+                                                 float* __restrict__ ox, float* __restrict__ oy,
+                                                 float* __restrict__ oz) {
     const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (s >= n) return;
+    if (scloud) {
+        const CloudDev& c = clouds[scloud[s]];
+        mn = c.min_extent;
+        d = c.extent;
+    }
     const int32_t o = perm[s];
     ox[o] = x[s] * d + mn;
     oy[o] = y[s] * d + mn;
@@ -183,10 +293,85 @@ __global__ void __launch_bounds__(BLK) k_knn_original(const int32_t* __restrict_
     out[(int64_t)perm[s] * k + j] = v >= 0 ? perm[v] : -1;
 }
 
+// Rows of one cloud of a batch, in that cloud's own original order and index space.
+__global__ void __launch_bounds__(BLK) k_knn_original_cloud(const int32_t* __restrict__ knn,
+                                                             const int32_t* __restrict__ perm, CloudDev c,
+                                                             int64_t in_base, int32_t* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (t >= c.n * c.k) return;
+    const int64_t r = t / c.k;
+    const int j = (int)(t - r * c.k);
+    const int32_t v = knn[c.row_off + t];
+    const int64_t s = c.base + r;
+    out[(perm[s] - in_base) * c.k + j] = v >= 0 ? (int32_t)(perm[c.base + v] - in_base) : -1;
+}
+
 int ilog2(int v) {
     int l = 0;
     while ((1 << l) < v) ++l;
     return l;
+}
+
+}  // namespace
+
+namespace {
+
+int ceil_log2(int64_t v) {
+    int l = 0;
+    while ((1ll << l) < v) ++l;
+    return l;
+}
+
+void upload_clouds(egt_ctx* h) {
+    h->dclouds.alloc(h->clouds.size());
+    HIP_CHECK(hipMemcpyAsync(h->dclouds.p, h->clouds.data(), h->clouds.size() * sizeof(CloudDev),
+                             hipMemcpyHostToDevice, h->stream));
+}
+
+// Bounding cube of one cloud from the per-axis min/max, GridKNNCuda.cu:1042-1077.
+void cube_from_minmax(const float v[6], float& mn, float& mx) {
+    const float minx = v[0] - 1e-6f, maxx = v[1] + 1e-6f, miny = v[2] - 1e-6f, maxy = v[3] + 1e-6f;
+    const float minz = v[4] - 1e-6f, maxz = v[5] + 1e-6f;
+    mn = std::min(std::min(minx, miny), minz);
+    mx = std::max(std::max(maxx, maxy), maxz);
+}
+
+// Normalize, key, stable sort (by cloud, then key) and gather into sorted space. icloud is null for one
+// cloud.
+void sort_into_sorted_space(egt_ctx* h, const float* dx, const float* dy, const float* dz, const int32_t* icloud) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    Scratch& sc = h->scratch;
+    const int log2B = ilog2(h->B);
+    const float a = -h->eps, b = 1.0f + h->eps, ba = b - a;
+    float* tx = sc.get<float>(9, n);
+    float* ty = sc.get<float>(10, n);
+    float* tz = sc.get<float>(11, n);
+    uint32_t* skey = sc.get<uint32_t>(12, n);
+    uint32_t* sidx = sc.get<uint32_t>(13, n);
+    uint32_t* key_in = icloud ? sc.get<uint32_t>(23, n) : nullptr;
+    k_normalize_key<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, n, h->min_extent, h->max_extent - h->min_extent,
+                                                      icloud, h->dclouds.p, a, ba, h->B, log2B, tx, ty, tz, skey,
+                                                      key_in, sidx);
+    LAUNCH_CHECK();
+    radix_sort_pairs(skey, sidx, n, 3 * log2B, s, sc);
+    if (icloud) {
+        // second stable pass by cloud: sorted by (cloud, key), ties in input order
+        uint32_t* cl = sc.get<uint32_t>(22, n);
+        k_gather_cloud<<<ceil_div(n, BLK), BLK, 0, s>>>(icloud, sidx, n, cl);
+        LAUNCH_CHECK();
+        radix_sort_pairs(cl, sidx, n, std::max(1, ceil_log2(h->nclouds)), s, sc);
+        h->scloud.alloc(n);
+    }
+    h->x.alloc(n);
+    h->y.alloc(n);
+    h->z.alloc(n);
+    h->perm.alloc(n);
+    h->key.alloc(n);
+    k_gather_sorted<<<ceil_div(n, BLK), BLK, 0, s>>>(tx, ty, tz, skey, key_in, icloud, sidx, n, h->x.p, h->y.p,
+                                                      h->z.p, h->perm.p, h->key.p,
+                                                      icloud ? h->scloud.p : nullptr);
+    LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -219,34 +404,74 @@ void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz) {
             v[c] = std::min(v[c], hp[(size_t)b * 6 + c]);
             v[c + 1] = std::max(v[c + 1], hp[(size_t)b * 6 + c + 1]);
         }
-    const float minx = v[0] - 1e-6f, maxx = v[1] + 1e-6f, miny = v[2] - 1e-6f, maxy = v[3] + 1e-6f;
-    const float minz = v[4] - 1e-6f, maxz = v[5] + 1e-6f;
-    const float mn = std::min(std::min(minx, miny), minz);
-    const float mx = std::max(std::max(maxx, maxy), maxz);
+    float mn, mx;
+    cube_from_minmax(v, mn, mx);
     h->min_extent = mn;
     h->max_extent = mx;
-    const float d = mx - mn;
+    h->nclouds = 1;
+    h->clouds.assign(1, CloudDev{});
+    h->clouds[0].base = 0;
+    h->clouds[0].n = n;
+    h->clouds[0].min_extent = mn;
+    h->clouds[0].extent = mx - mn;
+    h->clouds[0].max_extent = mx;
+    upload_clouds(h);
+    sort_into_sorted_space(h, dx, dy, dz, nullptr);
+    build_search_structure(h);
+    HIP_CHECK(hipEventRecord(e1, s));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipEventElapsedTime(&h->stats.grid_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
 
-    // normalize + key + stable sort
-    const int log2B = ilog2(h->B);
-    const float a = -h->eps, b = 1.0f + h->eps, ba = b - a;
-    float* tx = sc.get<float>(9, n);
-    float* ty = sc.get<float>(10, n);
-    float* tz = sc.get<float>(11, n);
-    uint32_t* skey = sc.get<uint32_t>(12, n);
-    uint32_t* sidx = sc.get<uint32_t>(13, n);
-    k_normalize_key<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, n, mn, d, a, ba, h->B, log2B, tx, ty, tz, skey,
-                                                      sidx);
+void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float* dz, const int64_t* offsets,
+                      int nclouds) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    Scratch& sc = h->scratch;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    h->nclouds = nclouds;
+    h->in_off.alloc(nclouds + 1);
+    HIP_CHECK(hipMemcpyAsync(h->in_off.p, offsets, (nclouds + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    int32_t* icloud = sc.get<int32_t>(21, n);
+    k_cloud_of<<<ceil_div(n, BLK), BLK, 0, s>>>(h->in_off.p, nclouds, n, icloud);
     LAUNCH_CHECK();
-    radix_sort_pairs(skey, sidx, n, 3 * log2B, s, sc);
-    h->x.alloc(n);
-    h->y.alloc(n);
-    h->z.alloc(n);
-    h->perm.alloc(n);
-    h->key.alloc(n);
-    k_gather_sorted<<<ceil_div(n, BLK), BLK, 0, s>>>(tx, ty, tz, skey, sidx, n, h->x.p, h->y.p, h->z.p, h->perm.p,
-                                                      h->key.p);
+    int* mm = sc.get<int>(8, (size_t)nclouds * 6 + 1);
+    uint32_t* dbad = reinterpret_cast<uint32_t*>(mm + (size_t)nclouds * 6);
+    k_minmax_init<<<ceil_div(nclouds * 6, BLK), BLK, 0, s>>>(mm, nclouds);
     LAUNCH_CHECK();
+    HIP_CHECK(hipMemsetAsync(dbad, 0, sizeof(uint32_t), s));
+    k_cloud_minmax<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, icloud, n, mm, dbad);
+    LAUNCH_CHECK();
+    std::vector<int> hm((size_t)nclouds * 6 + 1);
+    HIP_CHECK(hipMemcpyAsync(hm.data(), mm, hm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (hm[(size_t)nclouds * 6]) throw ArgError{EGT_ERR_INVALID_ARG, "input coordinates contain NaN or Inf"};
+    h->clouds.assign(nclouds, CloudDev{});
+    for (int c = 0; c < nclouds; ++c) {
+        float v[6];
+        for (int j = 0; j < 6; ++j) {
+            int iv = hm[(size_t)c * 6 + j];
+            iv = iv >= 0 ? iv : iv ^ 0x7FFFFFFF;
+            std::memcpy(&v[j], &iv, 4);
+        }
+        float mn, mx;
+        cube_from_minmax(v, mn, mx);
+        CloudDev& cd = h->clouds[c];
+        cd.base = offsets[c];
+        cd.n = offsets[c + 1] - offsets[c];
+        cd.min_extent = mn;
+        cd.extent = mx - mn;
+        cd.max_extent = mx;
+    }
+    h->min_extent = h->clouds[0].min_extent;
+    h->max_extent = h->clouds[0].max_extent;
+    upload_clouds(h);
+    sort_into_sorted_space(h, dx, dy, dz, icloud);
     build_search_structure(h);
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
@@ -260,6 +485,7 @@ void build_search_structure(egt_ctx* h) {
     hipStream_t s = h->stream;
     Scratch& sc = h->scratch;
     const int log2B = ilog2(h->B);
+    const bool batch = h->nclouds > 1;
     DBuf<uint32_t>& code = sc.bufs[12];
     DBuf<uint32_t>& order = sc.bufs[13];
     code.alloc(n);
@@ -267,17 +493,26 @@ void build_search_structure(egt_ctx* h) {
     k_morton_keys<<<ceil_div(n, BLK), BLK, 0, s>>>(h->key.p, n, log2B, code.p, order.p);
     LAUNCH_CHECK();
     radix_sort_pairs(code.p, order.p, n, 3 * log2B, s, sc);
+    if (batch) {
+        // second stable pass by cloud: Morton order within each cloud, clouds in input order
+        uint32_t* cl = sc.get<uint32_t>(22, n);
+        k_gather_cloud<<<ceil_div(n, BLK), BLK, 0, s>>>(h->scloud.p, order.p, n, cl);
+        LAUNCH_CHECK();
+        radix_sort_pairs(cl, order.p, n, std::max(1, ceil_log2(h->nclouds)), s, sc);
+        h->mcloud.alloc(n);
+    }
     h->mpts.alloc(n);
     h->mcell.alloc(n);
     h->mcode.alloc(n);
     h->morder.alloc(n);
     h->minv.alloc(n);
-    k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, order.p, n, log2B, h->mpts.p,
-                                                      h->mcell.p, h->morder.p, h->minv.p);
+    k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, batch ? h->scloud.p : nullptr,
+                                                      order.p, n, log2B, h->mpts.p, h->mcell.p, h->mcode.p,
+                                                      batch ? h->mcloud.p : nullptr, h->morder.p, h->minv.p);
     LAUNCH_CHECK();
-    HIP_CHECK(hipMemcpyAsync(h->mcode.p, code.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    const int32_t* mcl = batch ? h->mcloud.p : nullptr;
 
-    // level tables: level l groups 2^l fine cells per axis (Morton code >> 3l)
+    // level tables: level l groups 2^l fine cells per axis (Morton code >> 3l), per cloud
     h->nlev = log2B + 1;
     DBuf<uint32_t>& flag = sc.bufs[9];
     DBuf<uint32_t>& seg = sc.bufs[10];
@@ -288,7 +523,7 @@ void build_search_structure(egt_ctx* h) {
     std::vector<uint32_t> cells(h->nlev);
     uint64_t total = 0;
     for (int l = 0; l < h->nlev; ++l) {
-        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, n, 3 * l, flag.p);
+        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
         LAUNCH_CHECK();
         exclusive_scan_u32(flag.p, seg.p, n, s, sc, &cells[l]);
         uint32_t log2cap = 6;
@@ -300,20 +535,20 @@ void build_search_structure(egt_ctx* h) {
     }
     h->hkeys.alloc(total);
     h->hvals.alloc(total);
-    k_fill_u32<<<1024, BLK, 0, s>>>(h->hkeys.p, total, HASH_EMPTY);
+    k_fill_u64<<<1024, BLK, 0, s>>>(h->hkeys.p, total, HASH_EMPTY);
     LAUNCH_CHECK();
     DBuf<uint32_t>& dmax = sc.bufs[14];
     dmax.alloc(h->nlev);
     HIP_CHECK(hipMemsetAsync(dmax.p, 0, h->nlev * sizeof(uint32_t), s));
     for (int l = 0; l < h->nlev; ++l) {
-        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, n, 3 * l, flag.p);
+        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
         LAUNCH_CHECK();
         exclusive_scan_u32(flag.p, seg.p, n, s, sc, nullptr);
         k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts.p);
         LAUNCH_CHECK();
         k_level_insert<<<std::max(1, ceil_div(cells[l], BLK)), BLK, 0, s>>>(
-            starts.p, cells[l], n, h->mcode.p, 3 * l, h->hkeys.p, h->hvals.p, h->lev[l].offset, h->lev[l].log2cap,
-            reinterpret_cast<int*>(dmax.p) + l);
+            starts.p, cells[l], n, h->mcode.p, mcl, 3 * l, reinterpret_cast<unsigned long long*>(h->hkeys.p),
+            h->hvals.p, h->lev[l].offset, h->lev[l].log2cap, reinterpret_cast<int*>(dmax.p) + l);
         LAUNCH_CHECK();
     }
     std::vector<int32_t> mx(h->nlev);
@@ -349,8 +584,9 @@ void run_output(egt_ctx* h, float* x, float* y, float* z) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     float* ox = h->scratch.get<float>(9, 3 * n);
-    k_output<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->perm.p, n, h->min_extent,
-                                              h->max_extent - h->min_extent, ox, ox + n, ox + 2 * n);
+    k_output<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->perm.p,
+                                              h->nclouds > 1 ? h->scloud.p : nullptr, h->dclouds.p, n,
+                                              h->min_extent, h->max_extent - h->min_extent, ox, ox + n, ox + 2 * n);
     LAUNCH_CHECK();
     HIP_CHECK(hipMemcpyAsync(x, ox, n * sizeof(float), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(y, ox + n, n * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -365,6 +601,17 @@ void run_knn_original(egt_ctx* h, int32_t* out) {
     k_knn_original<<<ceil_div(n * k, BLK), BLK, 0, s>>>(h->knn.p, h->perm.p, n, (int)k, tmp);
     LAUNCH_CHECK();
     HIP_CHECK(hipMemcpyAsync(out, tmp, n * k * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void run_knn_original_cloud(egt_ctx* h, int cloud, int32_t* out) {
+    const CloudDev& c = h->clouds[cloud];
+    hipStream_t s = h->stream;
+    const int64_t cnt = c.n * c.k;
+    int32_t* tmp = h->scratch.get<int32_t>(9, std::max<int64_t>(cnt, 1));
+    k_knn_original_cloud<<<std::max(1, ceil_div(cnt, BLK)), BLK, 0, s>>>(h->knn.p, h->perm.p, c, c.base, tmp);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(out, tmp, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 

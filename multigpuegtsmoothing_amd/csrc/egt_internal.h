@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "../../include/egt.h"
+#include "egt_device.h"
 
 #define EGT_MAX_LEVELS 11
 
@@ -64,6 +65,25 @@ struct LevelTable {
     int64_t cells = 0;
 };
 
+// Per-cloud parameters (a single-cloud handle has one entry). Sorted space and Morton order both hold the
+// clouds one after another in input order, so a cloud's points are positions [base, base + n) in both.
+struct CloudDev {
+    int64_t base;     // first sorted index / Morton position of the cloud
+    int64_t n;        // point count
+    int64_t row_off;  // first element of the cloud's neighbour rows (rows are n x k, cloud after cloud)
+    int32_t k;        // neighbour count of the last search
+    float thres_n;    // squared search radius in normalized units
+    float min_extent, extent;  // de-normalization: x = x' * extent + min_extent
+    int32_t ref_level;         // learned proposal: feature level
+    float max_extent;          // bounding cube upper end (as computed, not min_extent + extent)
+};
+
+// A neighbour-count class of a batch: its K template and its range in the chunk work list.
+struct WorkClass {
+    int K;
+    int64_t first, count;
+};
+
 // Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code (15 = k-th
 // distance of the last search), 16 host-input staging, 17-21 EGG / Taubin, 24-26 learned proposal.
 struct Scratch {
@@ -101,10 +121,21 @@ struct egt_ctx {
     egt::DBuf<int32_t> morder;   // Morton position -> sorted index
     egt::DBuf<int32_t> minv;     // sorted index -> Morton position
     bool search_stale = false;   // points moved since the search structure was built
-    egt::DBuf<uint32_t> hkeys;   // all level tables
+    egt::DBuf<uint64_t> hkeys;   // all level tables (keys: cell_key(cloud, level code))
     egt::DBuf<uint2> hvals;
     egt::LevelTable lev[EGT_MAX_LEVELS];
     int nlev = 0;
+    egt::DBuf<egt::LevelDev> dlev;  // device copy of the level descriptors (kNN kernel)
+
+    // clouds (a batch handle holds several independent clouds; a plain handle holds one)
+    int nclouds = 1;
+    std::vector<egt::CloudDev> clouds;  // host copy
+    egt::DBuf<egt::CloudDev> dclouds;   // device copy
+    egt::DBuf<int32_t> mcloud;          // cloud of each Morton position (batch only, else empty)
+    egt::DBuf<int32_t> scloud;          // cloud of each sorted index (batch only)
+    egt::DBuf<int64_t> in_off;          // input offsets of the clouds (batch only)
+    egt::DBuf<int64_t> work_lists;      // kNN / EGG work lists of 64-point chunks (batch only)
+    std::vector<egt::WorkClass> work_classes;
 
     // EGG adjacency as CSR in Morton order (rows and column indices are Morton positions)
     egt::DBuf<int32_t> rowptr, col;
@@ -120,8 +151,9 @@ struct egt_ctx {
     egt::DBuf<int8_t> qlevel;
 
     // options
-    int start_level = -1;
+    int start_level = 0;  // reference schedule: every query starts at its own level-0 cell
     int learned = 0;
+    int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;
     int maxc = 100000;
 
@@ -152,12 +184,16 @@ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
 
 // grid (egt_grid.hip)
 void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz);
+void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float* dz, const int64_t* offsets,
+                      int nclouds);
 void build_search_structure(egt_ctx* h);
 void refresh_search_structure(egt_ctx* h);
 
 // kNN (egt_knn.hip)
 void run_knn(egt_ctx* h, int k, float thres_n);
+void run_knn_clouds(egt_ctx* h);
 int auto_start_level(const egt_ctx* h, int k);
+int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms);
 
 // learned proposal (egt_mlp.hip)
 int mlp_param_count();
@@ -189,6 +225,16 @@ void run_normals(egt_ctx* h, float* nx_host, float* ny_host, float* nz_host);
 // output (egt_grid.hip)
 void run_output(egt_ctx* h, float* x, float* y, float* z);
 void run_knn_original(egt_ctx* h, int32_t* out);
+void run_knn_original_cloud(egt_ctx* h, int cloud, int32_t* out);
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// One-cloud handles: mirror the handle's k into the cloud table (rows start at 0) and upload it.
+inline void sync_single_cloud(egt_ctx* h) {
+    if (h->nclouds != 1) return;
+    h->clouds[0].k = h->k;
+    h->clouds[0].row_off = 0;
+    h->dclouds.alloc(1);
+    HIP_CHECK(hipMemcpyAsync(h->dclouds.p, h->clouds.data(), sizeof(CloudDev), hipMemcpyHostToDevice, h->stream));
+}
 }  // namespace egt

@@ -27,18 +27,25 @@ constexpr uint64_t KEY_EMPTY = ~0ull;
 struct KnnParams {
     const float4* __restrict__ mpts;
     const uint32_t* __restrict__ mcell;
-    const uint32_t* __restrict__ hkeys;
+    const uint64_t* __restrict__ hkeys;
     const uint2* __restrict__ hvals;
     const int32_t* __restrict__ key;       // sorted-space fine key (member cell coordinates)
     const int8_t* __restrict__ qlevel;     // per-query start level (Morton order) or null
-    int32_t* __restrict__ knn;             // n*k rows, sorted space
+    const CloudDev* __restrict__ clouds;   // per-cloud k, radius, row offset
+    const int32_t* __restrict__ mcloud;    // cloud of each Morton position, null for one cloud
+    const int64_t* __restrict__ chunks;    // work list: first Morton position of each 64-query chunk, or
+                                           // null for positions 0, 64, 128, ...
+    unsigned long long* __restrict__ queue;  // work-queue counter (persistent launch) or null
+    int32_t* __restrict__ knn;             // rows, cloud after cloud, local sorted indices
     float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
-    unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks
-    LevelDev lev[EGT_MAX_LEVELS];
-    int64_t n;
-    int k, B, log2B, nlev, start_level, stages, maxc;
-    uint32_t bigmask;  // levels holding a cell with >= maxc points
-    float a, ba, thres_n;
+    unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks,
+                                                // [4] cells looked up
+    const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
+    int64_t nchunks;   // chunks in the work list (after sampling)
+    int stride;        // 1 = every chunk; s > 1 = every s-th chunk of the list (sampled trial runs)
+    int B, log2B, nlev, start_level, stages, maxc;
+    uint32_t bigmask;  // levels holding a cell with >= maxc points (any cloud)
+    float a, ba;
 };
 
 template <int K>
@@ -53,8 +60,8 @@ __device__ inline void topk_insert(uint64_t (&L)[K], uint64_t key) {
 }
 
 // Reference certificate (GridKNNCuda.cu:694-741) at stage s: true = the reference keeps searching.
-__device__ inline bool ref_continue(const KnnParams& P, int s, float x, float y, float z, int fx, int fy, int fz, int c,
-                                    float kthd) {
+__device__ inline bool ref_continue(const KnnParams& P, int kq, float thres_n, int s, float x, float y, float z, int fx,
+                                    int fy, int fz, int c, float kthd) {
     const int lvl = s == 0 ? 0 : s - 1;
     const int off = s == 0 ? 0 : 1;
     const int Bl = P.B >> lvl;
@@ -62,8 +69,8 @@ __device__ inline bool ref_continue(const KnnParams& P, int s, float x, float y,
     const float v[3] = {x, y, z};
     const int ci[3] = {fx >> lvl, fy >> lvl, fz >> lvl};
     bool cont = false;
-    if (c < P.k) {
-        const float r = __builtin_sqrtf(P.thres_n);
+    if (c < kq) {
+        const float r = __builtin_sqrtf(thres_n);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             if (ci[ax] - off >= 0 && r > __builtin_fmaf(-(float)(ci[ax] - off), h, v[ax] - P.a)) cont = true;
@@ -80,18 +87,51 @@ __device__ inline bool ref_continue(const KnnParams& P, int s, float x, float y,
     return cont;
 }
 
-__device__ inline uint32_t cell_count(const KnnParams& P, int lvl, int cx, int cy, int cz) {
+__device__ inline uint32_t cell_count(const KnnParams& P, uint32_t cloud, int lvl, int cx, int cy, int cz) {
     const int Bl = P.B >> lvl;
     if (cx < 0 || cy < 0 || cz < 0 || cx >= Bl || cy >= Bl || cz >= Bl) return 0;
-    return level_lookup(P.hkeys, P.hvals, P.lev[lvl], morton_code(cx, cy, cz)).y;
+    return level_lookup(P.hkeys, P.hvals, P.lev[lvl], cell_key(cloud, morton_code(cx, cy, cz))).y;
 }
 
-template <int K>
+// The 27 cells of a 3x3x3 block (t = dx + 1 + 3 (dy + 1) + 9 (dz + 1)) nearest first: centre, faces, edges,
+// corners, so the bound tightens early and the farther cells can be pruned.
+__constant__ int8_t c_shell_order[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11, 15, 17,
+                                         19, 21, 23, 25, 0, 2, 6, 8, 18, 20, 24, 26};
+
+// Next chunk of the wave: from the shared work queue (persistent launch) or the wave's fixed slot.
+template <bool PERSIST>
+__device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
+    if (!PERSIST) return cur < 0 ? (int64_t)blockIdx.x * (KBLK / 64) + (threadIdx.x >> 6) : P.nchunks;
+    unsigned long long v = 0;
+    if (__lane_id() == 0) v = atomicAdd(P.queue, 1ull);
+    return (int64_t)__shfl(v, 0, 64);
+}
+
+template <int K, bool PERSIST>
 __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
-    const int64_t m = (int64_t)blockIdx.x * KBLK + threadIdx.x;
-    const bool active = m < P.n;
     uint64_t cand = 0;
-    uint32_t invalid_row = 0, learned = 0, fallback = 0;
+    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0;
+    // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
+    // persistent launch, chunk after chunk from a work queue until the list is exhausted.
+    for (int64_t v = next_chunk<PERSIST>(P, -1); v < P.nchunks; v = next_chunk<PERSIST>(P, v)) {
+    const int64_t start = P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64;
+    const int64_t m = start + __lane_id();
+    const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[start] : 0u;
+    const CloudDev cd = P.clouds[cloud];
+    const int kq = cd.k;
+    const float thres_n = cd.thres_n;
+    const bool active = m < cd.base + cd.n;
+    // Learned start: every lane of a wave starts at the smallest level proposed in the wave, so the wave's
+    // 64 neighbouring queries scan the same cells in step; lanes whose proposal is higher expand from there.
+    int own_level = P.start_level;
+    int wave_level = own_level;
+    if (P.qlevel) {
+        own_level = active ? (int)P.qlevel[m] : 127;
+        wave_level = own_level;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) wave_level = min(wave_level, __shfl_xor(wave_level, d, 64));
+        learned += active ? 1u : 0u;
+    }
     if (active) {
         const float4 q = P.mpts[m];
         const int32_t jq = __float_as_int(q.w);
@@ -99,15 +139,12 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
         const int fx = qc & 1023, fy = (qc >> 10) & 1023, fz = qc >> 20;
         uint64_t L[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) L[i] = (i < K - P.k) ? 0ull : KEY_EMPTY;
+        for (int i = 0; i < K; ++i) L[i] = (i < K - kq) ? 0ull : KEY_EMPTY;
 
-        int level = P.start_level;
-        if (P.qlevel) {
-            level = P.qlevel[m];
-            learned = 1;
-        }
+        int level = wave_level;
         level = level < 0 ? 0 : (level >= P.nlev ? P.nlev - 1 : level);
         const int first_level = level;
+        uint32_t fallback_q = 0;
         int plo[3] = {P.B, P.B, P.B}, phi[3] = {-1, -1, -1};  // previous box (fine coords), empty
         for (;;) {
             const int Bl = P.B >> level;
@@ -123,13 +160,15 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                     bool big = false;
                     for (int t = 0; t < 27 && !big; ++t) {
                         if (t == 13) continue;
-                        big = cell_count(P, level, cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1,
+                        big = cell_count(P, cloud, level, cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1,
                                          cq[2] + (t / 9) - 1) >= (uint32_t)P.maxc;
                     }
                     if (big) break;
                 }
             }
-            for (int t = 0; t < 27; ++t) {
+            const float hcell = P.ba / (float)Bl;
+            for (int oi = 0; oi < 27; ++oi) {
+                const int t = c_shell_order[oi];
                 const int c3[3] = {cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1, cq[2] + (t / 9) - 1};
                 if (c3[0] < 0 || c3[1] < 0 || c3[2] < 0 || c3[0] >= Bl || c3[1] >= Bl || c3[2] >= Bl) continue;
                 bool inside = true, overlap = true;
@@ -140,7 +179,25 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                     overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
                 }
                 if (inside) continue;
-                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level], morton_code(c3[0], c3[1], c3[2]));
+                // Cell pruning: skip a cell whose box is farther from the query than the current bound (the
+                // k-th key distance once the list is full, else the search radius). The box distance is
+                // shrunk by 1e-6 (normalized units), far more than the rounding of a point's float distance
+                // or of its cell assignment, so no point of a skipped cell could have entered the list.
+                {
+                    const float bnd = L[K - 1] != KEY_EMPTY ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
+                    const float qv[3] = {q.x, q.y, q.z};
+                    float dd = 0.0f;
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) {
+                        const float lo = P.a + (float)c3[ax] * hcell;
+                        const float gap = fmaxf(fmaxf(lo - qv[ax], qv[ax] - (lo + hcell)) - 1e-6f, 0.0f);
+                        dd = __builtin_fmaf(gap, gap, dd);
+                    }
+                    if (dd > bnd) continue;
+                }
+                ++cells;
+                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level],
+                                             cell_key(cloud, morton_code(c3[0], c3[1], c3[2])));
                 const uint32_t pend = r.x + r.y;
                 // candidates are fetched CH at a time before any is used (one memory latency per chunk); the
                 // top-K under the (distance, index) order does not depend on the offering order
@@ -168,7 +225,7 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                         }
                         ++cand;
                         const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
-                        if (d <= P.thres_n) {
+                        if (d <= thres_n) {
                             const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
                             if (kk < L[K - 1]) topk_insert<K>(L, kk);
                         }
@@ -178,7 +235,7 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
             if (level >= P.nlev - 1) break;
             // exactness: every unscanned point is farther than the current bound
             const bool full = L[K - 1] != KEY_EMPTY;
-            const float bound = full ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : P.thres_n;
+            const float bound = full ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
             const double R = __builtin_sqrt((double)bound) + 2e-6;
             const double hl = (double)P.ba / (double)Bl;
             const double qv[3] = {q.x, q.y, q.z};
@@ -189,7 +246,7 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                 if (cq[ax] + 1 <= Bl - 2 && !(R < ((double)P.a + (double)(cq[ax] + 2) * hl) - qv[ax])) exact = false;
             }
             if (exact) break;
-            if (level == first_level && learned) fallback = 1;
+            if (P.qlevel && level >= own_level) fallback_q = 1;  // the proposed box did not certify the list
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 plo[ax] = max(0, (cq[ax] - 1) << level);
@@ -204,7 +261,7 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
         const uint32_t bm = (uint32_t)P.B - 1u;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            if (i >= K - P.k && L[i] != KEY_EMPTY) {
+            if (i >= K - kq && L[i] != KEY_EMPTY) {
                 ++c;
                 const uint32_t kk = (uint32_t)P.key[(uint32_t)L[i]];
                 const int mx = kk & bm, my = (kk >> P.log2B) & bm, mz = kk >> (2 * P.log2B);
@@ -222,12 +279,12 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
             if ((P.bigmask >> lvl) & 1u) {
                 bool big = false;
                 if (s == 0) {
-                    big = cell_count(P, 0, cx, cy, cz) >= (uint32_t)P.maxc;
+                    big = cell_count(P, cloud, 0, cx, cy, cz) >= (uint32_t)P.maxc;
                 } else {
                     for (int t = 0; t < 27 && !big; ++t) {
                         if (t == 13) continue;
-                        big = cell_count(P, lvl, cx + (t % 3) - 1, cy + ((t / 3) % 3) - 1, cz + (t / 9) - 1) >=
-                              (uint32_t)P.maxc;
+                        big = cell_count(P, cloud, lvl, cx + (t % 3) - 1, cy + ((t / 3) % 3) - 1,
+                                         cz + (t / 9) - 1) >= (uint32_t)P.maxc;
                     }
                 }
                 if (big) break;
@@ -235,28 +292,32 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
             const bool inside = c == 0 || ((lo[0] >> lvl) >= cx - off && (hi[0] >> lvl) <= cx + off &&
                                            (lo[1] >> lvl) >= cy - off && (hi[1] >> lvl) <= cy + off &&
                                            (lo[2] >> lvl) >= cz - off && (hi[2] >> lvl) <= cz + off);
-            if (inside && !ref_continue(P, s, q.x, q.y, q.z, fx, fy, fz, c, kthd)) {
+            if (inside && !ref_continue(P, kq, thres_n, s, q.x, q.y, q.z, fx, fy, fz, c, kthd)) {
                 valid = true;
                 break;
             }
         }
-        int32_t* row = P.knn + (int64_t)jq * P.k;
+        int32_t* row = P.knn + cd.row_off + (int64_t)(jq - cd.base) * kq;
+        const int32_t base = (int32_t)cd.base;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            if (i >= K - P.k) {
-                const int32_t v = (valid && L[i] != KEY_EMPTY) ? (int32_t)(uint32_t)L[i] : -1;
-                row[i - (K - P.k)] = v;
+            if (i >= K - kq) {
+                const int32_t val = (valid && L[i] != KEY_EMPTY) ? (int32_t)(uint32_t)L[i] - base : -1;
+                row[i - (K - kq)] = val;
             }
         }
         P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
-        invalid_row = valid ? 0u : 1u;
+        invalid_row += valid ? 0u : 1u;
+        fallback += fallback_q;
+    }
     }
     // one atomic per wave
     unsigned long long c0 = cand;
-    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback;
+    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         c0 += __shfl_xor(c0, d, 64);
+        c4 += __shfl_xor(c4, d, 64);
         c1 += __shfl_xor(c1, d, 64);
         c2 += __shfl_xor(c2, d, 64);
         c3 += __shfl_xor(c3, d, 64);
@@ -266,12 +327,127 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
         if (c1) atomicAdd(&P.counters[1], (unsigned long long)c1);
         if (c2) atomicAdd(&P.counters[2], (unsigned long long)c2);
         if (c3) atomicAdd(&P.counters[3], (unsigned long long)c3);
+        if (c4) atomicAdd(&P.counters[4], (unsigned long long)c4);
     }
+}
+
+int kclass(int k) {
+    return k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : k <= 32 ? 32 : k <= 40 ? 40 : k <= 48 ? 48 : k <= 56 ? 56 : 64;
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    return cus;
 }
 
 template <int K>
 void launch_knn(const KnnParams& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_knn<K>, dim3(ceil_div(P.n, KBLK)), dim3(KBLK), 0, s, P);
+    int grid = ceil_div(P.nchunks, KBLK / 64);
+    if (grid < 1) return;
+    if (P.queue) {
+        // persistent: as many workgroups as fit on the GPU at once, each wave looping over the queue
+        int per_cu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_knn<K, true>, KBLK, 0));
+        grid = (int)std::min<int64_t>(grid, (int64_t)std::max(1, per_cu) * device_cus());
+        hipLaunchKernelGGL((k_knn<K, true>), dim3(grid), dim3(KBLK), 0, s, P);
+    } else {
+        hipLaunchKernelGGL((k_knn<K, false>), dim3(grid), dim3(KBLK), 0, s, P);
+    }
+    LAUNCH_CHECK();
+}
+
+void launch_any(const KnnParams& P, int K, hipStream_t s) {
+    switch (K) {
+        case 8: launch_knn<8>(P, s); break;
+        case 16: launch_knn<16>(P, s); break;
+        case 24: launch_knn<24>(P, s); break;
+        case 32: launch_knn<32>(P, s); break;
+        case 40: launch_knn<40>(P, s); break;
+        case 48: launch_knn<48>(P, s); break;
+        case 56: launch_knn<56>(P, s); break;
+        default: launch_knn<64>(P, s); break;
+    }
+}
+
+KnnParams knn_params(egt_ctx* h) {
+    KnnParams P{};
+    P.mpts = h->mpts.p;
+    P.mcell = h->mcell.p;
+    P.hkeys = h->hkeys.p;
+    P.hvals = h->hvals.p;
+    P.key = h->key.p;
+    P.qlevel = (h->learned && h->qlevel.p) ? h->qlevel.p : nullptr;
+    P.clouds = h->dclouds.p;
+    P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
+    P.knn = h->knn.p;
+    P.kth = h->scratch.get<float>(15, h->n);
+    P.counters = reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 6));
+    P.queue = nullptr;
+    P.bigmask = 0;
+    LevelDev lv[EGT_MAX_LEVELS];
+    for (int l = 0; l < h->nlev; ++l) {
+        lv[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
+        if (h->lev[l].max_count >= h->maxc) P.bigmask |= 1u << l;
+    }
+    h->dlev.alloc(EGT_MAX_LEVELS);
+    HIP_CHECK(hipMemcpyAsync(h->dlev.p, lv, h->nlev * sizeof(LevelDev), hipMemcpyHostToDevice, h->stream));
+    P.lev = h->dlev.p;
+    P.B = h->B;
+    int log2B = 0;
+    while ((1 << log2B) < h->B) ++log2B;
+    P.log2B = log2B;
+    P.nlev = h->nlev;
+    P.start_level = h->start_level >= 0 ? std::min(h->start_level, h->nlev - 1)
+                                        : auto_start_level(h, h->clouds[0].k);
+    int stages = 0;
+    for (int b = h->B; b >= 8; b >>= 1) ++stages;
+    P.stages = std::max(stages, 1);
+    P.maxc = h->maxc;
+    P.a = -h->eps;
+    P.ba = (1.0f + h->eps) - P.a;
+    P.stride = 1;
+    return P;
+}
+
+// Work lists per neighbour-count class: chunks of 64 Morton-consecutive queries that never straddle two
+// clouds, class after class.
+std::vector<WorkClass> build_work_lists(egt_ctx* h, DBuf<int64_t>& lists) {
+    std::vector<WorkClass> out;
+    if (h->nclouds == 1) {
+        out.push_back(WorkClass{kclass(h->clouds[0].k), 0, (h->n + 63) / 64});
+        return out;
+    }
+    std::vector<int64_t> all;
+    for (int K : {8, 16, 24, 32, 40, 48, 56, 64}) {
+        const int64_t first = (int64_t)all.size();
+        for (const CloudDev& c : h->clouds) {
+            if (kclass(c.k) != K) continue;
+            for (int64_t p = 0; p < c.n; p += 64) all.push_back(c.base + p);
+        }
+        if ((int64_t)all.size() > first) out.push_back(WorkClass{K, first, (int64_t)all.size() - first});
+    }
+    lists.alloc(std::max<size_t>(all.size(), 1));
+    HIP_CHECK(hipMemcpyAsync(lists.p, all.data(), all.size() * sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
+    HIP_CHECK(hipStreamSynchronize(h->stream));
+    return out;
+}
+
+// Lays out the rows (cloud after cloud, n_c x k_c) and uploads the per-cloud parameters.
+void prepare_clouds(egt_ctx* h) {
+    int64_t off = 0;
+    for (CloudDev& c : h->clouds) {
+        c.row_off = off;
+        off += c.n * c.k;
+    }
+    h->knn.alloc((size_t)std::max<int64_t>(off, 1));
+    h->dclouds.alloc(h->clouds.size());
+    HIP_CHECK(hipMemcpyAsync(h->dclouds.p, h->clouds.data(), h->clouds.size() * sizeof(CloudDev),
+                             hipMemcpyHostToDevice, h->stream));
 }
 
 }  // namespace
@@ -286,63 +462,31 @@ int auto_start_level(const egt_ctx* h, int k) {
     return h->nlev - 1;
 }
 
-void run_knn(egt_ctx* h, int k, float thres_n) {
-    const int64_t n = h->n;
+// Search of every cloud of the handle with the k and radius stored in h->clouds.
+void run_knn_clouds(egt_ctx* h) {
     hipStream_t s = h->stream;
-    KnnParams P{};
-    P.mpts = h->mpts.p;
-    P.mcell = h->mcell.p;
-    P.hkeys = h->hkeys.p;
-    P.hvals = h->hvals.p;
-    P.key = h->key.p;
-    P.qlevel = (h->learned && h->qlevel.p) ? h->qlevel.p : nullptr;
-    h->knn.alloc((size_t)n * k);
-    P.knn = h->knn.p;
-    P.kth = h->scratch.get<float>(15, n);
-    unsigned long long* counters = reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 4));
-    HIP_CHECK(hipMemsetAsync(counters, 0, 4 * sizeof(uint64_t), s));
-    P.counters = counters;
-    P.bigmask = 0;
-    for (int l = 0; l < h->nlev; ++l) {
-        P.lev[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
-        if (h->lev[l].max_count >= h->maxc) P.bigmask |= 1u << l;
-    }
-    P.n = n;
-    P.k = k;
-    P.B = h->B;
-    int log2B = 0;
-    while ((1 << log2B) < h->B) ++log2B;
-    P.log2B = log2B;
-    P.nlev = h->nlev;
-    P.start_level = h->start_level >= 0 ? h->start_level : auto_start_level(h, k);
-    int stages = 0;
-    for (int b = h->B; b >= 8; b >>= 1) ++stages;
-    P.stages = std::max(stages, 1);
-    P.maxc = h->maxc;
-    P.a = -h->eps;
-    P.ba = (1.0f + h->eps) - P.a;
-    P.thres_n = thres_n;
-
+    prepare_clouds(h);
+    KnnParams P = knn_params(h);
+    const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
+    h->work_classes = classes;
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
+    unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
-    const int K = k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : k <= 32 ? 32 : k <= 40 ? 40 : k <= 48 ? 48
-                : k <= 56 ? 56 : 64;
-    switch (K) {
-        case 8: launch_knn<8>(P, s); break;
-        case 16: launch_knn<16>(P, s); break;
-        case 24: launch_knn<24>(P, s); break;
-        case 32: launch_knn<32>(P, s); break;
-        case 40: launch_knn<40>(P, s); break;
-        case 48: launch_knn<48>(P, s); break;
-        case 56: launch_knn<56>(P, s); break;
-        default: launch_knn<64>(P, s); break;
+    for (const WorkClass& cr : classes) {
+        P.chunks = h->nclouds > 1 ? h->work_lists.p + cr.first : nullptr;
+        P.nchunks = cr.count;
+        if (h->knn_persistent) {
+            HIP_CHECK(hipMemsetAsync(queues, 0, sizeof(uint64_t), s));
+            P.queue = queues;
+        }
+        launch_any(P, cr.K, s);
     }
-    LAUNCH_CHECK();
     HIP_CHECK(hipEventRecord(e1, s));
-    uint64_t hc[4];
-    HIP_CHECK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    uint64_t hc[5];
+    HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -351,8 +495,62 @@ void run_knn(egt_ctx* h, int k, float thres_n) {
     h->stats.invalid_rows = (int64_t)hc[1];
     h->stats.learned_queries = (int64_t)hc[2];
     h->stats.learned_fallbacks = (int64_t)hc[3];
-    h->k = k;
+    h->stats.knn_cells = (int64_t)hc[4];
     h->have_knn = true;
+}
+
+void run_knn(egt_ctx* h, int k, float thres_n) {
+    h->clouds[0].k = k;
+    h->clouds[0].thres_n = thres_n;
+    run_knn_clouds(h);
+    h->k = k;
+}
+
+// Autotuner for the search's start level (the cell size of the first search box): times the plain search
+// on an evenly spread sample of chunks (about 8192 chunks of 64 Morton-consecutive queries, enough to fill
+// the GPU several times) for each candidate level and keeps the fastest. Rows do not depend on the choice.
+// Writes the trial times (ms, -1 for untried levels) and returns the chosen level. One cloud only.
+int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms) {
+    hipStream_t s = h->stream;
+    h->clouds[0].k = k;
+    h->clouds[0].thres_n = thres_n;
+    prepare_clouds(h);
+    const int saved_learned = h->learned;
+    h->learned = 0;
+    KnnParams P = knn_params(h);
+    h->learned = saved_learned;
+    const int64_t total = (h->n + 63) / 64;
+    P.stride = (int)std::max<int64_t>(1, total / 8192);
+    P.nchunks = (total + P.stride - 1) / P.stride;
+    const int K = kclass(k);
+    const int top = std::min(h->nlev - 1, std::max(3, auto_start_level(h, k) + 1));
+    for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = -1.0f;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0.0f;
+    P.start_level = 0;
+    launch_any(P, K, s);  // warm-up (caches, clocks)
+    for (int l = 0; l <= top; ++l) {
+        P.start_level = l;
+        HIP_CHECK(hipEventRecord(e0, s));
+        launch_any(P, K, s);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        trial_ms[l] = ms;
+        if (l == 0 || ms < best_ms) {
+            best = l;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->start_level = best;
+    h->have_knn = false;  // the trial rows cover only the sample
+    return best;
 }
 
 }  // namespace egt

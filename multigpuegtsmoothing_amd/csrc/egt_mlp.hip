@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_adam(float* __restrict__ P, float*
 struct FeatParams {
     const float4* mpts;
     const uint32_t* mcell;
-    const uint32_t* hkeys;
+    const uint64_t* hkeys;
     const uint2* hvals;
     LevelDev lev;
     int64_t n;
@@ -362,7 +362,7 @@ __device__ inline float feature(const FeatParams& F, uint32_t qc, int t) {
     const int x = cx + (t % 3) - 1, y = cy + ((t / 3) % 3) - 1, z = cz + (t / 9) - 1;
     uint32_t c = 0;
     if (x >= 0 && y >= 0 && z >= 0 && x < Bl && y < Bl && z < Bl)
-        c = level_lookup(F.hkeys, F.hvals, F.lev, morton_code(x, y, z)).y;
+        c = level_lookup(F.hkeys, F.hvals, F.lev, cell_key(0u, morton_code(x, y, z))).y;
     return log2f(1.0f + (float)c) * (1.0f / 16.0f);
 }
 
@@ -451,6 +451,7 @@ FeatParams feat_params(const egt_ctx* h, int k, int ref_level) {
 }
 
 void check_fine_grid(const egt_ctx* h) {
+    if (h->nclouds != 1) throw ArgError{EGT_ERR_UNSUPPORTED, "the learned proposal applies to one-cloud handles"};
     if (h->B != 1024)
         throw ArgError{EGT_ERR_UNSUPPORTED,
                        "the learned proposal needs bucketCount 1024 (its features live on the 1024-cell pyramid)"};

@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "libegt.so")
 
 EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
-OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE = 0, 1, 2, 3
+OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT = 0, 1, 2, 3, 4
 
 
 class EGTError(RuntimeError):
@@ -39,7 +39,8 @@ class Stats(ctypes.Structure):
                 ("learned_queries", ctypes.c_int64), ("learned_fallbacks", ctypes.c_int64),
                 ("msf_edges", ctypes.c_int64), ("msf_trees", ctypes.c_int64), ("msf_weight", ctypes.c_double),
                 ("knn_ms", ctypes.c_float), ("egg_ms", ctypes.c_float), ("taubin_ms", ctypes.c_float),
-                ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("propose_ms", ctypes.c_float)]
+                ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("propose_ms", ctypes.c_float),
+                ("knn_cells", ctypes.c_int64)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -95,6 +96,13 @@ def _load():
         "egt_set_option": ([_H, i32, i64], st),
         "egt_destroy": ([_H], st),
         "egt_get_stream": ([_H], ctypes.c_void_p),
+        "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N], st),
+        "egt_batch_create": ([_F32P, _F32P, _F32P, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
+                              ctypes.c_int32, f32, i32, _PH], st),
+        "egt_batch_knn": ([_H, _I32P, f32], st),
+        "egt_batch_get_knn_original": ([_H, ctypes.c_int32, _I32P], st),
+        "egt_batch_cloud_info": ([_H, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(f32), ctypes.POINTER(f32)], st),
         "egt_model_param_count": ([], ctypes.c_int32),
         "egt_model_feature_count": ([], ctypes.c_int32),
         "egt_model_create": ([ctypes.c_uint64, _PH], st),
@@ -248,6 +256,14 @@ class GridStructure(_Provider):
             _check(_lib.egt_grid_create(x, y, z, x.size, num_cards, eps, bucketCount, maxDivision, ctypes.byref(h)))
         self._h = h
 
+    def autotune(self, k, thres=1e8):
+        """Picks the fastest start level for this cloud and k on a sample of queries (no reference
+        counterpart). Returns (level, per-level sample times in ms, -1 where untried)."""
+        lvl = ctypes.c_int32()
+        ms = np.empty(11, np.float32)
+        _check(_lib.egt_autotune(self.handle, k, thres, ctypes.byref(lvl), ms))
+        return lvl.value, ms
+
     def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
         _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))
 
@@ -266,6 +282,43 @@ class GridMeshCuda(_Provider):
 
     def GetNeighborsFromTriangles(self, k):
         _check(_lib.egt_mesh_neighbors(self.handle, int(k)))
+
+
+class BatchGrid(_Provider):
+    """Many independent clouds in one handle (no reference counterpart). Each cloud behaves exactly like its
+    own GridStructure; EGT_CUDA / Normal_CUDA / getOutputPoints work on the whole batch and use the
+    concatenated input order (split() cuts such an array back into clouds)."""
+
+    def __init__(self, clouds, eps=1e-6, bucketCount=1024):
+        super().__init__()
+        parts = [_xyz(c) for c in clouds]
+        self.sizes = np.array([p[0].size for p in parts], np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        x, y, z = (np.ascontiguousarray(np.concatenate([p[i] for p in parts]), np.float32) for i in range(3))
+        h = ctypes.c_void_p()
+        _check(_lib.egt_batch_create(x, y, z, self.offsets, len(parts), eps, bucketCount, ctypes.byref(h)))
+        self._h = h.value
+        self.k = None
+
+    def knn(self, k, thres=1e8):
+        """GRIDCUDAKNNSELF_COMPACT for every cloud; k is one count or one per cloud."""
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (len(self.sizes),)), np.int32)
+        _check(_lib.egt_batch_knn(self.handle, ks, thres))
+        self.k = ks
+
+    def GetKNNIndexes(self, cloud):
+        out = np.empty((int(self.sizes[cloud]), int(self.k[cloud])), np.int32)
+        _check(_lib.egt_batch_get_knn_original(self.handle, cloud, out))
+        return out
+
+    def cloud_info(self, cloud):
+        n, k, mn, mx = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_float(), ctypes.c_float()
+        _check(_lib.egt_batch_cloud_info(self.handle, cloud, ctypes.byref(n), ctypes.byref(k), ctypes.byref(mn),
+                                         ctypes.byref(mx)))
+        return n.value, k.value, mn.value, mx.value
+
+    def split(self, arr):
+        return [arr[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.sizes))]
 
 
 class EGT_CUDA:

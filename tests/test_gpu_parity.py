@@ -51,7 +51,20 @@ def test_knn_rows_bit_exact(egt_mod, oracle_mod, case):
     np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.parametrize("start_level", [0, 2, 5, 9])
+def test_autotuned_start_level_keeps_rows(egt_mod, oracle_mod):
+    E, O = egt_mod, oracle_mod
+    pts = D.torus(60_000, seed=23)
+    g = E.GridStructure(pts, 1, 1e-6, 1024, 4)
+    lvl, ms = g.autotune(40)
+    _log("autotune level", lvl, "trial ms", np.round(ms, 4).tolist())
+    assert 0 <= lvl <= 10 and ms[lvl] > 0 and ms[lvl] == ms[ms > 0].min()
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    og = O.Grid(*pts)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(og.knn(40)[0]))
+    g.close()
+
+
+@pytest.mark.parametrize("start_level", [-1, 0, 2, 5, 9])
 def test_knn_independent_of_start_level(egt_mod, oracle_mod, start_level):
     E, O = egt_mod, oracle_mod
     pts = D.sphere(8_000, seed=21)
