@@ -61,8 +61,10 @@ typedef enum {
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
-    EGT_OPT_KNN_PERSISTENT = 4   /* kNN: 1 = one persistent work-queue launch per neighbour-count class
+    EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class
                                     (default for batch handles), 0 = one workgroup per 256 queries */
+    EGT_OPT_TAUBIN_KERNEL = 5    /* Taubin pass: 1 = persistent workgroups with double-buffered tiles
+                                    (default), 0 = one workgroup per tile */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -131,12 +133,14 @@ egt_status egt_get_stats(egt_handle h, egt_stats* st);
 /* Tuning switches (no reference counterpart). */
 egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value);
 
-/* Autotuner (no reference counterpart): times the plain search with neighbour count k and radius thres
+/* Autotuner (no reference counterpart): runs the plain search with neighbour count k and radius thres
  * (input units, as egt_grid_knn_self) on an evenly spread sample of queries for each candidate start level
- * (the cell size of the first search box), and sets EGT_OPT_START_LEVEL to the fastest. Rows do not
- * depend on the choice. trial_ms (optional, 11 floats) receives each level's sample time in ms, -1 where
- * untried. The handle's neighbour rows are not valid afterwards: run egt_grid_knn_self. */
-egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms);
+ * (the cell size of the first search box), scores each level by its measured work per query (candidates
+ * tested + 3 x cells looked up) and sets EGT_OPT_START_LEVEL to the cheapest. Rows do not depend on the
+ * choice. trial_ms and trial_cost (optional, 11 floats each) receive each level's sample time in ms and its
+ * score, -1 where untried. The handle's neighbour rows are not valid afterwards: run egt_grid_knn_self. */
+egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms,
+                        float* trial_cost);
 
 /* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA
  * leaked (EGT_CUDA.cu:490-492). */
@@ -155,6 +159,9 @@ void* egt_get_stream(egt_handle h);
  * concatenated order of the input; egt_grid_knn_self and egt_get_knn_original are replaced below. */
 egt_status egt_batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
                             float eps, int bucket_count, egt_handle* out);
+/* Same with x, y, z already in device memory on the current device (offsets stay on the host). */
+egt_status egt_batch_create_device(const float* dx, const float* dy, const float* dz, const int64_t* offsets,
+                                   int32_t nclouds, float eps, int bucket_count, egt_handle* out);
 /* GRIDCUDAKNNSELF_COMPACT for every cloud: k[c] neighbours (1..64) for cloud c, radius thres in the input
  * units of each cloud (GridKNNCuda.cu:2322-2634 per cloud). */
 egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres);
@@ -197,11 +204,17 @@ egt_status egt_model_adam_step(egt_model m, const float* features, const float* 
  * sets the safety margin to the given quantile of the training residuals. loss = final training MSE. */
 egt_status egt_model_train(egt_model m, egt_handle h, int k, int epochs, int batch, int64_t max_samples, float lr,
                            float quantile, float* loss);
+/* The same on a batch handle (egt_batch_create), k[c] neighbours for cloud c: one model learns all the
+ * clouds and neighbour counts of the batch. */
+egt_status egt_model_train_batch(egt_model m, egt_handle h, const int32_t* k, int epochs, int batch,
+                                 int64_t max_samples, float lr, float quantile, float* loss);
 egt_status egt_model_get_safety(egt_model m, float* safety);
 egt_status egt_model_set_safety(egt_model m, float safety);
 /* Proposes a start level for every query of h for neighbour count k and switches h to learned starts
  * (EGT_OPT_LEARNED = 1); the next egt_grid_knn_self uses them. */
 egt_status egt_model_propose(egt_model m, egt_handle h, int k);
+/* Batch handle: proposals for every cloud with k[c] neighbours; the next egt_batch_knn uses them. */
+egt_status egt_model_propose_batch(egt_model m, egt_handle h, const int32_t* k);
 /* Diagnostics: feature rows (n x 28, sorted order), targets of the handle's last search with this k (NaN
  * where invalid or absent) and the reference level; each output optional. */
 egt_status egt_model_features(egt_handle h, int k, float* features, float* targets, int32_t* ref_level);

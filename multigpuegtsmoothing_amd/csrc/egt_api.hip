@@ -249,11 +249,16 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
         case EGT_OPT_KNN_PERSISTENT:
             h->knn_persistent = value ? 1 : 0;
             return EGT_OK;
+        case EGT_OPT_TAUBIN_KERNEL:
+            if (value < 0 || value > 1) return fail(EGT_ERR_INVALID_ARG, "Taubin kernel must be 0 or 1");
+            h->taubin_kernel = (int)value;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }
 
-egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms) {
+egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms,
+                        float* trial_cost) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
     if (h->kind != 0) return fail(EGT_ERR_STATE, "the autotuner needs a one-cloud point-cloud handle");
     if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
@@ -263,17 +268,20 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
         float t = thres / (h->max_extent - h->min_extent);
         t *= t;
         if (h->search_stale) egt::refresh_search_structure(h);
-        float ms[EGT_MAX_LEVELS];
-        const int best = egt::autotune_start_level(h, k, t, ms);
+        float ms[EGT_MAX_LEVELS], cost[EGT_MAX_LEVELS];
+        const int best = egt::autotune_start_level(h, k, t, ms, cost);
         h->csr_valid = false;
         if (start_level) *start_level = best;
-        if (trial_ms)
-            for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = ms[l];
+        for (int l = 0; l < EGT_MAX_LEVELS; ++l) {
+            if (trial_ms) trial_ms[l] = ms[l];
+            if (trial_cost) trial_cost[l] = cost[l];
+        }
     });
 }
 
-egt_status egt_batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
-                            float eps, int bucket_count, egt_handle* out) {
+namespace {
+egt_status batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
+                        float eps, int bucket_count, bool device_input, egt_handle* out) {
     if (!out) return fail(EGT_ERR_INVALID_ARG, "null handle pointer");
     if (!x || !y || !z || !offsets) return fail(EGT_ERR_INVALID_ARG, "null coordinate or offset array");
     if (nclouds < 1 || nclouds > (1 << 20)) return fail(EGT_ERR_INVALID_ARG, "cloud count must be in [1, 2^20]");
@@ -296,11 +304,15 @@ egt_status egt_batch_create(const float* x, const float* y, const float* z, cons
         h->knn_persistent = 1;
         h->stats.n_points = n;
         h->stats.bucket_count = bucket_count;
-        float* tmp = h->scratch.get<float>(16, 3 * (size_t)n);
-        HIP_CHECK(hipMemcpyAsync(tmp, x, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        HIP_CHECK(hipMemcpyAsync(tmp + n, y, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        HIP_CHECK(hipMemcpyAsync(tmp + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
-        egt::build_grid_batch(h, tmp, tmp + n, tmp + 2 * n, offsets, nclouds);
+        if (device_input) {
+            egt::build_grid_batch(h, x, y, z, offsets, nclouds);
+        } else {
+            float* tmp = h->scratch.get<float>(16, 3 * (size_t)n);
+            HIP_CHECK(hipMemcpyAsync(tmp, x, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + n, y, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + 2 * n, z, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            egt::build_grid_batch(h, tmp, tmp + n, tmp + 2 * n, offsets, nclouds);
+        }
     });
     if (st != EGT_OK) {
         if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -309,6 +321,17 @@ egt_status egt_batch_create(const float* x, const float* y, const float* z, cons
     }
     *out = h;
     return EGT_OK;
+}
+}  // namespace
+
+egt_status egt_batch_create(const float* x, const float* y, const float* z, const int64_t* offsets, int32_t nclouds,
+                            float eps, int bucket_count, egt_handle* out) {
+    return batch_create(x, y, z, offsets, nclouds, eps, bucket_count, false, out);
+}
+
+egt_status egt_batch_create_device(const float* dx, const float* dy, const float* dz, const int64_t* offsets,
+                                   int32_t nclouds, float eps, int bucket_count, egt_handle* out) {
+    return batch_create(dx, dy, dz, offsets, nclouds, eps, bucket_count, true, out);
 }
 
 egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres) {
@@ -424,10 +447,46 @@ egt_status egt_model_adam_step(egt_model m, const float* features, const float* 
     return guarded([&] { egt::mlp_adam_step(m, features, targets, n, lr, grad, loss); });
 }
 
+namespace {
+egt_status check_train_args(int epochs, int batch, int64_t max_samples, float lr, float quantile) {
+    if (epochs < 0 || batch < 32) return fail(EGT_ERR_INVALID_ARG, "epochs must be >= 0 and batch >= 32");
+    if (max_samples < 0) return fail(EGT_ERR_INVALID_ARG, "max_samples must be >= 0");
+    if (!(lr > 0.0f) || !std::isfinite(lr)) return fail(EGT_ERR_INVALID_ARG, "learning rate must be > 0");
+    if (!(quantile >= 0.0f && quantile <= 1.0f)) return fail(EGT_ERR_INVALID_ARG, "quantile must be in [0, 1]");
+    return EGT_OK;
+}
+egt_status check_batch_ks(egt_handle h, const int32_t* k) {
+    if (!k) return fail(EGT_ERR_INVALID_ARG, "null k array");
+    for (int c = 0; c < h->nclouds; ++c)
+        if (k[c] < 1 || k[c] > 64) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, 64]");
+    return EGT_OK;
+}
+}  // namespace
+
+egt_status egt_model_train_batch(egt_model m, egt_handle h, const int32_t* k, int epochs, int batch,
+                                 int64_t max_samples, float lr, float quantile, float* loss) {
+    if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
+    if (h->kind != 2) return fail(EGT_ERR_STATE, "egt_model_train_batch needs a batch handle");
+    if (egt_status st = check_batch_ks(h, k)) return st;
+    if (egt_status st = check_train_args(epochs, batch, max_samples, lr, quantile)) return st;
+    return guarded([&] {
+        h->csr_valid = false;
+        h->smoothed = false;
+        egt::mlp_train(m, h, k, epochs, batch, max_samples, lr, quantile, loss);
+    });
+}
+
+egt_status egt_model_propose_batch(egt_model m, egt_handle h, const int32_t* k) {
+    if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
+    if (h->kind != 2) return fail(EGT_ERR_STATE, "egt_model_propose_batch needs a batch handle");
+    if (egt_status st = check_batch_ks(h, k)) return st;
+    return guarded([&] { egt::mlp_propose(m, h, k); });
+}
+
 egt_status egt_model_train(egt_model m, egt_handle h, int k, int epochs, int batch, int64_t max_samples, float lr,
                            float quantile, float* loss) {
     if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
-    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "training needs a cloud handle");
+    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "training needs a one-cloud point-cloud handle");
     if (k < 1 || k > 64 || k >= h->n) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, min(64, n - 1)]");
     if (epochs < 0 || batch < 32) return fail(EGT_ERR_INVALID_ARG, "epochs must be >= 0 and batch >= 32");
     if (max_samples < 0) return fail(EGT_ERR_INVALID_ARG, "max_samples must be >= 0");
@@ -436,7 +495,7 @@ egt_status egt_model_train(egt_model m, egt_handle h, int k, int epochs, int bat
     return guarded([&] {
         h->csr_valid = false;
         h->smoothed = false;
-        egt::mlp_train(m, h, k, epochs, batch, max_samples, lr, quantile, loss);
+        egt::mlp_train(m, h, &k, epochs, batch, max_samples, lr, quantile, loss);
     });
 }
 
@@ -455,13 +514,14 @@ egt_status egt_model_set_safety(egt_model m, float safety) {
 
 egt_status egt_model_propose(egt_model m, egt_handle h, int k) {
     if (!m || need(h)) return fail(EGT_ERR_INVALID_ARG, "null model or handle");
-    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "proposals apply to cloud handles");
+    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "proposals apply to one-cloud point-cloud handles");
     if (k < 1 || k > 64) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, 64]");
-    return guarded([&] { egt::mlp_propose(m, h, k); });
+    return guarded([&] { egt::mlp_propose(m, h, &k); });
 }
 
 egt_status egt_model_features(egt_handle h, int k, float* features, float* targets, int32_t* ref_level) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_UNSUPPORTED, "feature export applies to one-cloud point-cloud handles");
     if (k < 1 || k > 64) return fail(EGT_ERR_INVALID_ARG, "k must be in [1, 64]");
     return guarded([&] {
         int r = 0;

@@ -33,9 +33,18 @@ __device__ inline float sqd(float px, float py, float pz, float qx, float qy, fl
 
 __device__ inline int cell_of(float v, float a, float ba, int B) { return (int)(((v - a) / ba) * (float)B); }
 
-// exp(x): Cody-Waite reduction x = n*ln2 + r (ln2 split in a 24-bit head and a tail), degree-7 Taylor
-// polynomial of exp(r) in Horner form, exact scaling by 2^n.
+// exp(x) near 0: for |x| <= 2^-6 the cubic Taylor polynomial in Horner form (truncation error
+// x^4/24 <= 2.5e-9, a small fraction of an ulp of the result).
+__device__ inline bool egt_expf_small(float x) { return x >= -0.015625f && x <= 0.015625f; }
+__device__ inline float egt_expf_cubic(float x) {
+    const float p = __builtin_fmaf(x, 0.166666672f, 0.5f);
+    return __builtin_fmaf(__builtin_fmaf(p, x, 1.0f), x, 1.0f);
+}
+
+// exp(x): the cubic near 0, else Cody-Waite reduction x = n*ln2 + r (ln2 split in a 24-bit head and a
+// tail), degree-7 Taylor polynomial of exp(r) in Horner form, exact scaling by 2^n.
 __device__ inline float egt_expf(float x) {
+    if (egt_expf_small(x)) return egt_expf_cubic(x);
     if (!(x >= -104.0f)) return (x != x) ? x : 0.0f;
     if (x > 88.0f) return __builtin_huge_valf();
     const float n = __builtin_rintf(x * 1.44269502163f);
@@ -58,10 +67,11 @@ __device__ inline float egt_expf(float x) {
     return p * __uint_as_float((uint32_t)(ni + 127) << 23);
 }
 
-// egt_expf for the Taubin weight argument x = -|q - p|^2 <= 0: on [-80, 0] the range checks of egt_expf
-// never fire and 2^n is a normal number, so the same reduction and polynomial followed by an exact ldexp
-// give identical results; other arguments go through egt_expf.
+// egt_expf for the Taubin weight argument x = -|q - p|^2 <= 0: the same cubic near 0; on [-80, 0] the
+// range checks of egt_expf never fire and 2^n is a normal number, so the same reduction and polynomial
+// followed by an exact ldexp give identical results; other arguments go through egt_expf.
 __device__ inline float egt_expf_taubin(float x) {
+    if (egt_expf_small(x)) return egt_expf_cubic(x);
     if (!(x >= -80.0f)) return egt_expf(x);
     const float n = __builtin_rintf(x * 1.44269502163f);
     float r = __builtin_fmaf(-n, 0.693145751953125f, x);

@@ -257,6 +257,14 @@ __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict
     atomicMax(maxc, (int)(en - st));
 }
 
+// seg value at each cloud's first Morton position (running cell count), plus the total at the end.
+__global__ void __launch_bounds__(BLK) k_cells_at_bases(const uint32_t* __restrict__ seg, const CloudDev* __restrict__ clouds,
+                                                         int nclouds, uint32_t total, uint32_t* __restrict__ out) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c < nclouds) out[c] = seg[clouds[c].base];
+    if (c == nclouds) out[c] = total;
+}
+
 __global__ void __launch_bounds__(BLK) k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLK) p[i] = v;
 }
@@ -522,10 +530,24 @@ void build_search_structure(egt_ctx* h) {
     starts.alloc(n);
     std::vector<uint32_t> cells(h->nlev);
     uint64_t total = 0;
+    const int nc = h->nclouds;
+    h->cloud_cells.assign((size_t)nc * h->nlev, 0);
+    uint32_t* dbase_cells = batch ? sc.get<uint32_t>(22, nc + 1) : nullptr;
+    std::vector<uint32_t> hb(nc + 1);
     for (int l = 0; l < h->nlev; ++l) {
         k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
         LAUNCH_CHECK();
         exclusive_scan_u32(flag.p, seg.p, n, s, sc, &cells[l]);
+        if (batch) {
+            // cells of each cloud at this level (the running count at the cloud boundaries)
+            k_cells_at_bases<<<ceil_div(nc + 1, BLK), BLK, 0, s>>>(seg.p, h->dclouds.p, nc, cells[l], dbase_cells);
+            LAUNCH_CHECK();
+            HIP_CHECK(hipMemcpyAsync(hb.data(), dbase_cells, (nc + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int c = 0; c < nc; ++c) h->cloud_cells[(size_t)c * h->nlev + l] = hb[c + 1] - hb[c];
+        } else {
+            h->cloud_cells[l] = cells[l];
+        }
         uint32_t log2cap = 6;
         while ((1ull << log2cap) < 2ull * cells[l]) ++log2cap;
         h->lev[l].log2cap = log2cap;
@@ -555,6 +577,11 @@ void build_search_structure(egt_ctx* h) {
     HIP_CHECK(hipMemcpyAsync(mx.data(), dmax.p, h->nlev * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     for (int l = 0; l < h->nlev; ++l) h->lev[l].max_count = mx[l];
+    LevelDev lv[EGT_MAX_LEVELS];
+    for (int l = 0; l < h->nlev; ++l) lv[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
+    h->dlev.alloc(EGT_MAX_LEVELS);
+    HIP_CHECK(hipMemcpyAsync(h->dlev.p, lv, h->nlev * sizeof(LevelDev), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
 }
 
 namespace {

@@ -125,7 +125,8 @@ struct egt_ctx {
     egt::DBuf<uint2> hvals;
     egt::LevelTable lev[EGT_MAX_LEVELS];
     int nlev = 0;
-    egt::DBuf<egt::LevelDev> dlev;  // device copy of the level descriptors (kNN kernel)
+    egt::DBuf<egt::LevelDev> dlev;  // device copy of the level descriptors
+    std::vector<int64_t> cloud_cells;  // non-empty cells per cloud and level [cloud * nlev + level]
 
     // clouds (a batch handle holds several independent clouds; a plain handle holds one)
     int nclouds = 1;
@@ -155,6 +156,7 @@ struct egt_ctx {
     int learned = 0;
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;
+    int taubin_kernel = 1;  // 0 = one workgroup per tile, 1 = persistent double-buffered tiles
     int maxc = 100000;
 
     egt_stats stats{};
@@ -192,8 +194,8 @@ void refresh_search_structure(egt_ctx* h);
 // kNN (egt_knn.hip)
 void run_knn(egt_ctx* h, int k, float thres_n);
 void run_knn_clouds(egt_ctx* h);
-int auto_start_level(const egt_ctx* h, int k);
-int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms);
+int auto_start_level(const egt_ctx* h, int k, int cloud = 0);
+int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost);
 
 // learned proposal (egt_mlp.hip)
 int mlp_param_count();
@@ -203,9 +205,9 @@ void mlp_set_params(egt_model_s* m, const float* params);
 void mlp_get_params(egt_model_s* m, float* params);
 void mlp_eval(egt_model_s* m, const float* X, const float* T, int64_t n, float* y, float* grad, float* loss);
 void mlp_adam_step(egt_model_s* m, const float* X, const float* T, int64_t n, float lr, float* grad, float* loss);
-void mlp_train(egt_model_s* m, egt_ctx* h, int k, int epochs, int batch, int64_t max_samples, float lr,
+void mlp_train(egt_model_s* m, egt_ctx* h, const int* ks, int epochs, int batch, int64_t max_samples, float lr,
                float quantile, float* loss);
-void mlp_propose(egt_model_s* m, egt_ctx* h, int k);
+void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks);
 void mlp_features(egt_ctx* h, int k, float* X, float* T, int* ref_level);
 void mlp_levels(egt_ctx* h, int8_t* out);
 

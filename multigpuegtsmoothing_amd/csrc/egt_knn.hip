@@ -389,13 +389,8 @@ KnnParams knn_params(egt_ctx* h) {
     P.counters = reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 6));
     P.queue = nullptr;
     P.bigmask = 0;
-    LevelDev lv[EGT_MAX_LEVELS];
-    for (int l = 0; l < h->nlev; ++l) {
-        lv[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
+    for (int l = 0; l < h->nlev; ++l)
         if (h->lev[l].max_count >= h->maxc) P.bigmask |= 1u << l;
-    }
-    h->dlev.alloc(EGT_MAX_LEVELS);
-    HIP_CHECK(hipMemcpyAsync(h->dlev.p, lv, h->nlev * sizeof(LevelDev), hipMemcpyHostToDevice, h->stream));
     P.lev = h->dlev.p;
     P.B = h->B;
     int log2B = 0;
@@ -452,11 +447,12 @@ void prepare_clouds(egt_ctx* h) {
 
 }  // namespace
 
-int auto_start_level(const egt_ctx* h, int k) {
-    // smallest level whose mean non-empty cell holds at least k/4 points (a 3x3x3 box around a surface
-    // point then holds ~2k candidates)
+int auto_start_level(const egt_ctx* h, int k, int cloud) {
+    // smallest level whose mean non-empty cell of the cloud holds at least k/4 points (a 3x3x3 box around a
+    // surface point then holds ~2k candidates)
+    const double n = (double)h->clouds[cloud].n;
     for (int l = 0; l < h->nlev; ++l) {
-        const double mean = (double)h->n / (double)std::max<int64_t>(1, h->lev[l].cells);
+        const double mean = n / (double)std::max<int64_t>(1, h->cloud_cells[(size_t)cloud * h->nlev + l]);
         if (mean >= 0.25 * k) return l;
     }
     return h->nlev - 1;
@@ -506,11 +502,14 @@ void run_knn(egt_ctx* h, int k, float thres_n) {
     h->k = k;
 }
 
-// Autotuner for the search's start level (the cell size of the first search box): times the plain search
-// on an evenly spread sample of chunks (about 8192 chunks of 64 Morton-consecutive queries, enough to fill
-// the GPU several times) for each candidate level and keeps the fastest. Rows do not depend on the choice.
-// Writes the trial times (ms, -1 for untried levels) and returns the chosen level. One cloud only.
-int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms) {
+// Autotuner for the search's start level (the cell size of the first search box). For each candidate level
+// it runs the plain search on an evenly spread sample of chunks (about 8192 chunks of 64 Morton-consecutive
+// queries) and reads back the work counters. The score per sampled query is candidates + 3 x cells looked up
+// (a cell lookup costs about three candidate tests on gfx950: a dependent hash probe and range fetch against a
+// coalesced distance test). Counters, not the sample's wall time, rank the levels: a sample's time is set by
+// its slowest wave (isolated queries next to dense clusters), not by the bulk of the work. Rows do not depend
+// on the choice. trial_ms / trial_cost get each level's sample time and score (-1 for untried levels).
+int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost) {
     hipStream_t s = h->stream;
     h->clouds[0].k = k;
     h->clouds[0].thres_n = thres_n;
@@ -524,26 +523,30 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms) {
     P.nchunks = (total + P.stride - 1) / P.stride;
     const int K = kclass(k);
     const int top = std::min(h->nlev - 1, std::max(3, auto_start_level(h, k) + 1));
-    for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = -1.0f;
+    for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = trial_cost[l] = -1.0f;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     int best = 0;
-    float best_ms = 0.0f;
-    P.start_level = 0;
-    launch_any(P, K, s);  // warm-up (caches, clocks)
+    double best_cost = 0.0;
     for (int l = 0; l <= top; ++l) {
         P.start_level = l;
+        HIP_CHECK(hipMemsetAsync(P.counters, 0, 5 * sizeof(uint64_t), s));
         HIP_CHECK(hipEventRecord(e0, s));
         launch_any(P, K, s);
         HIP_CHECK(hipEventRecord(e1, s));
-        HIP_CHECK(hipEventSynchronize(e1));
+        uint64_t hc[5];
+        HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
         float ms = 0.0f;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const double q = (double)std::min<int64_t>(h->n, P.nchunks * 64);
+        const double cost = ((double)hc[0] + 3.0 * (double)hc[4]) / q;
         trial_ms[l] = ms;
-        if (l == 0 || ms < best_ms) {
+        trial_cost[l] = (float)cost;
+        if (l == 0 || cost < best_cost) {
             best = l;
-            best_ms = ms;
+            best_cost = cost;
         }
     }
     (void)hipEventDestroy(e0);

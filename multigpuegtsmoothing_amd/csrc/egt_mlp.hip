@@ -347,22 +347,26 @@ struct FeatParams {
     const uint32_t* mcell;
     const uint64_t* hkeys;
     const uint2* hvals;
-    LevelDev lev;
+    const LevelDev* lev;     // level descriptors (device)
+    const CloudDev* clouds;  // per cloud: reference level, neighbour count
+    const int32_t* mcloud;   // cloud of each Morton position, null for one cloud
     int64_t n;
-    int ref_level;
-    float kfeat;
 };
 
-__device__ inline float feature(const FeatParams& F, uint32_t qc, int t) {
-    if (t == NF - 1) return F.kfeat;
+// Feature t of the query at Morton position m (its cloud's reference level and k).
+__device__ inline float feature(const FeatParams& F, int64_t m, int t) {
     if (t >= NF) return 0.0f;
-    const int Bl = 1024 >> F.ref_level;
-    const int cx = (int)(qc & 1023u) >> F.ref_level, cy = (int)((qc >> 10) & 1023u) >> F.ref_level,
-              cz = (int)(qc >> 20) >> F.ref_level;
+    const uint32_t cloud = F.mcloud ? (uint32_t)F.mcloud[m] : 0u;
+    const CloudDev& cd = F.clouds[cloud];
+    if (t == NF - 1) return log2f((float)cd.k) * (1.0f / 8.0f);
+    const int ref = cd.ref_level;
+    const uint32_t qc = F.mcell[m];
+    const int Bl = 1024 >> ref;
+    const int cx = (int)(qc & 1023u) >> ref, cy = (int)((qc >> 10) & 1023u) >> ref, cz = (int)(qc >> 20) >> ref;
     const int x = cx + (t % 3) - 1, y = cy + ((t / 3) % 3) - 1, z = cz + (t / 9) - 1;
     uint32_t c = 0;
     if (x >= 0 && y >= 0 && z >= 0 && x < Bl && y < Bl && z < Bl)
-        c = level_lookup(F.hkeys, F.hvals, F.lev, cell_key(0u, morton_code(x, y, z))).y;
+        c = level_lookup(F.hkeys, F.hvals, F.lev[ref], cell_key(cloud, morton_code(x, y, z))).y;
     return log2f(1.0f + (float)c) * (1.0f / 16.0f);
 }
 
@@ -374,18 +378,20 @@ __global__ void __launch_bounds__(256) k_features(FeatParams F, int by_sorted, f
     const int t = (int)(i % NIN);
     if (m >= F.n) return;
     const int64_t row = by_sorted ? (int64_t)__float_as_int(F.mpts[m].w) : m;
-    X[row * NIN + t] = feature(F, F.mcell[m], t);
+    X[row * NIN + t] = feature(F, m, t);
 }
 
 // Training targets from the plain search: log2(r_k / h_ref) per Morton position; ok = 0 where the row is
 // invalid or r_k == 0.
 __global__ void __launch_bounds__(256) k_targets(const float4* __restrict__ mpts, const float* __restrict__ kth,
-                                                  int64_t n, float inv_href, float* __restrict__ target,
-                                                  uint32_t* __restrict__ ok) {
+                                                  const CloudDev* __restrict__ clouds,
+                                                  const int32_t* __restrict__ mcloud, int64_t n, float ba,
+                                                  float* __restrict__ target, uint32_t* __restrict__ ok) {
     const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (m >= n) return;
     const float d = kth[__float_as_int(mpts[m].w)];
     const bool good = d > 0.0f;
+    const float inv_href = (float)(1024 >> clouds[mcloud ? mcloud[m] : 0].ref_level) / ba;
     ok[m] = good ? 1u : 0u;
     target[m] = good ? log2f(sqrtf(d) * inv_href) : 0.0f;
 }
@@ -400,19 +406,19 @@ __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ 
     FwdOps o;
     load_fwd(P, o);
     const int l = __lane_id();
-    const float href = ba / (float)(1024 >> F.ref_level);
     const int64_t ntile = (F.n + TS - 1) / TS;
     for (int64_t t = (int64_t)blockIdx.x * WPB + threadIdx.x / 64; t < ntile; t += (int64_t)gridDim.x * WPB) {
         const int64_t first = t * TS;
         for (int q = l; q < TS * NIN; q += 64) {
             const int i = q / NIN, f = q % NIN;
             const int64_t m = first + i;
-            L.X[i][f] = bf(m < F.n ? feature(F, F.mcell[m], f) : 0.0f);
+            L.X[i][f] = bf(m < F.n ? feature(F, m, f) : 0.0f);
         }
         wave_sync();
         tile_forward(L, P, o);
         const int64_t m = first + l;
         if (l < TS && m < F.n) {
+            const float href = ba / (float)(1024 >> F.clouds[F.mcloud ? F.mcloud[m] : 0].ref_level);
             const float r = href * exp2f(L.y[l] + safety);
             const float4 p = F.mpts[m];
             const float v[3] = {p.x, p.y, p.z};
@@ -437,21 +443,32 @@ __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ 
     }
 }
 
-FeatParams feat_params(const egt_ctx* h, int k, int ref_level) {
+FeatParams feat_params(const egt_ctx* h) {
     FeatParams F{};
     F.mpts = h->mpts.p;
     F.mcell = h->mcell.p;
     F.hkeys = h->hkeys.p;
     F.hvals = h->hvals.p;
-    F.lev = LevelDev{h->lev[ref_level].offset, h->lev[ref_level].log2cap, h->lev[ref_level].max_count};
+    F.lev = h->dlev.p;
+    F.clouds = h->dclouds.p;
+    F.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     F.n = h->n;
-    F.ref_level = ref_level;
-    F.kfeat = std::log2((float)k) / 8.0f;
     return F;
 }
 
+// Sets every cloud's neighbour count and reference level (the start level from the mean cell occupancy)
+// and uploads the cloud table. ks: one k per cloud.
+void set_cloud_features(egt_ctx* h, const int* ks) {
+    for (int c = 0; c < h->nclouds; ++c) {
+        h->clouds[c].k = ks[c];
+        h->clouds[c].ref_level = auto_start_level(h, ks[c], c);
+    }
+    h->dclouds.alloc(h->clouds.size());
+    HIP_CHECK(hipMemcpyAsync(h->dclouds.p, h->clouds.data(), h->clouds.size() * sizeof(CloudDev),
+                             hipMemcpyHostToDevice, h->stream));
+}
+
 void check_fine_grid(const egt_ctx* h) {
-    if (h->nclouds != 1) throw ArgError{EGT_ERR_UNSUPPORTED, "the learned proposal applies to one-cloud handles"};
     if (h->B != 1024)
         throw ArgError{EGT_ERR_UNSUPPORTED,
                        "the learned proposal needs bucketCount 1024 (its features live on the 1024-cell pyramid)"};
@@ -564,10 +581,11 @@ void mlp_adam_step(egt_model_s* m, const float* Xh, const float* Th, int64_t n, 
 void mlp_features(egt_ctx* h, int k, float* Xh, float* Th, int* ref_level_out) {
     check_fine_grid(h);
     if (h->search_stale) refresh_search_structure(h);
-    const int ref = auto_start_level(h, k);
+    set_cloud_features(h, &k);
+    const int ref = h->clouds[0].ref_level;
     const int64_t n = h->n;
     float* X = h->scratch.get<float>(24, (size_t)n * NIN);
-    k_features<<<ceil_div(n * NIN, 256), 256, 0, h->stream>>>(feat_params(h, k, ref), 1, X);
+    k_features<<<ceil_div(n * NIN, 256), 256, 0, h->stream>>>(feat_params(h), 1, X);
     LAUNCH_CHECK();
     std::vector<float> buf((size_t)n * NIN);
     HIP_CHECK(hipMemcpyAsync(buf.data(), X, buf.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
@@ -591,7 +609,7 @@ void mlp_features(egt_ctx* h, int k, float* Xh, float* Th, int* ref_level_out) {
     if (ref_level_out) *ref_level_out = ref;
 }
 
-void mlp_train(egt_model_s* m, egt_ctx* h, int k, int epochs, int batch, int64_t max_samples, float lr,
+void mlp_train(egt_model_s* m, egt_ctx* h, const int* ks, int epochs, int batch, int64_t max_samples, float lr,
                float quantile, float* loss_out) {
     check_fine_grid(h);
     if (h->search_stale) refresh_search_structure(h);
@@ -600,17 +618,22 @@ void mlp_train(egt_model_s* m, egt_ctx* h, int k, int epochs, int batch, int64_t
     // exact k-th radii from the plain search (no radius limit)
     const int saved = h->learned;
     h->learned = 0;
-    run_knn(h, k, 3.4e38f);
+    for (int c = 0; c < h->nclouds; ++c) {
+        h->clouds[c].k = ks[c];
+        h->clouds[c].thres_n = 3.4e38f;
+    }
+    run_knn_clouds(h);
+    if (h->nclouds == 1) h->k = ks[0];
     h->learned = saved;
-    const int ref = auto_start_level(h, k);
+    set_cloud_features(h, ks);
     const float ba = (1.0f + h->eps) + h->eps;
     float* Xall = h->scratch.get<float>(24, (size_t)n * NIN);
     float* Tall = h->scratch.get<float>(25, n);
     uint32_t* ok = h->scratch.get<uint32_t>(26, n);
-    k_features<<<ceil_div(n * NIN, 256), 256, 0, s>>>(feat_params(h, k, ref), 0, Xall);
+    k_features<<<ceil_div(n * NIN, 256), 256, 0, s>>>(feat_params(h), 0, Xall);
     LAUNCH_CHECK();
-    k_targets<<<ceil_div(n, 256), 256, 0, s>>>(h->mpts.p, h->scratch.get<float>(15, n), n, (float)(1024 >> ref) / ba,
-                                               Tall, ok);
+    k_targets<<<ceil_div(n, 256), 256, 0, s>>>(h->mpts.p, h->scratch.get<float>(15, n), h->dclouds.p,
+                                               h->nclouds > 1 ? h->mcloud.p : nullptr, n, ba, Tall, ok);
     LAUNCH_CHECK();
     std::vector<uint32_t> okh(n);
     HIP_CHECK(hipMemcpyAsync(okh.data(), ok, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -672,13 +695,13 @@ void mlp_train(egt_model_s* m, egt_ctx* h, int k, int epochs, int batch, int64_t
     if (loss_out) *loss_out = (float)(mse / (double)ns);
 }
 
-void mlp_propose(egt_model_s* m, egt_ctx* h, int k) {
+void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks) {
     check_fine_grid(h);
     if (h->search_stale) refresh_search_structure(h);
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     h->qlevel.alloc(n);
-    const int ref = auto_start_level(h, k);
+    set_cloud_features(h, ks);
     const float a = -h->eps, ba = (1.0f + h->eps) - a;
     HIP_CHECK(hipStreamSynchronize(m->stream));  // parameters written on the model stream are final
     hipEvent_t e0, e1;
@@ -687,7 +710,7 @@ void mlp_propose(egt_model_s* m, egt_ctx* h, int k) {
     HIP_CHECK(hipEventRecord(e0, s));
     const int64_t tiles = (n + TS - 1) / TS;
     k_mlp_propose<<<(int)std::min<int64_t>(8192, (tiles + WPB - 1) / WPB), MBLK, 0, s>>>(
-        m->P.p, feat_params(h, k, ref), a, ba, h->nlev, m->safety, h->qlevel.p);
+        m->P.p, feat_params(h), a, ba, h->nlev, m->safety, h->qlevel.p);
     LAUNCH_CHECK();
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));

@@ -143,22 +143,27 @@ __global__ void __launch_bounds__(NBLK) k_bv_reset(int64_t n, uint64_t* __restri
 
 // 64-bit atomicMin on base[idx]; when every active lane of the wave targets the same address (one giant
 // component late in Boruvka) the wave reduces first and issues a single atomic. Call from all lanes.
+// atomicMin(base[idx], val) for the active lanes of a wave. Lanes that share a target are reduced in the
+// wave first (one atomic per shared target), taking the targets of the first active lane one after another
+// while each round still merges at least two lanes; the remaining lanes issue their own atomics.
 __device__ inline void wave_atomic_min(uint64_t* base, int32_t idx, uint64_t val, bool active) {
-    const uint64_t amask = __ballot(active);
-    if (!amask) return;
-    const int leader = __ffsll((unsigned long long)amask) - 1;
-    const int32_t lidx = __shfl(idx, leader, 64);
-    if (__ballot(active && idx != lidx) == 0) {
-        unsigned long long v = active ? (unsigned long long)val : ~0ull;
+    for (int round = 0; round < 8; ++round) {
+        const uint64_t amask = __ballot(active);
+        if (!amask) return;
+        const int leader = __ffsll((unsigned long long)amask) - 1;
+        const int32_t lidx = __shfl(idx, leader, 64);
+        const bool mine = active && idx == lidx;
+        if (__popcll(__ballot(mine)) < 2) break;
+        unsigned long long v = mine ? (unsigned long long)val : ~0ull;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
             const unsigned long long o = __shfl_xor(v, d, 64);
             v = o < v ? o : v;
         }
         if (__lane_id() == leader) atomicMin((unsigned long long*)&base[lidx], v);
-    } else if (active) {
-        atomicMin((unsigned long long*)&base[idx], (unsigned long long)val);
+        active = active && !mine;
     }
+    if (active) atomicMin((unsigned long long*)&base[idx], (unsigned long long)val);
 }
 
 // Block-level variant: when all active lanes of the block target one address, one atomic per block.
@@ -253,17 +258,23 @@ __global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __re
                                                    const uint64_t* __restrict__ best, uint32_t* __restrict__ H,
                                                    uint8_t* __restrict__ msf, uint32_t* __restrict__ hooks) {
     const int64_t c = (int64_t)blockIdx.x * NBLK + threadIdx.x;
-    if (c >= n || comp[c] != c) return;
-    const uint64_t b = best[c];
-    if (b == KEY_EMPTY) return;
-    const uint32_t e = (uint32_t)b;
-    const int32_t u = eu[e], v = ev[e];
-    const int32_t cu = comp[u], cv = comp[v];
-    const int32_t other = cu == (int32_t)c ? cv : cu;
-    if (best[other] == b && (int32_t)c < other) return;  // mutual pair: the smaller id stays root
-    H[c] = ((uint32_t)other << 1) | (uint32_t)(par[u] ^ par[v] ^ erel[e]);
-    msf[e] = 1;
-    atomicAdd(hooks, 1u);
+    bool hooked = false;
+    if (c < n && comp[c] == c) {
+        const uint64_t b = best[c];
+        if (b != KEY_EMPTY) {
+            const uint32_t e = (uint32_t)b;
+            const int32_t u = eu[e], v = ev[e];
+            const int32_t cu = comp[u], cv = comp[v];
+            const int32_t other = cu == (int32_t)c ? cv : cu;
+            if (!(best[other] == b && (int32_t)c < other)) {  // mutual pair: the smaller id stays root
+                H[c] = ((uint32_t)other << 1) | (uint32_t)(par[u] ^ par[v] ^ erel[e]);
+                msf[e] = 1;
+                hooked = true;
+            }
+        }
+    }
+    const uint64_t hm = __ballot(hooked);  // one counter update per wave
+    if (hm && __lane_id() == __ffsll((unsigned long long)hm) - 1) atomicAdd(hooks, (uint32_t)__popcll(hm));
 }
 
 // Follows hook pointers to the root. The hook graph is a forest (edge keys are a strict total order and

@@ -20,11 +20,12 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libegt.so")
+LIB_PATH = os.environ.get("EGT_LIB_PATH") or os.path.join(_HERE, "libegt.so")
 
 EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
 OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT = 0, 1, 2, 3, 4
+OPT_TAUBIN_KERNEL = 5
 
 
 class EGTError(RuntimeError):
@@ -96,9 +97,12 @@ def _load():
         "egt_set_option": ([_H, i32, i64], st),
         "egt_destroy": ([_H], st),
         "egt_get_stream": ([_H], ctypes.c_void_p),
-        "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N], st),
+        "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N, _F32N], st),
         "egt_batch_create": ([_F32P, _F32P, _F32P, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
                               ctypes.c_int32, f32, i32, _PH], st),
+        "egt_batch_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"), ctypes.c_int32, f32, i32,
+                                     _PH], st),
         "egt_batch_knn": ([_H, _I32P, f32], st),
         "egt_batch_get_knn_original": ([_H, ctypes.c_int32, _I32P], st),
         "egt_batch_cloud_info": ([_H, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32),
@@ -115,6 +119,8 @@ def _load():
         "egt_model_get_safety": ([_H, ctypes.POINTER(f32)], st),
         "egt_model_set_safety": ([_H, f32], st),
         "egt_model_propose": ([_H, _H, i32], st),
+        "egt_model_train_batch": ([_H, _H, _I32P, i32, i32, i64, f32, f32, ctypes.POINTER(f32)], st),
+        "egt_model_propose_batch": ([_H, _H, _I32P], st),
         "egt_model_features": ([_H, i32, _F32N, _F32N, ctypes.POINTER(ctypes.c_int32)], st),
         "egt_model_levels": ([_H, np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")], st),
     }
@@ -257,12 +263,14 @@ class GridStructure(_Provider):
         self._h = h
 
     def autotune(self, k, thres=1e8):
-        """Picks the fastest start level for this cloud and k on a sample of queries (no reference
-        counterpart). Returns (level, per-level sample times in ms, -1 where untried)."""
+        """Picks the start level with the least measured work per query for this cloud and k, on a sample of
+        queries (no reference counterpart). Returns (level, per-level sample times in ms, per-level scores),
+        -1 where untried."""
         lvl = ctypes.c_int32()
         ms = np.empty(11, np.float32)
-        _check(_lib.egt_autotune(self.handle, k, thres, ctypes.byref(lvl), ms))
-        return lvl.value, ms
+        cost = np.empty(11, np.float32)
+        _check(_lib.egt_autotune(self.handle, k, thres, ctypes.byref(lvl), ms, cost))
+        return lvl.value, ms, cost
 
     def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
         _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))
@@ -289,14 +297,23 @@ class BatchGrid(_Provider):
     own GridStructure; EGT_CUDA / Normal_CUDA / getOutputPoints work on the whole batch and use the
     concatenated input order (split() cuts such an array back into clouds)."""
 
-    def __init__(self, clouds, eps=1e-6, bucketCount=1024):
+    def __init__(self, clouds, eps=1e-6, bucketCount=1024, device=None):
+        """clouds: list of point sets, or (with device=(dx, dy, dz) torch tensors holding the clouds one
+        after another) the list of cloud sizes."""
         super().__init__()
-        parts = [_xyz(c) for c in clouds]
-        self.sizes = np.array([p[0].size for p in parts], np.int64)
-        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
-        x, y, z = (np.ascontiguousarray(np.concatenate([p[i] for p in parts]), np.float32) for i in range(3))
         h = ctypes.c_void_p()
-        _check(_lib.egt_batch_create(x, y, z, self.offsets, len(parts), eps, bucketCount, ctypes.byref(h)))
+        if device is not None:
+            self.sizes = np.asarray(clouds, np.int64)
+            self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+            ptrs = [ctypes.c_void_p(t.data_ptr()) for t in device]
+            _check(_lib.egt_batch_create_device(*ptrs, self.offsets, len(self.sizes), eps, bucketCount,
+                                                ctypes.byref(h)))
+        else:
+            parts = [_xyz(c) for c in clouds]
+            self.sizes = np.array([p[0].size for p in parts], np.int64)
+            self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+            x, y, z = (np.ascontiguousarray(np.concatenate([p[i] for p in parts]), np.float32) for i in range(3))
+            _check(_lib.egt_batch_create(x, y, z, self.offsets, len(parts), eps, bucketCount, ctypes.byref(h)))
         self._h = h.value
         self.k = None
 
@@ -415,9 +432,16 @@ class RadiusModel:
         return g, loss.value
 
     def train(self, provider, k, epochs=4, batch=4096, max_samples=2_000_000, lr=2e-3, quantile=0.9):
+        """Trains on a GridStructure (one k) or a BatchGrid (k per cloud); returns the final training MSE."""
         loss = ctypes.c_float()
-        _check(_lib.egt_model_train(self.handle, provider.handle, k, epochs, batch, max_samples, lr, quantile,
-                                    ctypes.byref(loss)))
+        if isinstance(provider, BatchGrid):
+            ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (len(provider.sizes),)), np.int32)
+            _check(_lib.egt_model_train_batch(self.handle, provider.handle, ks, epochs, batch, max_samples, lr,
+                                              quantile, ctypes.byref(loss)))
+            provider.k = ks
+        else:
+            _check(_lib.egt_model_train(self.handle, provider.handle, k, epochs, batch, max_samples, lr, quantile,
+                                        ctypes.byref(loss)))
         return loss.value
 
     @property
@@ -431,7 +455,11 @@ class RadiusModel:
         _check(_lib.egt_model_set_safety(self.handle, value))
 
     def propose(self, provider, k):
-        _check(_lib.egt_model_propose(self.handle, provider.handle, k))
+        if isinstance(provider, BatchGrid):
+            ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (len(provider.sizes),)), np.int32)
+            _check(_lib.egt_model_propose_batch(self.handle, provider.handle, ks))
+        else:
+            _check(_lib.egt_model_propose(self.handle, provider.handle, k))
 
 
 def model_features(provider, k):

@@ -94,3 +94,33 @@ def test_batch_errors(egt_mod):
     with pytest.raises(E.EGTError):
         b.knn([8, 65])
     b.close()
+
+
+def test_learned_proposal_on_batch(egt_mod, oracle_mod):
+    E, O = egt_mod, oracle_mod
+    train = [D.superquadric(n, seed=700 + i) for i, n in enumerate([20000, 8000, 30000, 5000, 12000, 25000])]
+    ktrain = np.array([16, 40, 64, 16, 40, 64], np.int32)
+    evalc, keval = _clouds()
+    model = E.RadiusModel(seed=5)
+    bt = E.BatchGrid(train)
+    loss = model.train(bt, ktrain, epochs=6, batch=2048, max_samples=0, lr=2e-3, quantile=0.9)
+    _log("batch training mse", loss, "safety", model.safety)
+    assert np.isfinite(loss) and loss < 1.0
+    bt.close()
+    plain = E.BatchGrid(evalc)
+    plain.knn(keval)
+    learned = E.BatchGrid(evalc)
+    model.propose(learned, keval)
+    learned.knn(keval)
+    st = learned.stats()
+    n = int(plain.sizes.sum())
+    _log("learned batch: queries", st["learned_queries"], "fallbacks", st["learned_fallbacks"],
+         "cells/query", st["knn_cells"] / n, "plain cells/query", plain.stats()["knn_cells"] / n)
+    assert st["learned_queries"] == n
+    for c in range(len(evalc)):
+        np.testing.assert_array_equal(learned.GetKNNIndexes(c), plain.GetKNNIndexes(c), err_msg=f"cloud {c}")
+    og = O.Grid(*evalc[5])
+    np.testing.assert_array_equal(learned.GetKNNIndexes(5), og.to_original(og.knn(int(keval[5]))[0]))
+    plain.close()
+    learned.close()
+    model.close()

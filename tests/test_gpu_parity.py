@@ -55,9 +55,9 @@ def test_autotuned_start_level_keeps_rows(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     pts = D.torus(60_000, seed=23)
     g = E.GridStructure(pts, 1, 1e-6, 1024, 4)
-    lvl, ms = g.autotune(40)
-    _log("autotune level", lvl, "trial ms", np.round(ms, 4).tolist())
-    assert 0 <= lvl <= 10 and ms[lvl] > 0 and ms[lvl] == ms[ms > 0].min()
+    lvl, ms, cost = g.autotune(40)
+    _log("autotune level", lvl, "trial ms", np.round(ms, 4).tolist(), "cost", np.round(cost, 1).tolist())
+    assert 0 <= lvl <= 10 and cost[lvl] > 0 and cost[lvl] == cost[cost > 0].min()
     g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
     og = O.Grid(*pts)
     np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(og.knn(40)[0]))
