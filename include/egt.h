@@ -61,10 +61,8 @@ typedef enum {
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
-    EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class
+    EGT_OPT_KNN_PERSISTENT = 4   /* kNN: 1 = one persistent work-queue launch per neighbour-count class
                                     (default for batch handles), 0 = one workgroup per 256 queries */
-    EGT_OPT_TAUBIN_KERNEL = 5    /* Taubin pass: 1 = persistent workgroups with double-buffered tiles
-                                    (default), 0 = one workgroup per tile */
 } egt_option;
 
 const char* egt_last_error_string(void);

@@ -249,10 +249,7 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
         case EGT_OPT_KNN_PERSISTENT:
             h->knn_persistent = value ? 1 : 0;
             return EGT_OK;
-        case EGT_OPT_TAUBIN_KERNEL:
-            if (value < 0 || value > 1) return fail(EGT_ERR_INVALID_ARG, "Taubin kernel must be 0 or 1");
-            h->taubin_kernel = (int)value;
-            return EGT_OK;
+
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }

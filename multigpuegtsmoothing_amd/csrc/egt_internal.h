@@ -156,7 +156,6 @@ struct egt_ctx {
     int learned = 0;
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;
-    int taubin_kernel = 1;  // 0 = one workgroup per tile, 1 = persistent double-buffered tiles
     int maxc = 100000;
 
     egt_stats stats{};

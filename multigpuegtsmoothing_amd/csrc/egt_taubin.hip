@@ -237,101 +237,6 @@ __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restr
     }
 }
 
-// Persistent, double-buffered variant of the tiled pass. Each workgroup owns a strided sequence of tiles
-// (tiles of one XCD are contiguous, walked by that XCD's workgroups in step). While a tile is computed from
-// one LDS buffer, every thread already holds in registers its share of the next tile (its point, its row
-// start and at most one halo point); after the compute those registers go to the other buffer. The global
-// latency of staging is thus hidden behind the compute of the previous tile instead of stalling the block.
-// Requires TBLK_TILE == TILE (one thread per point). Same arithmetic and neighbour order as k_taubin_tile.
-struct TileRegs {
-    float4 pt, hp;
-    int32_t rp, rp_end, H;
-};
-
-__device__ inline void tile_prefetch(int64_t t, const float4* __restrict__ in, const int32_t* __restrict__ rowptr,
-                                     const int32_t* __restrict__ halo, const int32_t* __restrict__ hcount, int64_t n,
-                                     TileRegs& r) {
-    const int tid = threadIdx.x;
-    const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
-    r.H = hcount[t];
-    r.rp = base + tid <= end ? rowptr[base + tid] : 0;
-    r.rp_end = tid == 0 ? rowptr[end] : 0;
-    r.pt = base + tid < end ? in[base + tid] : make_float4(0.f, 0.f, 0.f, 0.f);
-    r.hp = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (tid < r.H) r.hp = in[halo[t * HALO_MAX + tid]];
-}
-
-__device__ inline void tile_store(const TileRegs& r, int64_t t, int64_t n, float4* buf, int32_t* srow) {
-    const int tid = threadIdx.x;
-    const int64_t base = t * TILE, cnt = min<int64_t>(base + TILE, n) - base;
-    if (tid <= cnt) srow[tid] = r.rp;
-    if (tid == 0) srow[cnt] = r.rp_end;
-    if (r.H >= 0) {
-        if (tid < cnt) buf[tid] = r.pt;
-        if (tid < r.H) buf[TILE + tid] = r.hp;
-    }
-}
-
-__global__ void __launch_bounds__(TBLK_TILE) __attribute__((amdgpu_waves_per_eu(8))) k_taubin_stream(const float4* __restrict__ in, float4* __restrict__ out,
-                                                         const int32_t* __restrict__ rowptr,
-                                                         const int32_t* __restrict__ col,
-                                                         const uint16_t* __restrict__ col16,
-                                                         const int32_t* __restrict__ halo,
-                                                         const int32_t* __restrict__ hcount, int64_t n, float scale,
-                                                         int ntiles, int maxH) {
-    extern __shared__ float4 lds[];
-    float4* buf[2] = {lds, lds + TILE + maxH};
-    int32_t* srow_base = reinterpret_cast<int32_t*>(lds + 2 * (TILE + maxH));
-    int32_t* srow[2] = {srow_base, srow_base + TILE + 1};
-    // tile sequence of this workgroup: XCD x = blockIdx % 8 owns tiles [x * per, (x + 1) * per), its
-    // workgroups j = blockIdx / 8 take tiles x * per + j, + groups_per_xcd, ...
-    const int x = blockIdx.x % NXCD, j = blockIdx.x / NXCD;
-    const int gpx = (gridDim.x + NXCD - 1 - x) / NXCD;  // workgroups on this XCD
-    const int per = (ntiles + NXCD - 1) / NXCD;
-    const int64_t first = (int64_t)x * per, last = min<int64_t>(first + per, ntiles);
-    int64_t t = first + j;
-    if (t >= last) return;
-    TileRegs r;
-    tile_prefetch(t, in, rowptr, halo, hcount, n, r);
-    int cur = 0;
-    tile_store(r, t, n, buf[cur], srow[cur]);
-    __syncthreads();
-    for (;;) {
-        const int64_t tn = t + gpx;
-        const int32_t H = r.H;
-        if (tn < last) tile_prefetch(tn, in, rowptr, halo, hcount, n, r);
-        const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
-        const int64_t i = base + threadIdx.x;
-        if (i < end) {
-            const float4* src = buf[cur];
-            const int32_t* sr = srow[cur];
-            const float4 p = H >= 0 ? src[i - base] : in[i];
-            float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
-            const int32_t e0 = sr[i - base], e1 = sr[i - base + 1];
-            if (H >= 0) {
-                taubin_point(p, src, col16, e0, e1, cgx, cgy, cgz, sum);
-            } else {
-                for (int32_t e = e0; e < e1; ++e) {
-                    const float4 q = in[col[e]];
-                    const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
-                    const float w = egt_expf_taubin(-dist);
-                    cgx = __builtin_fmaf(w, q.x - p.x, cgx);
-                    cgy = __builtin_fmaf(w, q.y - p.y, cgy);
-                    cgz = __builtin_fmaf(w, q.z - p.z, cgz);
-                    sum += w;
-                }
-            }
-            if (sum == 0.0f) sum = 1e-8f;
-            out[i] = make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
-        }
-        if (tn >= last) break;  // uniform across the workgroup
-        cur ^= 1;
-        tile_store(r, tn, n, buf[cur], srow[cur]);
-        __syncthreads();
-        t = tn;
-    }
-}
-
 // Writes the smoothed points back to the sorted-space SoA arrays and refreshes the Morton copy.
 __global__ void __launch_bounds__(TBLK) k_writeback(const float4* __restrict__ p, int64_t n,
                                                      float4* __restrict__ mpts, float* __restrict__ x,
@@ -378,24 +283,9 @@ void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     h->taubin_fallback_tiles = fallback_tiles;
     const size_t lds_bytes = (size_t)(TILE + maxH) * sizeof(float4);
     const dim3 tgrid(ceil_div(ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
-    // persistent double-buffered kernel (one thread per point): as many workgroups as fit at once
-    const bool stream = h->taubin_kernel == 1 && TBLK_TILE == TILE;
-    const size_t stream_lds = 2 * (size_t)(TILE + maxH) * sizeof(float4) + 2 * (TILE + 1) * sizeof(int32_t);
-    int sgrid = 0;
-    if (stream) {
-        int per_cu = 0, cus = 0, dev = 0;
-        HIP_CHECK(hipGetDevice(&dev));
-        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_taubin_stream, TBLK_TILE, stream_lds));
-        sgrid = std::min(ntiles, std::max(1, per_cu) * cus);
-    }
     auto pass = [&](const float4* in, float4* out, float scale) {
-        if (stream)
-            k_taubin_stream<<<sgrid, tblk, stream_lds, s>>>(in, out, h->rowptr.p, h->col.p, col16, halo, hcount, n,
-                                                           scale, ntiles, maxH);
-        else
-            k_taubin_tile<<<tgrid, tblk, lds_bytes, s>>>(in, out, h->rowptr.p, h->col.p, col16, halo, hcount, n,
-                                                         scale, ntiles);
+        k_taubin_tile<<<tgrid, tblk, lds_bytes, s>>>(in, out, h->rowptr.p, h->col.p, col16, halo, hcount, n, scale,
+                                                     ntiles);
     };
     if (iters > 0) {
         if (h->taubin_graph) {
