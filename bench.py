@@ -238,7 +238,7 @@ def main():
     dominant = max(stage, key=stage.get)
     if dominant == "taubin":
         ach = taub_bytes / (taub_pass_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_taubin_tile", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        roof = {"kernel": f"k_taubin_tile<{stats[-1]['taubin_tile']}>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(), "bytes_per_launch": taub_bytes,
                 "avg_launch_ms": taub_pass_ms}
     else:

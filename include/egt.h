@@ -52,6 +52,9 @@ typedef struct {
     float knn_ms, egg_ms, taubin_ms, normals_ms, grid_ms; /* device time of the last call of each stage */
     float propose_ms;          /* device time of the last learned proposal (features + MLP + level choice) */
     int64_t knn_cells;         /* cells looked up by the kNN search */
+    float taubin_tile_ms[2];   /* Taubin tile autotuner: ms per pass at tile width 1024 and 2048 (0 if untried) */
+    int32_t taubin_tile;       /* tile width of the last smoothing call */
+    int32_t taubin_fallback_tiles; /* tiles of that call whose halo did not fit in LDS (global gathers) */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */
@@ -61,8 +64,11 @@ typedef enum {
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
-    EGT_OPT_KNN_PERSISTENT = 4   /* kNN: 1 = one persistent work-queue launch per neighbour-count class
+    EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class
                                     (default for batch handles), 0 = one workgroup per 256 queries */
+    EGT_OPT_TAUBIN_TILE = 5      /* Taubin: points per LDS tile, 1024 or 2048 (default); 0 = autotune on the
+                                    next smoothing call (times both on copies, keeps the faster; the
+                                    per-pass times land in egt_stats.taubin_tile_ms) */
 } egt_option;
 
 const char* egt_last_error_string(void);

@@ -249,6 +249,11 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
         case EGT_OPT_KNN_PERSISTENT:
             h->knn_persistent = value ? 1 : 0;
             return EGT_OK;
+        case EGT_OPT_TAUBIN_TILE:
+            if (value != 0 && value != 1024 && value != 2048)
+                return fail(EGT_ERR_INVALID_ARG, "Taubin tile width must be 1024, 2048 or 0 (autotune)");
+            h->taubin_tile = (int)value;
+            return EGT_OK;
 
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");

@@ -130,6 +130,126 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
     kept[m] = (uint32_t)__popcll(keep);
 }
 
+// Same filter with LPP lanes per point (LPP consecutive lanes of a wave share one point): the point's
+// neighbour coordinates are staged once in LDS ([slot][point], 12 B per slot per point instead of per
+// lane), and lane r tests the slots j = r, r + LPP, ... against all valid m < j. LDS per lane drops LPP-fold,
+// so several times more waves fit on a CU to hide the gather latency; every test is the one of k_egg, so
+// the kept sets are identical. A workgroup of 64 lanes covers 64 / LPP points; 64-point chunks take LPP
+// workgroups.
+template <int K, int LPP>
+__global__ void __launch_bounds__(EBLK) k_egg_shared(const float4* __restrict__ mpts, const int32_t* __restrict__ minv,
+                                                      int32_t* __restrict__ knn, const CloudDev* __restrict__ clouds,
+                                                      const int32_t* __restrict__ mcloud,
+                                                      const int64_t* __restrict__ chunks, float alpha,
+                                                      int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
+                                                      uint32_t* __restrict__ kept) {
+    constexpr int PPB = EBLK / LPP;  // points per workgroup
+    __shared__ float sx[K][PPB], sy[K][PPB], sz[K][PPB];
+    const int tid = threadIdx.x, pi = tid / LPP, r = tid % LPP;
+    const int64_t chunk = blockIdx.x / LPP, sub = blockIdx.x % LPP;
+    const int64_t start = chunks ? chunks[chunk] : chunk * EBLK;
+    const CloudDev cd = clouds[mcloud ? mcloud[start] : 0];
+    const int k = cd.k;
+    const int64_t m = start + sub * PPB + pi;
+    const bool act = m < cd.base + cd.n;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    int32_t* row = nullptr;
+    int32_t* mr = nullptr;
+    const int32_t base = (int32_t)cd.base;
+    constexpr int NJ = (K + LPP - 1) / LPP;  // slots per lane
+    uint64_t valid = 0;
+    if (act) {
+        p = mpts[m];
+        const int32_t s = __float_as_int(p.w);
+        row = knn + cd.row_off + (int64_t)(s - cd.base) * k;
+        mr = mrow + cd.row_off + (m - cd.base) * k;
+        int32_t nbv[NJ];
+#pragma unroll
+        for (int u = 0; u < NJ; ++u) {
+            const int j = r + u * LPP;
+            nbv[u] = j < k ? row[j] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < NJ; ++u) nbv[u] = nbv[u] >= 0 ? minv[nbv[u] + base] : -1;
+#pragma unroll
+        for (int u = 0; u < NJ; ++u) {
+            const int j = r + u * LPP;
+            if (nbv[u] >= 0) {
+                valid |= 1ull << j;
+                const float4 c = mpts[nbv[u]];
+                sx[j][pi] = c.x;
+                sy[j][pi] = c.y;
+                sz[j][pi] = c.z;
+            }
+            if (j < k) mr[j] = nbv[u];
+        }
+    }
+    // the valid mask of the point: OR over its LPP lanes
+#pragma unroll
+    for (int d = 1; d < LPP; d <<= 1) valid |= __shfl_xor(valid, d, 64);
+    __syncthreads();
+    uint64_t drop = 0;
+    if (act) {
+        const float x = p.x, y = p.y, z = p.z;
+        const float aa = alpha * alpha;
+        for (int u = NJ - 1; u >= 0; --u) {
+            const int j = r + u * LPP;
+            if (j >= K || !((valid >> j) & 1ull)) continue;
+            const float xi = sx[j][pi], yi = sy[j][pi], zi = sz[j][pi];
+            const float xo = 0.5f * (xi + x), yo = 0.5f * (yi + y), zo = 0.5f * (zi + z);
+            const float d = __builtin_sqrtf(sqd(xi, yi, zi, x, y, z)) / 2.0f;
+            if ((double)d < 1e-6) continue;  // kept
+            float ax = xi - x, ay = yi - y, az = zi - z;
+            normalize3(ax, ay, az);
+            float mxv = __builtin_fabsf(ax);
+            int ci = 0;
+            if (mxv < __builtin_fabsf(ay)) {
+                ci = 1;
+                mxv = __builtin_fabsf(ay);
+            }
+            if (mxv < __builtin_fabsf(az)) ci = 2;
+            float bx = 1.0f, by = 1.0f, bz = 1.0f;
+            if (ci == 0) bx = (-ay - az) / ax;
+            else if (ci == 1) by = (-ax - az) / ay;
+            else bz = (-ax - ay) / az;
+            normalize3(bx, by, bz);
+            float cx = ay * bz - by * az;
+            float cy = bx * az - ax * bz;
+            float cz = ax * by - bx * ay;
+            normalize3(cx, cy, cz);
+            const float tx = -rowt(ax, ay, az, 0.0f, xo, yo, zo);
+            const float ty = -rowt(bx, by, bz, 0.0f, xo, yo, zo);
+            const float tz = -rowt(cx, cy, cz, 0.0f, xo, yo, zo);
+            const float dd = d * d;
+            const float rb = d * fmaxf(1.0f, alpha) * 1.0001f + 4e-6f;
+            const float rb2 = rb * rb;
+            uint64_t mset = valid & ((1ull << j) - 1ull);
+            while (mset) {
+                const int mm = __builtin_ctzll(mset);
+                mset &= mset - 1ull;
+                const float px = sx[mm][pi], py = sy[mm][pi], pz = sz[mm][pi];
+                if (sqd(px, py, pz, xo, yo, zo) >= rb2) continue;
+                const float ex = rowt(ax, ay, az, tx, px, py, pz);
+                const float ey = rowt(bx, by, bz, ty, px, py, pz);
+                const float ez = rowt(cx, cy, cz, tz, px, py, pz);
+                const float v = ex * ex + ey * ey / aa + ez * ez / aa;
+                if (v < dd) {
+                    drop |= 1ull << j;
+                    row[j] = -1;
+                    break;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 1; d < LPP; d <<= 1) drop |= __shfl_xor(drop, d, 64);
+    if (act && r == 0) {
+        const uint64_t keep = valid & ~drop;
+        keepm[m] = keep;
+        kept[m] = (uint32_t)__popcll(keep);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mrow, const uint64_t* __restrict__ keepm,
                                                   int64_t n, const CloudDev* __restrict__ clouds,
                                                   const int32_t* __restrict__ mcloud, const uint32_t* __restrict__ off,
@@ -226,12 +346,12 @@ void run_egg(egt_ctx* h, float alpha) {
     const int32_t* mcl = h->nclouds > 1 ? h->mcloud.p : nullptr;
     for (const WorkClass& wc : classes) {
         const int64_t* chunks = h->nclouds > 1 ? h->work_lists.p + wc.first : nullptr;
-        const dim3 grid((unsigned)wc.count), blk(EBLK);
+        const dim3 grid((unsigned)wc.count), grid4((unsigned)(wc.count * 4)), blk(EBLK);
 #define EGG_ARGS h->mpts.p, h->minv.p, h->knn.p, h->dclouds.p, mcl, chunks, alpha, mrow, keepm, kept
-        if (wc.K <= 16) hipLaunchKernelGGL(k_egg<16>, grid, blk, 0, s, EGG_ARGS);
-        else if (wc.K <= 32) hipLaunchKernelGGL(k_egg<32>, grid, blk, 0, s, EGG_ARGS);
-        else if (wc.K <= 40) hipLaunchKernelGGL(k_egg<40>, grid, blk, 0, s, EGG_ARGS);
-        else hipLaunchKernelGGL(k_egg<64>, grid, blk, 0, s, EGG_ARGS);
+        if (wc.K <= 16) hipLaunchKernelGGL((k_egg<16>), grid, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 32) hipLaunchKernelGGL((k_egg_shared<32, 4>), grid4, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 40) hipLaunchKernelGGL((k_egg_shared<40, 4>), grid4, blk, 0, s, EGG_ARGS);
+        else hipLaunchKernelGGL((k_egg_shared<64, 4>), grid4, blk, 0, s, EGG_ARGS);
 #undef EGG_ARGS
         LAUNCH_CHECK();
     }

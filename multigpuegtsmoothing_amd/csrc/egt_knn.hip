@@ -22,6 +22,10 @@ namespace egt {
 namespace {
 
 constexpr int KBLK = 256;
+#ifndef EGT_KNN_SPLIT
+#define EGT_KNN_SPLIT 48
+#endif
+constexpr uint32_t SPLIT_COUNT = EGT_KNN_SPLIT;  // cells with more points are scanned child by child
 constexpr uint64_t KEY_EMPTY = ~0ull;
 
 struct KnnParams {
@@ -108,7 +112,7 @@ __device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
 }
 
 template <int K, bool PERSIST>
-__global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
+__global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 40 && !PERSIST ? 3 : 1))) k_knn(KnnParams P) {
     uint64_t cand = 0;
     uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0;
     // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
@@ -166,43 +170,14 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                     if (big) break;
                 }
             }
-            const float hcell = P.ba / (float)Bl;
-            for (int oi = 0; oi < 27; ++oi) {
-                const int t = c_shell_order[oi];
-                const int c3[3] = {cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1, cq[2] + (t / 9) - 1};
-                if (c3[0] < 0 || c3[1] < 0 || c3[2] < 0 || c3[0] >= Bl || c3[1] >= Bl || c3[2] >= Bl) continue;
-                bool inside = true, overlap = true;
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    const int lo = c3[ax] << level, hi = ((c3[ax] + 1) << level) - 1;
-                    inside = inside && lo >= plo[ax] && hi <= phi[ax];
-                    overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
-                }
-                if (inside) continue;
-                // Cell pruning: skip a cell whose box is farther from the query than the current bound (the
-                // k-th key distance once the list is full, else the search radius). The box distance is
-                // shrunk by 1e-6 (normalized units), far more than the rounding of a point's float distance
-                // or of its cell assignment, so no point of a skipped cell could have entered the list.
-                {
-                    const float bnd = L[K - 1] != KEY_EMPTY ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
-                    const float qv[3] = {q.x, q.y, q.z};
-                    float dd = 0.0f;
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) {
-                        const float lo = P.a + (float)c3[ax] * hcell;
-                        const float gap = fmaxf(fmaxf(lo - qv[ax], qv[ax] - (lo + hcell)) - 1e-6f, 0.0f);
-                        dd = __builtin_fmaf(gap, gap, dd);
-                    }
-                    if (dd > bnd) continue;
-                }
-                ++cells;
-                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level],
-                                             cell_key(cloud, morton_code(c3[0], c3[1], c3[2])));
-                const uint32_t pend = r.x + r.y;
-                // candidates are fetched CH at a time before any is used (one memory latency per chunk); the
-                // top-K under the (distance, index) order does not depend on the offering order
+            const float qv[3] = {q.x, q.y, q.z};
+            // Offers the points of Morton range [start, pend) (skipping those inside the previous box when the
+            // range's cell overlaps it). Candidates are fetched CH at a time before any is used (one memory
+            // latency per chunk); the top-K under the (distance, index) order does not depend on the offering
+            // order.
+            auto scan = [&](uint32_t start, uint32_t pend, bool overlap) {
                 constexpr int CH = 4;
-                for (uint32_t p0 = r.x; p0 < pend; p0 += CH) {
+                for (uint32_t p0 = start; p0 < pend; p0 += CH) {
                     float4 ev[CH];
                     uint32_t cc[CH];
 #pragma unroll
@@ -231,6 +206,70 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
                         }
                     }
                 }
+            };
+            // Cell test at level lv (cell coords c): skip when fully inside the previous box or when its box
+            // is farther from the query than the current bound (the k-th key distance once the list is full,
+            // else the search radius). The box distance is shrunk by 1e-6 (normalized units), far more than
+            // the rounding of a point's float distance or of its cell assignment, so no point of a skipped
+            // cell could have entered the list. Returns 0 skip, 1 scan, 2 scan with the previous-box filter.
+            auto cell_test = [&](int lv, const int c[3]) -> int {
+                bool inside = true, overlap = true;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int lo = c[ax] << lv, hi = ((c[ax] + 1) << lv) - 1;
+                    inside = inside && lo >= plo[ax] && hi <= phi[ax];
+                    overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
+                }
+                if (inside) return 0;
+                const float hc = P.ba / (float)(P.B >> lv);
+                const float bnd = L[K - 1] != KEY_EMPTY ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
+                float dd = 0.0f;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    const float lo = P.a + (float)c[ax] * hc;
+                    const float gap = fmaxf(fmaxf(lo - qv[ax], qv[ax] - (lo + hc)) - 1e-6f, 0.0f);
+                    dd = __builtin_fmaf(gap, gap, dd);
+                }
+                if (dd > bnd) return 0;
+                return overlap ? 2 : 1;
+            };
+            for (int oi = 0; oi < 27; ++oi) {
+                const int t = c_shell_order[oi];
+                const int c3[3] = {cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1, cq[2] + (t / 9) - 1};
+                if (c3[0] < 0 || c3[1] < 0 || c3[2] < 0 || c3[0] >= Bl || c3[1] >= Bl || c3[2] >= Bl) continue;
+                const int ct = cell_test(level, c3);
+                if (!ct) continue;
+                ++cells;
+                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level],
+                                             cell_key(cloud, morton_code(c3[0], c3[1], c3[2])));
+                // A populous cell is tested and scanned child by child, nearest child first (the child on the
+                // query's side along each axis, then across one, two and three mid-planes), so the list fills
+                // with near points early and the farther children can be pruned.
+                const bool split = level > 0 && r.y > SPLIT_COUNT;
+                int near = 0;
+                if (split) {
+                    const float hc = P.ba / (float)(P.B >> (level - 1));
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax)
+                        near |= (qv[ax] >= P.a + (float)(2 * c3[ax] + 1) * hc ? 1 : 0) << ax;
+                }
+                for (int oc = 0; oc < (split ? 8 : 1); ++oc) {
+                    uint32_t start = r.x, count = r.y;
+                    bool ov = ct == 2;
+                    if (split) {
+                        const int ch = near ^ ((0x76534210u >> (4 * oc)) & 7);  // xor masks 0,1,2,4,3,5,6,7
+                        const int cc3[3] = {2 * c3[0] + (ch & 1), 2 * c3[1] + ((ch >> 1) & 1), 2 * c3[2] + (ch >> 2)};
+                        const int cct = cell_test(level - 1, cc3);
+                        if (!cct) continue;
+                        ++cells;
+                        const uint2 rc = level_lookup(P.hkeys, P.hvals, P.lev[level - 1],
+                                                      cell_key(cloud, morton_code(cc3[0], cc3[1], cc3[2])));
+                        start = rc.x;
+                        count = rc.y;
+                        ov = cct == 2;
+                    }
+                    scan(start, start + count, ov);
+                }
             }
             if (level >= P.nlev - 1) break;
             // exactness: every unscanned point is farther than the current bound
@@ -238,12 +277,12 @@ __global__ void __launch_bounds__(KBLK) k_knn(KnnParams P) {
             const float bound = full ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
             const double R = __builtin_sqrt((double)bound) + 2e-6;
             const double hl = (double)P.ba / (double)Bl;
-            const double qv[3] = {q.x, q.y, q.z};
+            const double qd[3] = {q.x, q.y, q.z};
             bool exact = true;
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                if (cq[ax] - 1 >= 1 && !(R < qv[ax] - ((double)P.a + (double)(cq[ax] - 1) * hl))) exact = false;
-                if (cq[ax] + 1 <= Bl - 2 && !(R < ((double)P.a + (double)(cq[ax] + 2) * hl) - qv[ax])) exact = false;
+                if (cq[ax] - 1 >= 1 && !(R < qd[ax] - ((double)P.a + (double)(cq[ax] - 1) * hl))) exact = false;
+                if (cq[ax] + 1 <= Bl - 2 && !(R < ((double)P.a + (double)(cq[ax] + 2) * hl) - qd[ax])) exact = false;
             }
             if (exact) break;
             if (P.qlevel && level >= own_level) fallback_q = 1;  // the proposed box did not certify the list

@@ -62,20 +62,15 @@ __global__ void __launch_bounds__(TBLK) k_taubin_pass(const float4* __restrict__
 }
 
 // ---------------------------------------------------------------------------------------------------
-// LDS-tiled pass. A tile is TILE consecutive Morton positions; its 1-hop halo (neighbours outside the tile,
-// deduplicated and sorted) is listed once per smoothing call, and every CSR column is rewritten as a 16-bit
-// local index (< TILE: inside the tile; TILE + h: halo entry h). A pass then stages tile + halo coordinates
-// in LDS and reads all neighbours from LDS. Tiles whose halo exceeds HALO_MAX use global gathers.
-#ifndef EGT_TAUBIN_TILE
-#define EGT_TAUBIN_TILE 1024
-#endif
-#ifndef EGT_TAUBIN_THREADS
-#define EGT_TAUBIN_THREADS 1024
-#endif
-constexpr int TILE = EGT_TAUBIN_TILE;
-constexpr int HALO_MAX = TILE;
+// LDS-tiled pass. A tile is TILE consecutive Morton positions (TILE = 1024 or 2048, 1024 threads, so one or
+// two points per thread); its 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most
+// TILE entries) is listed once per smoothing call, and every CSR column is rewritten as a 16-bit local index
+// (< TILE: inside the tile; TILE + h: halo entry h). A pass then stages tile + halo coordinates in LDS and
+// reads all neighbours from LDS. Tiles whose halo exceeds TILE use global gathers. Two points per thread
+// (TILE 2048) give each thread two independent gather chains; on a 4M-point torus at k=40 a pass takes
+// 57 ms / 600 vs 75 ms / 600 with one point per thread.
 constexpr int PBLK = 256;          // tile preparation
-constexpr int TBLK_TILE = EGT_TAUBIN_THREADS;  // tiled pass: TILE / TBLK_TILE points per thread
+constexpr int TBLK_TILE = 1024;  // tiled pass: TILE / TBLK_TILE points per thread
 constexpr int SORT_CAP = 8192;
 
 __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
@@ -97,6 +92,7 @@ __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
     }
 }
 
+template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                      int64_t n, uint16_t* __restrict__ col16,
                                                      int32_t* __restrict__ halo, int32_t* __restrict__ hcount) {
@@ -148,11 +144,11 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     for (int i = c0; i < c1; ++i)
         if (i == 0 || raw[i] != raw[i - 1]) uniq[off++] = raw[i];
     __syncthreads();
-    if (H > HALO_MAX) {
+    if (H > TILE) {
         if (threadIdx.x == 0) hcount[t] = -1;
         return;
     }
-    for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * HALO_MAX + h] = (int32_t)uniq[h];
+    for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * TILE + h] = (int32_t)uniq[h];
     for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
         const uint32_t c = (uint32_t)col[e];
         uint16_t loc;
@@ -196,6 +192,7 @@ __device__ inline void taubin_point(const float4 p, const float4* __restrict__ s
     }
 }
 
+template <int TILE>
 __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restrict__ in, float4* __restrict__ out,
                                                        const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ col,
@@ -212,7 +209,7 @@ __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restr
     for (int i = threadIdx.x; i <= end - base; i += TBLK_TILE) srow[i] = rowptr[base + i];
     if (H >= 0) {
         for (int i = threadIdx.x; i < end - base; i += TBLK_TILE) lds[i] = in[base + i];
-        for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * HALO_MAX + h]];
+        for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * TILE + h]];
     }
     __syncthreads();
     for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
@@ -253,40 +250,108 @@ __global__ void __launch_bounds__(TBLK) k_writeback(const float4* __restrict__ p
 
 }  // namespace
 
+namespace {
+
+// Tile preparation for tile width TILE (once per smoothing call): halo lists and 16-bit local columns.
+struct TilePlan {
+    int tile = 0, ntiles = 0, maxH = 0;
+    int64_t fallback_tiles = 0;
+    uint16_t* col16 = nullptr;
+    int32_t* halo = nullptr;
+    int32_t* hcount = nullptr;
+};
+
+template <int TILE>
+TilePlan prepare_tiles(egt_ctx* h) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    TilePlan tp;
+    tp.tile = TILE;
+    tp.ntiles = ceil_div(n, TILE);
+    tp.col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
+    tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * TILE);
+    tp.hcount = h->scratch.get<int32_t>(21, tp.ntiles);
+    k_tile_prep<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, tp.col16, tp.halo, tp.hcount);
+    LAUNCH_CHECK();
+    std::vector<int32_t> hc(tp.ntiles);
+    HIP_CHECK(hipMemcpyAsync(hc.data(), tp.hcount, tp.ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (int32_t v : hc) {
+        if (v < 0) ++tp.fallback_tiles;
+        else tp.maxH = std::max(tp.maxH, v);
+    }
+    return tp;
+}
+
+void launch_pass(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
+    const size_t lds_bytes = (size_t)(tp.tile + tp.maxH) * sizeof(float4);
+    const dim3 tgrid(ceil_div(tp.ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
+    if (tp.tile == 2048)
+        k_taubin_tile<2048><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.col16, tp.halo,
+                                                                   tp.hcount, h->n, scale, tp.ntiles);
+    else
+        k_taubin_tile<1024><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.col16, tp.halo,
+                                                                   tp.hcount, h->n, scale, tp.ntiles);
+}
+
+TilePlan prepare(egt_ctx* h, int tile) { return tile == 2048 ? prepare_tiles<2048>(h) : prepare_tiles<1024>(h); }
+
+// Tile-width autotuner: prepares each candidate width and times three lambda/mu pass pairs on copies of
+// the points (the smoothing itself is untouched), keeping the faster width for this handle.
+int tune_tile(egt_ctx* h, const float4* A, float lambda, float mu) {
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    float4* C = h->scratch.get<float4>(22, n);
+    float4* D = h->scratch.get<float4>(23, n);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int best = 2048;
+    float best_ms = 0.0f;
+    for (int tile : {1024, 2048}) {
+        const TilePlan tp = prepare(h, tile);
+        HIP_CHECK(hipMemcpyAsync(C, A, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+        launch_pass(h, tp, C, D, lambda);  // warm-up
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; ++r) {
+            launch_pass(h, tp, D, C, mu);
+            launch_pass(h, tp, C, D, lambda);
+        }
+        HIP_CHECK(hipEventRecord(e1, s));
+        LAUNCH_CHECK();
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        h->stats.taubin_tile_ms[tile == 2048 ? 1 : 0] = ms / 6.0f;
+        if (tile == 1024 || ms < best_ms) {
+            best = tile;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+}  // namespace
+
 void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     float4* A = h->scratch.get<float4>(12, n);
     float4* Bf = h->scratch.get<float4>(13, n);
     HIP_CHECK(hipMemcpyAsync(A, h->mpts.p, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    if (h->taubin_tile == 0 && iters > 0) h->taubin_tile = tune_tile(h, A, lambda, mu);
+    const int tile = h->taubin_tile == 1024 ? 1024 : 2048;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s));
-
-    // tile preparation (once per call): halo lists and 16-bit local columns
-    const int ntiles = ceil_div(n, TILE);
-    uint16_t* col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
-    int32_t* halo = h->scratch.get<int32_t>(20, (size_t)ntiles * HALO_MAX);
-    int32_t* hcount = h->scratch.get<int32_t>(21, ntiles);
-    k_tile_prep<<<ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, col16, halo, hcount);
-    LAUNCH_CHECK();
-    std::vector<int32_t> hc(ntiles);
-    HIP_CHECK(hipMemcpyAsync(hc.data(), hcount, ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    int maxH = 0;
-    int64_t fallback_tiles = 0;
-    for (int32_t v : hc) {
-        if (v < 0) ++fallback_tiles;
-        else maxH = std::max(maxH, v);
-    }
-    h->taubin_fallback_tiles = fallback_tiles;
-    const size_t lds_bytes = (size_t)(TILE + maxH) * sizeof(float4);
-    const dim3 tgrid(ceil_div(ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
-    auto pass = [&](const float4* in, float4* out, float scale) {
-        k_taubin_tile<<<tgrid, tblk, lds_bytes, s>>>(in, out, h->rowptr.p, h->col.p, col16, halo, hcount, n, scale,
-                                                     ntiles);
-    };
+    const TilePlan tp = prepare(h, tile);
+    h->taubin_fallback_tiles = tp.fallback_tiles;
+    h->stats.taubin_tile = tile;
+    h->stats.taubin_fallback_tiles = (int32_t)tp.fallback_tiles;
+    auto pass = [&](const float4* in, float4* out, float scale) { launch_pass(h, tp, in, out, scale); };
     if (iters > 0) {
         if (h->taubin_graph) {
             hipGraph_t g;

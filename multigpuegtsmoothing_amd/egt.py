@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("EGT_LIB_PATH") or os.path.join(_HERE, "libegt.so")
 
 EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
-OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT = 0, 1, 2, 3, 4
+OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT, OPT_TAUBIN_TILE = 0, 1, 2, 3, 4, 5
 
 
 class EGTError(RuntimeError):
@@ -40,10 +40,13 @@ class Stats(ctypes.Structure):
                 ("msf_edges", ctypes.c_int64), ("msf_trees", ctypes.c_int64), ("msf_weight", ctypes.c_double),
                 ("knn_ms", ctypes.c_float), ("egg_ms", ctypes.c_float), ("taubin_ms", ctypes.c_float),
                 ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("propose_ms", ctypes.c_float),
-                ("knn_cells", ctypes.c_int64)]
+                ("knn_cells", ctypes.c_int64), ("taubin_tile_ms", ctypes.c_float * 2),
+                ("taubin_tile", ctypes.c_int32), ("taubin_fallback_tiles", ctypes.c_int32)]
 
     def as_dict(self):
-        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+        d = {f[0]: getattr(self, f[0]) for f in self._fields_}
+        d["taubin_tile_ms"] = list(d["taubin_tile_ms"])
+        return d
 
 
 _F32P = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

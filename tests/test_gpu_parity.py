@@ -219,3 +219,21 @@ def test_error_paths(egt_mod):
     bad[0][3] = np.nan
     with pytest.raises(E.EGTError):
         E.GridStructure(bad)
+
+
+@pytest.mark.parametrize("tile", [1024, 2048, 0])
+def test_taubin_tile_width_keeps_results(egt_mod, oracle_mod, tile):
+    E, O = egt_mod, oracle_mod
+    pts = D.torus(50_000, seed=61)
+    g = E.GridStructure(pts, 1, 1e-6, 1024, 4)
+    g.set_option(E.OPT_TAUBIN_TILE, tile)
+    g.GRIDCUDAKNNSELF_COMPACT(24, 1e8)
+    E.EGT_CUDA(g).PerformSmoothing(8, 0.5, -0.53, 0.65)
+    out = E.getOutputPoints(g)
+    if tile == 0:
+        _log("tile autotune ms/pass (1024, 2048):", g.stats()["taubin_tile_ms"])
+        assert all(v > 0 for v in g.stats()["taubin_tile_ms"])
+    ref = O.pipeline(*pts, 24, 8, 0.5, -0.53, 0.65, normals=False)
+    for a, b in zip(out, ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    g.close()
