@@ -69,9 +69,10 @@ __device__ inline float egt_expf(float x) {
 
 // egt_expf for the Taubin weight argument x = -|q - p|^2 <= 0: the same cubic near 0; on [-80, 0] the
 // range checks of egt_expf never fire and 2^n is a normal number, so the same reduction and polynomial
-// followed by an exact ldexp give identical results; other arguments go through egt_expf.
-__device__ inline float egt_expf_taubin(float x) {
-    if (egt_expf_small(x)) return egt_expf_cubic(x);
+// followed by an exact ldexp give identical results; other arguments go through egt_expf. Neighbour
+// distances are almost always in the cubic's range, so the cubic is computed unconditionally and the
+// wide path is a branch that a wave skips when none of its lanes needs it.
+__device__ inline float egt_expf_wide_nonpos(float x) {
     if (!(x >= -80.0f)) return egt_expf(x);
     const float n = __builtin_rintf(x * 1.44269502163f);
     float r = __builtin_fmaf(-n, 0.693145751953125f, x);
@@ -85,6 +86,11 @@ __device__ inline float egt_expf_taubin(float x) {
     p = __builtin_fmaf(p, r, 1.0f);
     p = __builtin_fmaf(p, r, 1.0f);
     return __builtin_ldexpf(p, (int)n);
+}
+__device__ inline float egt_expf_taubin(float x) {
+    float w = egt_expf_cubic(x);
+    if (!egt_expf_small(x)) w = egt_expf_wide_nonpos(x);
+    return w;
 }
 
 // Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van Loan sec. 8.5); returns the

@@ -24,51 +24,16 @@ __device__ inline int64_t xcd_block(int nblocks) {
     return (int64_t)(b % NXCD) * per + b / NXCD;
 }
 
-__global__ void __launch_bounds__(TBLK) k_taubin_pass(const float4* __restrict__ in, float4* __restrict__ out,
-                                                       const int32_t* __restrict__ rowptr,
-                                                       const int32_t* __restrict__ col, int64_t n, float scale,
-                                                       int nblocks) {
-    const int64_t lb = xcd_block(nblocks);
-    if (lb >= nblocks) return;
-    const int64_t i = lb * TBLK + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = in[i];
-    float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
-    const int32_t e0 = rowptr[i], e1 = rowptr[i + 1];
-    // CH column indices, then CH gathers, are issued before any use so the row costs ~2 memory latencies
-    // instead of 2 per neighbour; the accumulation below stays in slot order.
-    constexpr int CH = 16;
-    for (int32_t e = e0; e < e1; e += CH) {
-        int32_t c[CH];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) c[u] = e + u < e1 ? col[e + u] : -1;
-        float4 q[CH];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) q[u] = c[u] >= 0 ? in[c[u]] : p;
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            if (c[u] >= 0) {
-                const float dist = sqd(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z);
-                const float w = egt_expf_taubin(-dist);
-                cgx = __builtin_fmaf(w, q[u].x - p.x, cgx);
-                cgy = __builtin_fmaf(w, q[u].y - p.y, cgy);
-                cgz = __builtin_fmaf(w, q[u].z - p.z, cgz);
-                sum += w;
-            }
-        }
-    }
-    if (sum == 0.0f) sum = 1e-8f;
-    out[i] = make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
-}
-
 // ---------------------------------------------------------------------------------------------------
 // LDS-tiled pass. A tile is TILE consecutive Morton positions (TILE = 1024 or 2048, 1024 threads, so one or
-// two points per thread); its 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most
+// two rows per thread); its 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most
 // TILE entries) is listed once per smoothing call, and every CSR column is rewritten as a 16-bit local index
-// (< TILE: inside the tile; TILE + h: halo entry h). A pass then stages tile + halo coordinates in LDS and
-// reads all neighbours from LDS. Tiles whose halo exceeds TILE use global gathers. Two points per thread
-// (TILE 2048) give each thread two independent gather chains; on a 4M-point torus at k=40 a pass takes
-// 57 ms / 600 vs 75 ms / 600 with one point per thread.
+// (< TILE: inside the tile; TILE + h: halo entry h). The columns are then laid out as sliced ELLPACK: the
+// tile's rows sorted by length, 64 rows (one wave) per slice, slot-major within a slice. A pass stages tile
+// + halo coordinates in LDS and reads all neighbours from LDS; the lanes of a wave walk rows of nearly equal
+// length and each slot load is one coalesced 128-byte line. The pass is bound by VALU issue (the weight's
+// exp per neighbour), so equal-length lanes are what saves time. Tiles whose halo exceeds TILE use CSR rows
+// with global gathers.
 constexpr int PBLK = 256;          // tile preparation
 constexpr int TBLK_TILE = 1024;  // tiled pass: TILE / TBLK_TILE points per thread
 constexpr int SORT_CAP = 8192;
@@ -95,12 +60,16 @@ __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
 template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                      int64_t n, uint16_t* __restrict__ col16,
-                                                     int32_t* __restrict__ halo, int32_t* __restrict__ hcount) {
+                                                     int32_t* __restrict__ halo, int32_t* __restrict__ hcount,
+                                                     uint16_t* __restrict__ perm, uint32_t* __restrict__ width,
+                                                     uint32_t* __restrict__ padded) {
     __shared__ uint32_t raw[SORT_CAP];
     __shared__ uint32_t uniq[SORT_CAP];
     __shared__ uint32_t cnt, wsum[PBLK / 64];
+    __shared__ uint32_t swid[TILE / 64];
     const int64_t t = blockIdx.x;
     const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
+    const int np = (int)(end - base);
     const int32_t eb = rowptr[base], ee = rowptr[end];
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
@@ -114,7 +83,10 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     __syncthreads();
     const uint32_t total = cnt;
     if (total > SORT_CAP) {
-        if (threadIdx.x == 0) hcount[t] = -1;
+        if (threadIdx.x == 0) {
+            hcount[t] = -1;
+            padded[t] = 0;
+        }
         return;
     }
     int n2 = 1;
@@ -145,7 +117,10 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
         if (i == 0 || raw[i] != raw[i - 1]) uniq[off++] = raw[i];
     __syncthreads();
     if (H > TILE) {
-        if (threadIdx.x == 0) hcount[t] = -1;
+        if (threadIdx.x == 0) {
+            hcount[t] = -1;
+            padded[t] = 0;
+        }
         return;
     }
     for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * TILE + h] = (int32_t)uniq[h];
@@ -165,61 +140,135 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
         }
         col16[e] = loc;
     }
-    if (threadIdx.x == 0) hcount[t] = (int32_t)H;
+    // Lane order of the pass: rows sorted by length (ties by position), so the 64 rows of a slice have
+    // nearly equal lengths; a slice's width is its longest row.
+    __syncthreads();
+    for (int i = threadIdx.x; i < TILE; i += PBLK) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (i < np) key = ((uint32_t)min(rowptr[base + i + 1] - rowptr[base + i], (1 << 20) - 1) << 12) | (uint32_t)i;
+        raw[i] = key;
+    }
+    for (int s = threadIdx.x; s < TILE / 64; s += PBLK) swid[s] = 0;
+    __syncthreads();
+    block_bitonic_sort(raw, TILE);
+    for (int j = threadIdx.x; j < np; j += PBLK) {
+        const uint32_t li = raw[j] & 0xFFFu;
+        perm[t * TILE + j] = (uint16_t)li;
+        atomicMax(&swid[j >> 6], (uint32_t)(rowptr[base + li + 1] - rowptr[base + li]));
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < TILE / 64; s += PBLK) width[t * (TILE / 64) + s] = swid[s];
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int s = 0; s < TILE / 64; ++s) tot += 64u * swid[s];
+        padded[t] = tot;
+        hcount[t] = (int32_t)H;
+    }
 }
 
-// Neighbour loop over LDS coordinates: CH local columns are fetched before use so a row costs about one
-// global latency; accumulation stays in slot order.
-__device__ inline void taubin_point(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ c16,
-                                    int32_t e0, int32_t e1, float& cgx, float& cgy, float& cgz, float& sum) {
-    constexpr int CH = 16;
-    for (int32_t e = e0; e < e1; e += CH) {
+// Sliced-ELLPACK columns of one tile: slice s (sorted positions 64 s .. 64 s + 63) stores its width W_s
+// slots slot-major, slot u of lane l at 64 u + l, so a wave's load of one slot is 128 contiguous bytes.
+// Slots past a row's end hold 0xFFFF. Slot order within a row is the CSR (neighbour) order.
+template <int TILE>
+__global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ rowptr,
+                                                     const uint16_t* __restrict__ col16, int64_t n,
+                                                     const uint16_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ width,
+                                                     const uint64_t* __restrict__ toff,
+                                                     const int32_t* __restrict__ hcount, uint16_t* __restrict__ scol) {
+    const int64_t t = blockIdx.x;
+    if (hcount[t] < 0) return;
+    const int64_t base = t * TILE;
+    const int np = (int)(min<int64_t>(base + TILE, n) - base);
+    uint16_t* dst = scol + toff[t];
+    uint32_t sb = 0;
+    for (int s = 0; s < TILE / 64; ++s) {
+        const uint32_t W = width[t * (TILE / 64) + s];
+        for (uint32_t x = threadIdx.x; x < 64u * W; x += PBLK) {
+            const uint32_t u = x >> 6;
+            const int j = s * 64 + (int)(x & 63u);
+            uint16_t v = 0xFFFFu;
+            if (j < np) {
+                const int li = perm[t * TILE + j];
+                const int32_t e0 = rowptr[base + li];
+                if ((int32_t)u < rowptr[base + li + 1] - e0) v = col16[e0 + u];
+            }
+            dst[sb + x] = v;
+        }
+        sb += 64u * W;
+    }
+}
+
+// Neighbour loop of one row over LDS coordinates, its local columns at sc[64 u] (sliced ELLPACK, W slots,
+// W uniform over the wave). The columns of CH slots, then their LDS coordinates, are fetched before use, so
+// a chunk costs one global and one LDS latency (CH = 8: 16 slots in flight cost more in VGPRs, and so in
+// occupancy, than they save); accumulation stays in slot order. The weighted difference uses
+// w * (q - p) = (-w) * (p - q): q - p is exactly -(p - q), and an accumulator that starts at +0 never
+// becomes -0, so the sign of a zero difference cannot change a sum; the reuse saves three subtractions
+// per neighbour.
+__device__ inline void taubin_sell_row(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ sc,
+                                       int W, float& cgx, float& cgy, float& cgz, float& sum) {
+    constexpr int CH = 8;
+    for (int u0 = 0; u0 < W; u0 += CH) {
         uint32_t c[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) c[u] = e + u < e1 ? (uint32_t)c16[e + u] : 0xFFFFFFFFu;
+        for (int u = 0; u < CH; ++u) c[u] = u0 + u < W ? (uint32_t)sc[(u0 + u) * 64] : 0xFFFFu;
+        // every LDS read of the chunk is issued before the first use (padding slots read entry 0)
+        float4 qs[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (u0 + u < W) qs[u] = src[c[u] != 0xFFFFu ? c[u] : 0u];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (c[u] != 0xFFFFFFFFu) {
-                const float4 q = src[c[u]];
-                const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
+            if (c[u] != 0xFFFFu) {
+                const float4 q = qs[u];
+                const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                const float dist = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 const float w = egt_expf_taubin(-dist);
-                cgx = __builtin_fmaf(w, q.x - p.x, cgx);
-                cgy = __builtin_fmaf(w, q.y - p.y, cgy);
-                cgz = __builtin_fmaf(w, q.z - p.z, cgz);
+                cgx = __builtin_fmaf(-w, dx, cgx);
+                cgy = __builtin_fmaf(-w, dy, cgy);
+                cgz = __builtin_fmaf(-w, dz, cgz);
                 sum += w;
             }
         }
     }
 }
 
+__device__ inline float4 taubin_result(const float4 p, float cgx, float cgy, float cgz, float sum, float scale) {
+    if (sum == 0.0f) sum = 1e-8f;
+    return make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+}
+
+// One lambda or mu pass over one tile. Wave w of the block takes slices w and S - 1 - w (S = TILE / 64
+// slices in ascending width), so the waves of a block carry similar work; lane l of slice s handles the
+// row perm[64 s + l]. Results go back through LDS so the global writes are coalesced.
 template <int TILE>
 __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restrict__ in, float4* __restrict__ out,
                                                        const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ col,
-                                                       const uint16_t* __restrict__ col16,
+                                                       const uint16_t* __restrict__ scol,
+                                                       const uint64_t* __restrict__ toff,
+                                                       const uint32_t* __restrict__ width,
+                                                       const uint16_t* __restrict__ perm,
                                                        const int32_t* __restrict__ halo,
                                                        const int32_t* __restrict__ hcount, int64_t n, float scale,
                                                        int ntiles) {
+    constexpr int S = TILE / 64;
+    constexpr int NW = TBLK_TILE / 64;
+    static_assert(S <= 64 && S % NW == 0 && (S / NW) <= 2, "slices per wave");
     extern __shared__ float4 lds[];
-    __shared__ int32_t srow[TILE + 1];
+    __shared__ uint32_t sbase[S + 1];
     const int64_t t = xcd_block(ntiles);
     if (t >= ntiles) return;
     const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
+    const int np = (int)(end - base);
     const int32_t H = hcount[t];
-    for (int i = threadIdx.x; i <= end - base; i += TBLK_TILE) srow[i] = rowptr[base + i];
-    if (H >= 0) {
-        for (int i = threadIdx.x; i < end - base; i += TBLK_TILE) lds[i] = in[base + i];
-        for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * TILE + h]];
-    }
-    __syncthreads();
-    for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
-        const float4 p = H >= 0 ? lds[i - base] : in[i];
-        float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
-        const int32_t e0 = srow[i - base], e1 = srow[i - base + 1];
-        if (H >= 0) {
-            taubin_point(p, lds, col16, e0, e1, cgx, cgy, cgz, sum);
-        } else {
-            for (int32_t e = e0; e < e1; ++e) {
+    if (H < 0) {
+        // halo larger than the tile: CSR rows with global gathers
+        for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
+            const float4 p = in[i];
+            float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
+            for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
                 const float4 q = in[col[e]];
                 const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
                 const float w = egt_expf_taubin(-dist);
@@ -228,10 +277,46 @@ __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restr
                 cgz = __builtin_fmaf(w, q.z - p.z, cgz);
                 sum += w;
             }
+            out[i] = taubin_result(p, cgx, cgy, cgz, sum, scale);
         }
-        if (sum == 0.0f) sum = 1e-8f;
-        out[i] = make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+        return;
     }
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv == 0) {  // slice offsets within the tile's column block: exclusive scan of 64 * width
+        uint32_t v = lane < S ? 64u * width[t * S + lane] : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(v, d, 64);
+            if (lane >= d) v += o;
+        }
+        if (lane < S) sbase[lane + 1] = v;
+        if (lane == 0) sbase[0] = 0;
+    }
+    for (int i = threadIdx.x; i < np; i += TBLK_TILE) lds[i] = in[base + i];
+    for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * TILE + h]];
+    __syncthreads();
+    const uint16_t* tc = scol + toff[t];
+    float4 res[S / NW];
+    int row[S / NW];
+#pragma unroll
+    for (int r = 0; r < S / NW; ++r) {
+        const int s = r == 0 ? wv : S - 1 - wv;
+        const int j = s * 64 + lane;
+        row[r] = j < np ? (int)perm[t * TILE + j] : -1;
+        const float4 p = lds[row[r] < 0 ? 0 : row[r]];
+        float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
+        // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
+        const uint32_t sb0 = __builtin_amdgcn_readfirstlane(sbase[s]);
+        const int W = (int)((__builtin_amdgcn_readfirstlane(sbase[s + 1]) - sb0) >> 6);
+        taubin_sell_row(p, lds, tc + sb0 + lane, W, cgx, cgy, cgz, sum);
+        res[r] = taubin_result(p, cgx, cgy, cgz, sum, scale);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < S / NW; ++r)
+        if (row[r] >= 0) lds[row[r]] = res[r];
+    __syncthreads();
+    for (int i = threadIdx.x; i < np; i += TBLK_TILE) out[base + i] = lds[i];
 }
 
 // Writes the smoothed points back to the sorted-space SoA arrays and refreshes the Morton copy.
@@ -256,9 +341,14 @@ namespace {
 struct TilePlan {
     int tile = 0, ntiles = 0, maxH = 0;
     int64_t fallback_tiles = 0;
-    uint16_t* col16 = nullptr;
+    uint16_t* col16 = nullptr;   // CSR-order local columns
     int32_t* halo = nullptr;
     int32_t* hcount = nullptr;
+    uint16_t* perm = nullptr;    // per tile: sorted position -> local row
+    uint32_t* width = nullptr;   // per slice: width in slots
+    uint64_t* toff = nullptr;    // per tile: start of its column block in scol
+    uint16_t* scol = nullptr;    // sliced-ELLPACK local columns
+    int64_t scol_elems = 0;
 };
 
 template <int TILE>
@@ -271,15 +361,33 @@ TilePlan prepare_tiles(egt_ctx* h) {
     tp.col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
     tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * TILE);
     tp.hcount = h->scratch.get<int32_t>(21, tp.ntiles);
-    k_tile_prep<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, tp.col16, tp.halo, tp.hcount);
+    tp.perm = h->scratch.get<uint16_t>(0, (size_t)tp.ntiles * TILE);
+    tp.width = h->scratch.get<uint32_t>(1, (size_t)tp.ntiles * (TILE / 64));
+    uint32_t* padded = h->scratch.get<uint32_t>(2, tp.ntiles);
+    tp.toff = h->scratch.get<uint64_t>(3, tp.ntiles);
+    k_tile_prep<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, tp.col16, tp.halo, tp.hcount, tp.perm,
+                                                 tp.width, padded);
     LAUNCH_CHECK();
     std::vector<int32_t> hc(tp.ntiles);
+    std::vector<uint32_t> pad(tp.ntiles);
     HIP_CHECK(hipMemcpyAsync(hc.data(), tp.hcount, tp.ntiles * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pad.data(), padded, tp.ntiles * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    for (int32_t v : hc) {
-        if (v < 0) ++tp.fallback_tiles;
-        else tp.maxH = std::max(tp.maxH, v);
+    std::vector<uint64_t> off(tp.ntiles);
+    uint64_t tot = 0;
+    for (int t = 0; t < tp.ntiles; ++t) {
+        if (hc[t] < 0) ++tp.fallback_tiles;
+        else tp.maxH = std::max(tp.maxH, hc[t]);
+        off[t] = tot;  // padded sizes are multiples of 64 slots, so every slice starts 128-byte aligned
+        tot += pad[t];
     }
+    tp.scol_elems = (int64_t)tot;
+    tp.scol = h->scratch.get<uint16_t>(27, std::max<uint64_t>(tot, 64));
+    HIP_CHECK(hipMemcpyAsync(tp.toff, off.data(), tp.ntiles * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    k_tile_sell<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, tp.col16, n, tp.perm, tp.width, tp.toff, tp.hcount,
+                                                 tp.scol);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipStreamSynchronize(s));  // the host offsets above must outlive the copy
     return tp;
 }
 
@@ -287,11 +395,13 @@ void launch_pass(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, 
     const size_t lds_bytes = (size_t)(tp.tile + tp.maxH) * sizeof(float4);
     const dim3 tgrid(ceil_div(tp.ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
     if (tp.tile == 2048)
-        k_taubin_tile<2048><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.col16, tp.halo,
-                                                                   tp.hcount, h->n, scale, tp.ntiles);
+        k_taubin_tile<2048><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
+                                                                   tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
+                                                                   tp.ntiles);
     else
-        k_taubin_tile<1024><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.col16, tp.halo,
-                                                                   tp.hcount, h->n, scale, tp.ntiles);
+        k_taubin_tile<1024><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
+                                                                   tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
+                                                                   tp.ntiles);
 }
 
 TilePlan prepare(egt_ctx* h, int tile) { return tile == 2048 ? prepare_tiles<2048>(h) : prepare_tiles<1024>(h); }
