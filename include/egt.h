@@ -137,6 +137,12 @@ egt_status egt_get_stats(egt_handle h, egt_stats* st);
 /* Tuning switches (no reference counterpart). */
 egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value);
 
+/* Device memory cache (no reference counterpart; the reference allocates and frees per call). Memory a
+ * destroyed handle held stays cached per device for later handles, up to EGT_DEVICE_CACHE_MB (environment,
+ * default 32768) per device. This call returns every cached block to the driver; *released (optional)
+ * receives the bytes returned. *cached (optional) receives the bytes cached before the call. */
+egt_status egt_release_cached_memory(uint64_t* released, uint64_t* cached);
+
 /* Autotuner (no reference counterpart): runs the plain search with neighbour count k and radius thres
  * (input units, as egt_grid_knn_self) on an evenly spread sample of queries for each candidate start level
  * (the cell size of the first search box), scores each level by its measured work per query (candidates

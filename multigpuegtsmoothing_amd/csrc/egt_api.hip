@@ -229,6 +229,14 @@ egt_status egt_get_stats(egt_handle h, egt_stats* st) {
     return EGT_OK;
 }
 
+egt_status egt_release_cached_memory(uint64_t* released, uint64_t* cached) {
+    return guarded([&] {
+        if (cached) *cached = egt::device_cache_bytes();
+        const size_t r = egt::device_cache_release_all();
+        if (released) *released = r;
+    });
+}
+
 egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
     switch (opt) {

@@ -29,29 +29,35 @@ struct ArgError {
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
 
+// Device memory cache (egt_memory.hip): device_alloc returns a block of at least `bytes` (its capacity in
+// *capacity), reusing a block an earlier handle released; device_release hands a block back (the caller
+// guarantees no work in flight uses it).
+void* device_alloc(size_t bytes, size_t* capacity);
+void device_release(void* p, size_t capacity);
+size_t device_cache_release_all();
+size_t device_cache_bytes();
+
 // Device allocation owned by a handle.
 template <typename T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;    // elements usable
+    size_t cap = 0;  // bytes of the block
     void alloc(size_t count) {
         if (count <= n && p) return;
         free();
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw HipError{e, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes"};
-        }
-        n = count;
+        p = static_cast<T*>(device_alloc(count * sizeof(T), &cap));
+        n = cap / sizeof(T);
     }
     void free() {
         if (p) {
             (void)hipDeviceSynchronize();  // never release memory an in-flight kernel may still touch
-            (void)hipFree(p);
+            device_release(p, cap);
         }
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     ~DBuf() { free(); }
 };

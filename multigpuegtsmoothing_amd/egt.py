@@ -97,6 +97,7 @@ def _load():
                             ctypes.POINTER(i64)], st),
         "egt_get_stats": ([_H, ctypes.POINTER(Stats)], st),
         "egt_set_option": ([_H, i32, i64], st),
+        "egt_release_cached_memory": ([ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)], st),
         "egt_destroy": ([_H], st),
         "egt_get_stream": ([_H], ctypes.c_void_p),
         "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N, _F32N], st),
@@ -478,6 +479,14 @@ def model_levels(provider):
     out = np.empty(provider.pointsRefCount(), np.int8)
     _check(_lib.egt_model_levels(provider.handle, out))
     return out
+
+
+def release_cached_memory():
+    """Returns the device blocks that destroyed handles left cached (egt_release_cached_memory);
+    gives (bytes released, bytes that were cached)."""
+    rel, cached = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(_lib.egt_release_cached_memory(ctypes.byref(rel), ctypes.byref(cached)))
+    return rel.value, cached.value
 
 
 def device_array(ptr, count, dtype):
