@@ -27,6 +27,12 @@ constexpr int KBLK = 256;
 #endif
 constexpr uint32_t SPLIT_COUNT = EGT_KNN_SPLIT;  // cells with more points are scanned child by child
 constexpr uint64_t KEY_EMPTY = ~0ull;
+// Per-lane buffer of keys awaiting insertion and candidates fetched per step, by list length (measured on
+// a 4M-point torus: K = 16 runs fastest with 4 and 2, K = 40 and 64 with 8 and 4).
+template <int K>
+constexpr int knn_buffer() { return K <= 16 ? 4 : 8; }
+template <int K>
+constexpr int knn_fetch() { return K <= 16 ? 2 : 4; }
 
 struct KnnParams {
     const float4* __restrict__ mpts;
@@ -144,6 +150,23 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
         uint64_t L[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) L[i] = (i < K - kq) ? 0ull : KEY_EMPTY;
+        // Keys that beat the list's bound wait in a small per-lane buffer and are inserted in batches: the
+        // wave executes an insertion step only as often as its busiest lane needs one, instead of at every
+        // candidate where any lane inserts. The list (and so the bound used for pruning) lags by at most
+        // the buffered keys, which only makes pruning less tight, never wrong.
+        constexpr int KNB = knn_buffer<K>();
+        uint64_t pbuf[KNB];
+        int npend = 0;
+#pragma unroll
+        for (int j = 0; j < KNB; ++j) pbuf[j] = KEY_EMPTY;
+        auto flush = [&]() {
+#pragma unroll
+            for (int j = 0; j < KNB; ++j) {
+                if (j < npend) topk_insert<K>(L, pbuf[j]);
+                pbuf[j] = KEY_EMPTY;
+            }
+            npend = 0;
+        };
 
         int level = wave_level;
         level = level < 0 ? 0 : (level >= P.nlev ? P.nlev - 1 : level);
@@ -176,7 +199,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
             // latency per chunk); the top-K under the (distance, index) order does not depend on the offering
             // order.
             auto scan = [&](uint32_t start, uint32_t pend, bool overlap) {
-                constexpr int CH = 4;
+                constexpr int CH = knn_fetch<K>();
                 for (uint32_t p0 = start; p0 < pend; p0 += CH) {
                     float4 ev[CH];
                     uint32_t cc[CH];
@@ -202,9 +225,14 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                         const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
                         if (d <= thres_n) {
                             const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
-                            if (kk < L[K - 1]) topk_insert<K>(L, kk);
+                            if (kk < L[K - 1]) {
+#pragma unroll
+                                for (int b = 0; b < KNB; ++b) pbuf[b] = b == npend ? kk : pbuf[b];
+                                ++npend;
+                            }
                         }
                     }
+                    if (__ballot(npend > KNB - CH)) flush();  // room for the next CH candidates
                 }
             };
             // Cell test at level lv (cell coords c): skip when fully inside the previous box or when its box
@@ -271,6 +299,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                     scan(start, start + count, ov);
                 }
             }
+            flush();
             if (level >= P.nlev - 1) break;
             // exactness: every unscanned point is farther than the current bound
             const bool full = L[K - 1] != KEY_EMPTY;
