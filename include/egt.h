@@ -55,6 +55,11 @@ typedef struct {
     float taubin_tile_ms[2];   /* Taubin tile autotuner: ms per pass at tile width 1024 and 2048 (0 if untried) */
     int32_t taubin_tile;       /* tile width of the last smoothing call */
     int32_t taubin_fallback_tiles; /* tiles of that call whose halo did not fit in LDS (global gathers) */
+    int64_t query_count;       /* queries of the last query-vs-reference search */
+    int64_t query_candidates;  /* candidate distances it evaluated */
+    int64_t query_invalid_rows; /* its rows blanked by the reference's validity rule */
+    float query_ms;            /* its device time (query sort + search) */
+    int32_t query_k;           /* its neighbour count */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */
@@ -88,6 +93,30 @@ egt_status egt_grid_create_device(const float* dx, const float* dy, const float*
  * thres in input units (main.cpp:109 passes 1e8). Rows are the exact k nearest neighbours ordered by
  * (distance, sorted index); rows the reference would blank are all -1. */
 egt_status egt_grid_knn_self(egt_handle h, int k, float thres);
+
+/* GridStructure::GRIDCUDAKNNCompact(float* qx, float* qy, float* qz, int pointsCountQuery, int k, float thres),
+ * GridKNNCuda.cuh:19, GridKNNCuda.cu:1645-1939 (kernel construct_neighbors :340-440): the k nearest reference
+ * points of each query. Host query arrays in input units and original order, finite coordinates. Queries are
+ * normalized with the reference cloud's extent. Candidates need distance < thres (strict), and no point is
+ * excluded. Rows are the exact k nearest reference points ordered by (distance, reference sorted index);
+ * rows the reference would blank are all -1. For a query outside the reference's bounding cube, the
+ * reference's ring schedule can skip cells and return a list that misses nearer points; this call returns the
+ * exact list there. The query rows are kept beside the self rows: smoothing and normals keep using the self
+ * rows. */
+egt_status egt_grid_knn_query(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq, int k,
+                              float thres);
+/* As egt_grid_knn_query with the query coordinates already in device memory (no reference counterpart:
+ * the reference always copies from the host, GridKNNCuda.cu:1693-1703). */
+egt_status egt_grid_knn_query_device(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq,
+                                     int k, float thres);
+/* GridStructure::GetKNNIndexes(int* knn) after a query search, GridKNNCuda.cu:1941-1968: host nq*k int32,
+ * original query order, original reference indices, -1 holes. */
+egt_status egt_get_knn_query_original(egt_handle h, int32_t* out);
+/* KNNInterface::GetKNNIndexesInPartitions() after a query search, KNNInterface.cuh:20 (device rows, reference
+ * sorted indices). Borrowed pointer valid until the next query search or egt_destroy. The rows are in
+ * original query order (the reference keeps them in its own query sort order, GridKNNCuda.cu:1208-1267).
+ * nq and k (optional) receive the row count and width; pointsQuerCount, KNNInterface.cuh:29. */
+egt_status egt_get_device_query_knn(egt_handle h, int32_t** rows, int64_t* nq, int32_t* k);
 
 /* GridMeshCuda(const Mesh&, int num_cards, float eps, int bucketCount), GridMeshCuda1.cuh:16,
  * GridMeshCuda1.cu:247-414. tri: ntri x 3 uint32 original vertex indices (Mesh.h:15-19). */

@@ -3,7 +3,8 @@
 // gfx950 library with one GPU:
 //
 //   KNNInterface                 KNNInterface.cuh:8-31
-//   GridStructure                GridKNNCuda.cuh:13-43 (constructor, GRIDCUDAKNNSELF_COMPACT, accessors)
+//   GridStructure                GridKNNCuda.cuh:13-43 (constructor, GRIDCUDAKNNSELF_COMPACT,
+//                                GRIDCUDAKNNCompact, accessors)
 //   GridMeshCuda                 GridMeshCuda1.cuh:13-30 (constructor, GetNeighborsFromTriangles, accessors)
 //   EGT_CUDA                     EGT_CUDA.cuh:11-15
 //   Normal_CUDA                  Normal_CUDA.cuh:10-14
@@ -182,7 +183,44 @@ public:
         egt::check(egt_grid_create(s.x.data(), s.y.data(), s.z.data(), (int64_t)mesh.nV(), num_cards, eps,
                                    bucketCount, maxDivision, &h_));
     }
-    void GRIDCUDAKNNSELF_COMPACT(int k, float thres) { egt::check(egt_grid_knn_self(h_, k, thres)); }
+    void GRIDCUDAKNNSELF_COMPACT(int k, float thres) {
+        egt::check(egt_grid_knn_self(h_, k, thres));
+        query_ = false;
+    }
+    // GRIDCUDAKNNCompact GridKNNCuda.cuh:19: k nearest reference points of each query (host arrays, input
+    // units). Afterwards the row accessors describe the query rows, as in the reference.
+    void GRIDCUDAKNNCompact(float* pointsQ_x, float* pointsQ_y, float* pointsQ_z, int pointsCountQuery, int k,
+                            float thres) {
+        egt::check(egt_grid_knn_query(h_, pointsQ_x, pointsQ_y, pointsQ_z, pointsCountQuery, k, thres));
+        query_ = true;
+    }
+    void GetKNNIndexes(int* knnIndexes) override {
+        if (query_) egt::check(egt_get_knn_query_original(h_, knnIndexes));
+        else KNNInterface::GetKNNIndexes(knnIndexes);
+    }
+    int** GetKNNIndexesInPartitions() override {
+        if (!query_) return KNNInterface::GetKNNIndexesInPartitions();
+        int32_t* p = nullptr;
+        egt::check(egt_get_device_query_knn(h_, &p, nullptr, nullptr));
+        qknn_ = p;
+        return &qknn_;
+    }
+    int pointsQuerCount() const override {
+        if (!query_) return pointsRefCount();
+        int64_t nq = 0;
+        egt::check(egt_get_device_query_knn(h_, nullptr, &nq, nullptr));
+        return (int)nq;
+    }
+    int NeighborCount() const override {
+        if (!query_) return KNNInterface::NeighborCount();
+        int32_t k = 0;
+        egt::check(egt_get_device_query_knn(h_, nullptr, nullptr, &k));
+        return k;
+    }
+
+private:
+    bool query_ = false;
+    int* qknn_ = nullptr;
 };
 
 class GridMeshCuda : public KNNInterface {

@@ -117,6 +117,68 @@ egt_status egt_grid_knn_self(egt_handle h, int k, float thres) {
     });
 }
 
+namespace {
+egt_status knn_query(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq, int k, float thres,
+                     bool device) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind == 2) return fail(EGT_ERR_STATE, "a batch handle has no query search");
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "the query search needs a point-cloud handle");
+    if (nq < 0) return fail(EGT_ERR_INVALID_ARG, "query count must be >= 0");
+    if (nq > 0 && (!qx || !qy || !qz)) return fail(EGT_ERR_INVALID_ARG, "null query array");
+    if (nq > INT32_MAX) return fail(EGT_ERR_UNSUPPORTED, "more than 2^31 - 1 queries");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    if (!device) {
+        for (int64_t i = 0; i < nq; ++i)
+            if (!std::isfinite(qx[i]) || !std::isfinite(qy[i]) || !std::isfinite(qz[i]))
+                return fail(EGT_ERR_INVALID_ARG, "query coordinates must be finite");
+    }
+    return guarded([&] {
+        // GridKNNCuda.cu:1649-1650
+        float t = thres / (h->max_extent - h->min_extent);
+        t *= t;
+        if (h->search_stale) egt::refresh_search_structure(h);
+        const float *dx = qx, *dy = qy, *dz = qz;
+        if (!device && nq > 0) {
+            float* tmp = h->scratch.get<float>(16, 3 * (size_t)nq);
+            HIP_CHECK(hipMemcpyAsync(tmp, qx, nq * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + nq, qy, nq * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            HIP_CHECK(hipMemcpyAsync(tmp + 2 * nq, qz, nq * sizeof(float), hipMemcpyHostToDevice, h->stream));
+            dx = tmp;
+            dy = tmp + nq;
+            dz = tmp + 2 * nq;
+        }
+        egt::run_knn_query(h, dx, dy, dz, nq, k, t);
+    });
+}
+}  // namespace
+
+egt_status egt_grid_knn_query(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq, int k,
+                              float thres) {
+    return knn_query(h, qx, qy, qz, nq, k, thres, false);
+}
+
+egt_status egt_grid_knn_query_device(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq,
+                                     int k, float thres) {
+    return knn_query(h, qx, qy, qz, nq, k, thres, true);
+}
+
+egt_status egt_get_knn_query_original(egt_handle h, int32_t* out) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (!h->have_query) return fail(EGT_ERR_STATE, "no query search yet");
+    if (!out && h->nq > 0) return fail(EGT_ERR_INVALID_ARG, "null output array");
+    return guarded([&] { egt::run_knn_query_original(h, out); });
+}
+
+egt_status egt_get_device_query_knn(egt_handle h, int32_t** rows, int64_t* nq, int32_t* k) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (rows) *rows = h->have_query ? h->qknn.p : nullptr;
+    if (nq) *nq = h->have_query ? h->nq : 0;
+    if (k) *k = h->have_query ? h->kq : 0;
+    return EGT_OK;
+}
+
 egt_status egt_mesh_create(const float* x, const float* y, const float* z, int64_t n, const uint32_t* tri,
                            int64_t ntri, int num_cards, float eps, int bucket_count, egt_handle* out) {
     egt_status st = check_common(n, num_cards, bucket_count, out);

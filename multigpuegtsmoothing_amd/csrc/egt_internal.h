@@ -91,9 +91,10 @@ struct WorkClass {
 };
 
 // Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code (15 = k-th
-// distance of the last search), 16 host-input staging, 17-21 EGG / Taubin, 24-26 learned proposal.
+// distance of the last search), 16 host-input staging, 17-23 EGG / Taubin, 24-26 learned proposal,
+// 27-31 Taubin sliced-ELLPACK tiles, 32-35 query-vs-reference search.
 struct Scratch {
-    DBuf<uint32_t> bufs[28];
+    DBuf<uint32_t> bufs[36];
     template <class T>
     T* get(int i, size_t count) {
         bufs[i].alloc((count * sizeof(T) + sizeof(uint32_t) - 1) / sizeof(uint32_t));
@@ -157,6 +158,15 @@ struct egt_ctx {
     // learned proposal
     egt::DBuf<int8_t> qlevel;
 
+    // query-vs-reference search (its rows live beside the self rows, which smoothing keeps using)
+    egt::DBuf<float4> qpts;       // queries in Morton order of their cell: x', y', z', bits(original index)
+    egt::DBuf<uint32_t> qcell;    // packed fine cell, clamped to the grid
+    egt::DBuf<int32_t> qknn;      // nq*kq rows in original query order, reference sorted indices
+    egt::DBuf<egt::CloudDev> dqcloud;
+    int64_t nq = 0;
+    int kq = 0;
+    bool have_query = false;
+
     // options
     int start_level = 0;  // reference schedule: every query starts at its own level-0 cell
     int learned = 0;
@@ -199,6 +209,9 @@ void refresh_search_structure(egt_ctx* h);
 
 // kNN (egt_knn.hip)
 void run_knn(egt_ctx* h, int k, float thres_n);
+void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* dqz, int64_t nq, int k,
+                   float thres_n);
+void run_knn_query_original(egt_ctx* h, int32_t* out);
 void run_knn_clouds(egt_ctx* h);
 int auto_start_level(const egt_ctx* h, int k, int cloud = 0);
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost);

@@ -51,6 +51,8 @@ struct KnnParams {
     unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks,
                                                 // [4] cells looked up
     const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
+    const float4* __restrict__ qpts;       // query search: queries in Morton order (x', y', z', bits(index))
+    const uint32_t* __restrict__ qcell;    // query search: packed fine cell of each query, clamped to the grid
     int64_t nchunks;   // chunks in the work list (after sampling)
     int stride;        // 1 = every chunk; s > 1 = every s-th chunk of the list (sampled trial runs)
     int B, log2B, nlev, start_level, stages, maxc;
@@ -69,22 +71,26 @@ __device__ inline void topk_insert(uint64_t (&L)[K], uint64_t key) {
     L[0] = key < L[0] ? key : L[0];
 }
 
-// Reference certificate (GridKNNCuda.cu:694-741) at stage s: true = the reference keeps searching.
-__device__ inline bool ref_continue(const KnnParams& P, int kq, float thres_n, int s, float x, float y, float z, int fx,
-                                    int fy, int fz, int c, float kthd) {
+// Reference certificate at stage s (self search GridKNNCuda.cu:694-741, query search :383-433) for a query
+// whose cell at the stage's level is ci: true = the reference keeps searching. With fewer than k listed the
+// self search compares the radius with > and the query search with >=.
+template <bool QUERY>
+__device__ inline bool ref_continue(const KnnParams& P, int kq, float thres_n, int s, float x, float y, float z,
+                                    const int (&ci)[3], int c, float kthd) {
     const int lvl = s == 0 ? 0 : s - 1;
     const int off = s == 0 ? 0 : 1;
     const int Bl = P.B >> lvl;
     const float h = P.ba / (float)Bl;
     const float v[3] = {x, y, z};
-    const int ci[3] = {fx >> lvl, fy >> lvl, fz >> lvl};
     bool cont = false;
     if (c < kq) {
         const float r = __builtin_sqrtf(thres_n);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            if (ci[ax] - off >= 0 && r > __builtin_fmaf(-(float)(ci[ax] - off), h, v[ax] - P.a)) cont = true;
-            if (ci[ax] + off + 1 <= Bl && r > __builtin_fmaf((float)(ci[ax] + off + 1), h, P.a - v[ax])) cont = true;
+            const float lo = __builtin_fmaf(-(float)(ci[ax] - off), h, v[ax] - P.a);
+            const float hi = __builtin_fmaf((float)(ci[ax] + off + 1), h, P.a - v[ax]);
+            if (ci[ax] - off >= 0 && (QUERY ? r >= lo : r > lo)) cont = true;
+            if (ci[ax] + off + 1 <= Bl && (QUERY ? r >= hi : r > hi)) cont = true;
         }
     } else {
         const float r = __builtin_sqrtf(kthd);
@@ -117,7 +123,15 @@ __device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
     return (int64_t)__shfl(v, 0, 64);
 }
 
-template <int K, bool PERSIST>
+// QUERY = false: self search of the reference points (GRIDCUDAKNNSELF_COMPACT). QUERY = true: search of a
+// separate query set against the reference points (GRIDCUDAKNNCompact, GridKNNCuda.cu:1645-1939, kernel
+// construct_neighbors :340-440), which differs in four places: no point is excluded, candidates need
+// d < thres (strict), the reference computes a query's cell at every level from its coordinates (it can lie
+// outside the grid, and (int) truncates toward zero) and its short-list certificate uses >=. Queries come
+// from qpts / qcell (Morton order of their cell clamped to the grid); rows go to knn at the query's original
+// index. A query outside the grid skips the early exit below: its search boxes (around the clamped cell) are
+// not the reference's stage boxes, so it searches until the list is exact.
+template <int K, bool PERSIST, bool QUERY = false>
 __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 40 && !PERSIST ? 3 : 1))) k_knn(KnnParams P) {
     uint64_t cand = 0;
     uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0;
@@ -143,10 +157,16 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
         learned += active ? 1u : 0u;
     }
     if (active) {
-        const float4 q = P.mpts[m];
+        const float4 q = QUERY ? P.qpts[m] : P.mpts[m];
         const int32_t jq = __float_as_int(q.w);
-        const uint32_t qc = P.mcell[m];
+        const uint32_t qc = QUERY ? P.qcell[m] : P.mcell[m];
         const int fx = qc & 1023, fy = (qc >> 10) & 1023, fz = qc >> 20;
+        bool outside = false;  // query search: the query's own (unclamped) cell lies outside the grid
+        if (QUERY) {
+            const int ux = cell_of(q.x, P.a, P.ba, P.B), uy = cell_of(q.y, P.a, P.ba, P.B),
+                      uz = cell_of(q.z, P.a, P.ba, P.B);
+            outside = ux < 0 || uy < 0 || uz < 0 || ux >= P.B || uy >= P.B || uz >= P.B;
+        }
         uint64_t L[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) L[i] = (i < K - kq) ? 0ull : KEY_EMPTY;
@@ -181,7 +201,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
             // list would. If stage l + 1 does not exist (l >= stages) or would meet a cell of >= maxc points
             // (the reference then invalidates the query), nothing beyond the scanned box can change the
             // outcome: the replay below either stops at an earlier stage with this list or blanks the row.
-            if (level > first_level) {
+            if (level > first_level && !outside) {
                 if (level >= P.stages) break;
                 if ((P.bigmask >> level) & 1u) {
                     bool big = false;
@@ -214,7 +234,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                         if (p0 + u >= pend) continue;
                         const float4 e = ev[u];
                         const int32_t j = __float_as_int(e.w);
-                        if (j == jq) continue;
+                        if (!QUERY && j == jq) continue;
                         if (overlap) {
                             const int ex = cc[u] & 1023, ey = (cc[u] >> 10) & 1023, ez = cc[u] >> 20;
                             if (ex >= plo[0] && ex <= phi[0] && ey >= plo[1] && ey <= phi[1] && ez >= plo[2] &&
@@ -223,7 +243,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                         }
                         ++cand;
                         const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
-                        if (d <= thres_n) {
+                        if (QUERY ? d < thres_n : d <= thres_n) {
                             const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
                             if (kk < L[K - 1]) {
 #pragma unroll
@@ -343,7 +363,12 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
         for (int s = 0; s <= P.stages; ++s) {
             const int lvl = s == 0 ? 0 : s - 1;
             const int off = s == 0 ? 0 : 1;
-            const int cx = fx >> lvl, cy = fy >> lvl, cz = fz >> lvl;
+            int cx = fx >> lvl, cy = fy >> lvl, cz = fz >> lvl;
+            if (QUERY) {  // the reference's cell of the query at this level (GridKNNCuda.cu:351-353)
+                cx = cell_of(q.x, P.a, P.ba, P.B >> lvl);
+                cy = cell_of(q.y, P.a, P.ba, P.B >> lvl);
+                cz = cell_of(q.z, P.a, P.ba, P.B >> lvl);
+            }
             if ((P.bigmask >> lvl) & 1u) {
                 bool big = false;
                 if (s == 0) {
@@ -360,13 +385,14 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
             const bool inside = c == 0 || ((lo[0] >> lvl) >= cx - off && (hi[0] >> lvl) <= cx + off &&
                                            (lo[1] >> lvl) >= cy - off && (hi[1] >> lvl) <= cy + off &&
                                            (lo[2] >> lvl) >= cz - off && (hi[2] >> lvl) <= cz + off);
-            if (inside && !ref_continue(P, kq, thres_n, s, q.x, q.y, q.z, fx, fy, fz, c, kthd)) {
+            const int cst[3] = {cx, cy, cz};
+            if (inside && !ref_continue<QUERY>(P, kq, thres_n, s, q.x, q.y, q.z, cst, c, kthd)) {
                 valid = true;
                 break;
             }
         }
         int32_t* row = P.knn + cd.row_off + (int64_t)(jq - cd.base) * kq;
-        const int32_t base = (int32_t)cd.base;
+        const int32_t base = QUERY ? 0 : (int32_t)cd.base;  // query rows hold reference sorted indices
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             if (i >= K - kq) {
@@ -374,7 +400,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                 row[i - (K - kq)] = val;
             }
         }
-        P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
+        if (!QUERY) P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
         invalid_row += valid ? 0u : 1u;
         fallback += fallback_q;
     }
@@ -399,6 +425,49 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
     }
 }
 
+// Query preparation: each query normalized with the reference cloud's extent (normalize_points
+// GridKNNCuda.cu:841-849: x' = (x - min) / (max - min)), its fine cell clamped to the grid (the search boxes
+// start there) and the Morton code of that cell as the sort key, so a wave's 64 queries are spatial
+// neighbours.
+__global__ void __launch_bounds__(KBLK) k_query_prep(const float* __restrict__ qx, const float* __restrict__ qy,
+                                                      const float* __restrict__ qz, int64_t nq, float mn, float mx,
+                                                      float a, float ba, int B, float4* __restrict__ qn,
+                                                      uint32_t* __restrict__ cellp, uint32_t* __restrict__ code,
+                                                      uint32_t* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (i >= nq) return;
+    const float d = mx - mn;
+    const float x = (qx[i] - mn) / d, y = (qy[i] - mn) / d, z = (qz[i] - mn) / d;
+    const int cx = min(max(cell_of(x, a, ba, B), 0), B - 1);
+    const int cy = min(max(cell_of(y, a, ba, B), 0), B - 1);
+    const int cz = min(max(cell_of(z, a, ba, B), 0), B - 1);
+    qn[i] = make_float4(x, y, z, 0.0f);
+    cellp[i] = (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
+    code[i] = morton_code(cx, cy, cz);
+    idx[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(KBLK) k_query_gather(const float4* __restrict__ qn, const uint32_t* __restrict__ cellp,
+                                                        const uint32_t* __restrict__ idx, int64_t nq,
+                                                        float4* __restrict__ qpts, uint32_t* __restrict__ qcell) {
+    const int64_t m = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (m >= nq) return;
+    const uint32_t i = idx[m];
+    const float4 v = qn[i];
+    qpts[m] = make_float4(v.x, v.y, v.z, __int_as_float((int32_t)i));
+    qcell[m] = cellp[i];
+}
+
+// Query rows with original reference indices (GetKNNIndexes GridKNNCuda.cu:1941-1968 after a query search).
+__global__ void __launch_bounds__(KBLK) k_query_original(const int32_t* __restrict__ rows,
+                                                          const int32_t* __restrict__ perm, int64_t count,
+                                                          int32_t* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (e >= count) return;
+    const int32_t r = rows[e];
+    out[e] = r >= 0 ? perm[r] : -1;
+}
+
 int kclass(int k) {
     return k <= 8 ? 8 : k <= 16 ? 16 : k <= 24 ? 24 : k <= 32 ? 32 : k <= 40 ? 40 : k <= 48 ? 48 : k <= 56 ? 56 : 64;
 }
@@ -414,10 +483,12 @@ int device_cus() {
 }
 
 template <int K>
-void launch_knn(const KnnParams& P, hipStream_t s) {
+void launch_knn(const KnnParams& P, hipStream_t s, bool query) {
     int grid = ceil_div(P.nchunks, KBLK / 64);
     if (grid < 1) return;
-    if (P.queue) {
+    if (query) {
+        hipLaunchKernelGGL((k_knn<K, false, true>), dim3(grid), dim3(KBLK), 0, s, P);
+    } else if (P.queue) {
         // persistent: as many workgroups as fit on the GPU at once, each wave looping over the queue
         int per_cu = 0;
         HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_knn<K, true>, KBLK, 0));
@@ -429,16 +500,16 @@ void launch_knn(const KnnParams& P, hipStream_t s) {
     LAUNCH_CHECK();
 }
 
-void launch_any(const KnnParams& P, int K, hipStream_t s) {
+void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false) {
     switch (K) {
-        case 8: launch_knn<8>(P, s); break;
-        case 16: launch_knn<16>(P, s); break;
-        case 24: launch_knn<24>(P, s); break;
-        case 32: launch_knn<32>(P, s); break;
-        case 40: launch_knn<40>(P, s); break;
-        case 48: launch_knn<48>(P, s); break;
-        case 56: launch_knn<56>(P, s); break;
-        default: launch_knn<64>(P, s); break;
+        case 8: launch_knn<8>(P, s, query); break;
+        case 16: launch_knn<16>(P, s, query); break;
+        case 24: launch_knn<24>(P, s, query); break;
+        case 32: launch_knn<32>(P, s, query); break;
+        case 40: launch_knn<40>(P, s, query); break;
+        case 48: launch_knn<48>(P, s, query); break;
+        case 56: launch_knn<56>(P, s, query); break;
+        default: launch_knn<64>(P, s, query); break;
     }
 }
 
@@ -568,6 +639,81 @@ void run_knn(egt_ctx* h, int k, float thres_n) {
     h->clouds[0].thres_n = thres_n;
     run_knn_clouds(h);
     h->k = k;
+}
+
+// Query-vs-reference search (GRIDCUDAKNNCompact): queries dqx/dqy/dqz (device, original order, input units)
+// against the handle's reference points with neighbour count k and squared normalized radius thres_n. Rows
+// land in h->qknn in original query order with reference sorted indices; the self rows are untouched.
+void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* dqz, int64_t nq, int k,
+                   float thres_n) {
+    hipStream_t s = h->stream;
+    h->nq = nq;
+    h->kq = k;
+    h->qknn.alloc((size_t)std::max<int64_t>(nq * k, 1));
+    h->qpts.alloc((size_t)std::max<int64_t>(nq, 1));
+    h->qcell.alloc((size_t)std::max<int64_t>(nq, 1));
+    KnnParams P = knn_params(h);
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, s));
+    CloudDev qcd{};
+    qcd.base = 0;
+    qcd.n = nq;
+    qcd.row_off = 0;
+    qcd.k = k;
+    qcd.thres_n = thres_n;
+    if (nq > 0) {
+        float4* qn = h->scratch.get<float4>(32, nq);
+        uint32_t* cellp = h->scratch.get<uint32_t>(33, nq);
+        uint32_t* code = h->scratch.get<uint32_t>(34, nq);
+        uint32_t* idx = h->scratch.get<uint32_t>(35, nq);
+        const int g = ceil_div(nq, KBLK);
+        k_query_prep<<<g, KBLK, 0, s>>>(dqx, dqy, dqz, nq, h->min_extent, h->max_extent, P.a, P.ba, h->B, qn, cellp,
+                                        code, idx);
+        LAUNCH_CHECK();
+        radix_sort_pairs(code, idx, nq, 30, s, h->scratch);
+        k_query_gather<<<g, KBLK, 0, s>>>(qn, cellp, idx, nq, h->qpts.p, h->qcell.p);
+        LAUNCH_CHECK();
+        h->dqcloud.alloc(1);
+        HIP_CHECK(hipMemcpyAsync(h->dqcloud.p, &qcd, sizeof(CloudDev), hipMemcpyHostToDevice, s));
+        P.qpts = h->qpts.p;
+        P.qcell = h->qcell.p;
+        P.knn = h->qknn.p;
+        P.kth = nullptr;
+        P.clouds = h->dqcloud.p;
+        P.mcloud = nullptr;
+        P.qlevel = nullptr;
+        P.chunks = nullptr;
+        P.queue = nullptr;
+        P.stride = 1;
+        P.nchunks = (nq + 63) / 64;
+        launch_any(P, kclass(k), s, true);
+    }
+    HIP_CHECK(hipEventRecord(e1, s));
+    uint64_t hc[5];
+    HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));  // also keeps qcd alive until its copy has run
+    HIP_CHECK(hipEventElapsedTime(&h->stats.query_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->stats.query_count = nq;
+    h->stats.query_k = k;
+    h->stats.query_candidates = (int64_t)hc[0];
+    h->stats.query_invalid_rows = (int64_t)hc[1];
+    h->have_query = true;
+}
+
+void run_knn_query_original(egt_ctx* h, int32_t* out) {
+    const int64_t cnt = h->nq * h->kq;
+    if (cnt <= 0) return;
+    hipStream_t s = h->stream;
+    int32_t* tmp = h->scratch.get<int32_t>(9, cnt);
+    k_query_original<<<ceil_div(cnt, KBLK), KBLK, 0, s>>>(h->qknn.p, h->perm.p, cnt, tmp);
+    LAUNCH_CHECK();
+    HIP_CHECK(hipMemcpyAsync(out, tmp, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // Autotuner for the search's start level (the cell size of the first search box). For each candidate level

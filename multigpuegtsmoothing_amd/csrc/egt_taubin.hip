@@ -361,10 +361,10 @@ TilePlan prepare_tiles(egt_ctx* h) {
     tp.col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
     tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * TILE);
     tp.hcount = h->scratch.get<int32_t>(21, tp.ntiles);
-    tp.perm = h->scratch.get<uint16_t>(0, (size_t)tp.ntiles * TILE);
-    tp.width = h->scratch.get<uint32_t>(1, (size_t)tp.ntiles * (TILE / 64));
-    uint32_t* padded = h->scratch.get<uint32_t>(2, tp.ntiles);
-    tp.toff = h->scratch.get<uint64_t>(3, tp.ntiles);
+    tp.perm = h->scratch.get<uint16_t>(28, (size_t)tp.ntiles * TILE);
+    tp.width = h->scratch.get<uint32_t>(29, (size_t)tp.ntiles * (TILE / 64));
+    uint32_t* padded = h->scratch.get<uint32_t>(30, tp.ntiles);
+    tp.toff = h->scratch.get<uint64_t>(31, tp.ntiles);
     k_tile_prep<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, h->col.p, n, tp.col16, tp.halo, tp.hcount, tp.perm,
                                                  tp.width, padded);
     LAUNCH_CHECK();

@@ -41,7 +41,9 @@ class Stats(ctypes.Structure):
                 ("knn_ms", ctypes.c_float), ("egg_ms", ctypes.c_float), ("taubin_ms", ctypes.c_float),
                 ("normals_ms", ctypes.c_float), ("grid_ms", ctypes.c_float), ("propose_ms", ctypes.c_float),
                 ("knn_cells", ctypes.c_int64), ("taubin_tile_ms", ctypes.c_float * 2),
-                ("taubin_tile", ctypes.c_int32), ("taubin_fallback_tiles", ctypes.c_int32)]
+                ("taubin_tile", ctypes.c_int32), ("taubin_fallback_tiles", ctypes.c_int32),
+                ("query_count", ctypes.c_int64), ("query_candidates", ctypes.c_int64),
+                ("query_invalid_rows", ctypes.c_int64), ("query_ms", ctypes.c_float), ("query_k", ctypes.c_int32)]
 
     def as_dict(self):
         d = {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -83,6 +85,10 @@ def _load():
         "egt_grid_create": ([_F32P, _F32P, _F32P, i64, i32, f32, i32, i32, _PH], st),
         "egt_grid_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, i64, i32, f32, i32, i32, _PH], st),
         "egt_grid_knn_self": ([_H, i32, f32], st),
+        "egt_grid_knn_query": ([_H, _F32P, _F32P, _F32P, i64, i32, f32], st),
+        "egt_grid_knn_query_device": ([_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, i64, i32, f32], st),
+        "egt_get_knn_query_original": ([_H, _I32P], st),
+        "egt_get_device_query_knn": ([_H, _PF, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32)], st),
         "egt_mesh_create": ([_F32P, _F32P, _F32P, i64, _U32P, i64, i32, f32, i32, _PH], st),
         "egt_mesh_neighbors": ([_H, i32], st),
         "egt_smooth": ([_H, i32, f32, f32, f32], st),
@@ -277,6 +283,48 @@ class GridStructure(_Provider):
 
     def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
         _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))
+        self._last_query = False
+
+    def GRIDCUDAKNNCompact(self, queries, k, thres=1e8):
+        """k nearest reference points of each query (GridKNNCuda.cu:1645-1939). queries: (nq, 3) array or
+        (x, y, z) in input units; a torch CUDA tensor triple is taken as device input. Afterwards
+        GetKNNIndexes() returns the query rows, as in the reference."""
+        if isinstance(queries, (tuple, list)) and len(queries) == 3 and hasattr(queries[0], "data_ptr"):
+            x, y, z = queries
+            _check(_lib.egt_grid_knn_query_device(self.handle, x.data_ptr(), y.data_ptr(), z.data_ptr(), x.numel(),
+                                                  int(k), float(thres)))
+        else:
+            x, y, z = _xyz(queries)
+            _check(_lib.egt_grid_knn_query(self.handle, x, y, z, x.size, int(k), float(thres)))
+        self._last_query = True
+
+    def pointsQuerCount(self):
+        if not getattr(self, "_last_query", False):
+            return self.pointsRefCount()
+        nq = ctypes.c_int64()
+        _check(_lib.egt_get_device_query_knn(self.handle, None, ctypes.byref(nq), None))
+        return nq.value
+
+    def GetKNNIndexesQuery(self):
+        """Rows of the last query search: (nq, k) int32, original query order, original reference indices."""
+        nq, k = ctypes.c_int64(), ctypes.c_int32()
+        _check(_lib.egt_get_device_query_knn(self.handle, None, ctypes.byref(nq), ctypes.byref(k)))
+        out = np.empty(max(nq.value * k.value, 1), np.int32)
+        _check(_lib.egt_get_knn_query_original(self.handle, out))
+        return out[:nq.value * k.value].reshape(nq.value, k.value)
+
+    def GetKNNIndexesQueryDevice(self):
+        """Device address of the query rows (reference sorted indices), with their count and width."""
+        p, nq, k = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_int32()
+        _check(_lib.egt_get_device_query_knn(self.handle, ctypes.byref(p), ctypes.byref(nq), ctypes.byref(k)))
+        return p.value, nq.value, k.value
+
+    def GetKNNIndexes(self):
+        """Neighbour rows in original index space (GridKNNCuda.cu:1941-1968): the query rows after a query
+        search, else the self rows."""
+        if getattr(self, "_last_query", False):
+            return self.GetKNNIndexesQuery()
+        return super().GetKNNIndexes()
 
 
 class GridMeshCuda(_Provider):

@@ -22,6 +22,10 @@
  *   orc_grid_build      -- pinned: exact integer/permutation checks against an independent numpy build
  *   orc_knn_self        -- pinned: brute-force O(N^2) kNN on small clouds, duplicate/lattice/isolated cases,
  *                          big-cell and N<=k invalidation cases built by hand
+ *   orc_knn_query       -- pinned: brute-force query kNN (strict radius, no self exclusion) on small clouds,
+ *                          queries inside, on the edge of and far outside the grid, duplicates of reference
+ *                          points, small radii (short and empty rows), big-cell invalidation; the literal
+ *                          reference schedule agrees row for row
  *   orc_egg             -- pinned: closed forms (collinear drop, unit-square alpha>1 rule, alpha->0 keeps,
  *                          EGG subset of kNN, monotone in alpha) with an FP64 predicate
  *   orc_taubin_f32/f64  -- pinned: lambda=mu=0 identity, symmetric-neighbour fixed point, planar stays planar,
@@ -468,6 +472,235 @@ int orc_knn_self(const float* xs, const float* ys, const float* zs, const int32_
         stats[0] = n_invalid;
         stats[1] = n_literal_diff;
         stats[2] = n_evals;
+    }
+    return 0;
+}
+
+/* f1: query-vs-reference kNN, GridStructure::GRIDCUDAKNNCompact GridKNNCuda.cu:1645-1939 with the kernel
+ * construct_neighbors :340-440. Queries (raw coordinates, original order) are normalized with the
+ * reference cloud's extent (normalize_points :841-849, called at :1700-1703) and searched against the
+ * sorted reference points with the self schedule (own cell at level 0, then ring 1 at bucket counts
+ * B, B/2, ..., 8). It differs from the self search in four places:
+ *   - candidates need d < thres (strict, :372; the self search has <=);
+ *   - no point is excluded (the query is not a reference point);
+ *   - the query's cell is computed at each level from its coordinates (:351-353); a query may lie outside
+ *     the grid, and (int) truncates toward zero, so below the grid the cell is not a shift of the finer one;
+ *   - the certificate's short-list branch (counter < k) compares with >= (:389-405; the self search has >).
+ * The query order used inside the reference (rearrangeQueryPoints :1208-1267) only permutes rows and is
+ * undone by GetKNNIndexes (:1941-1968), so rows here are in original query order. Entries are reference
+ * sorted indices, ordered by (distance, sorted index). As in orc_knn_self the exact list decides validity
+ * through the reference certificate. For a query inside the grid the ring stages cover the full 3x3x3 box of
+ * each level (a level's own cell lies in the previous level's box), so the scanned list is exact. Outside
+ * the grid that fails: the cell is truncated toward zero, so below the grid the own cell of the next level
+ * can reach past the previous box, and the rings skip it. The reference then can stop with a list that
+ * misses nearer points. Here such a query gets its exact list by brute force. stats[1] counts the rows
+ * where the literal reference schedule differs. stats otherwise as in orc_knn_self. */
+namespace {
+bool ref_continue_query(const GridGeom& g, int s, float x, float y, float z, const int* cells, int c, int k,
+                        float kth, float thres_n) {
+    const int lvl = s == 0 ? 0 : s - 1;
+    const int off = s == 0 ? 0 : 1;
+    const int Bl = g.B >> lvl;
+    const float h = g.ba / (float)Bl;
+    const float coords[3] = {x, y, z};
+    const float r = c < k ? std::sqrt(thres_n) : std::sqrt(kth);
+    for (int ax = 0; ax < 3; ++ax) {
+        const float v = coords[ax];
+        const int ci = cells[ax];
+        if (ci - off >= 0 && r >= std::fmaf(-(float)(ci - off), h, v - g.a)) return true;
+        if (ci + off + 1 <= Bl && r >= std::fmaf((float)(ci + off + 1), h, g.a - v)) return true;
+    }
+    return false;
+}
+}  // namespace
+
+int orc_knn_query(const float* xs, const float* ys, const float* zs, const int32_t* key, int64_t n,
+                  const float* qx_raw, const float* qy_raw, const float* qz_raw, int64_t nq, float min_extent,
+                  float max_extent, float eps, int B, int k, float thres_n, int maxc, int32_t* knn, float* kth_out,
+                  int32_t* invalid, int64_t* stats) {
+    if (n < 1 || nq < 0 || k < 1 || B < 1) return -1;
+    GridGeom g;
+    g.a = -eps;
+    g.b = 1.0f + eps;
+    g.ba = g.b - g.a;
+    g.B = B;
+    int nlev = 0;
+    for (int b = B; b >= 1; b >>= 1) ++nlev;
+    std::vector<Level> lev(nlev);
+    for (int l = 0; l < nlev; ++l) lev[l].build(xs, ys, zs, n, g.a, g.ba, B >> l);
+    const int S = num_ring_stages(B);
+    const float dext = max_extent - min_extent;
+    int64_t n_invalid = 0, n_literal_diff = 0, n_evals = 0;
+    const double margin = 2e-6;
+
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : n_invalid, n_literal_diff, n_evals)
+    for (int64_t q = 0; q < nq; ++q) {
+        const float x = (qx_raw[q] - min_extent) / dext;
+        const float y = (qy_raw[q] - min_extent) / dext;
+        const float z = (qz_raw[q] - min_extent) / dext;
+        TopK L(k);
+        bool lit_done = false, lit_invalid = false, exact = false;
+        std::vector<Entry> lit_list;
+        for (int s = 0;; ++s) {
+            const int lvl = s == 0 ? 0 : s - 1;
+            if (lvl >= nlev) break;
+            const Level& Lv = lev[lvl];
+            const int off = s == 0 ? 0 : 1;
+            const int Bl = B >> lvl;
+            const int cx = cell_of(x, g.a, g.ba, Bl), cy = cell_of(y, g.a, g.ba, Bl), cz = cell_of(z, g.a, g.ba, Bl);
+            bool big = false;
+            for (int dz = -off; dz <= off; ++dz)
+                for (int dy = -off; dy <= off; ++dy)
+                    for (int dx = -off; dx <= off; ++dx) {
+                        if (s > 0 && dx == 0 && dy == 0 && dz == 0) continue;
+                        int32_t st, en;
+                        const int32_t cnt = Lv.lookup(cx + dx, cy + dy, cz + dz, st, en);
+                        if (cnt <= 0) continue;
+                        if (cnt >= maxc) big = true;
+                        for (int32_t p = st; p < en; ++p) {
+                            const int32_t j = Lv.order[p];
+                            const float d = sqf(x, y, z, xs[j], ys[j], zs[j]);
+                            ++n_evals;
+                            if (d < thres_n) L.offer(d, j);
+                        }
+                    }
+            const int cells[3] = {cx, cy, cz};
+            if (!lit_done && s <= S) {
+                if (big) {
+                    lit_done = true;
+                    lit_invalid = true;
+                } else {
+                    const int c = (int)L.e.size();
+                    const float kth = c > 0 ? L.e.back().d : 0.0f;
+                    if (!ref_continue_query(g, s, x, y, z, cells, c, k, kth, thres_n)) {
+                        lit_done = true;
+                        lit_list = L.e;
+                    } else if (s == S) {
+                        lit_done = true;
+                        lit_invalid = true;
+                    }
+                }
+            }
+            // exactness of the list over the scanned boxes (double, with margin)
+            const double hl = (double)g.ba / (double)Bl;
+            const double bound = (int)L.e.size() == k ? (double)L.e.back().d : (double)thres_n;
+            const double R = std::sqrt(bound) + margin;
+            exact = true;
+            const double qc[3] = {x, y, z};
+            for (int ax = 0; ax < 3 && exact; ++ax) {
+                if (cells[ax] - off >= 1) {
+                    const double face = (double)g.a + (double)(cells[ax] - off) * hl;
+                    if (!(R < qc[ax] - face)) exact = false;
+                }
+                if (cells[ax] + off <= Bl - 2) {
+                    const double face = (double)g.a + (double)(cells[ax] + off + 1) * hl;
+                    if (!(R < face - qc[ax])) exact = false;
+                }
+                // a box that misses the grid on this axis certifies nothing
+                if (cells[ax] + off < 0 || cells[ax] - off > Bl - 1) exact = false;
+            }
+            if (exact && lit_done) break;
+        }
+        const int fq[3] = {cell_of(x, g.a, g.ba, B), cell_of(y, g.a, g.ba, B), cell_of(z, g.a, g.ba, B)};
+        const bool outside = fq[0] < 0 || fq[1] < 0 || fq[2] < 0 || fq[0] >= B || fq[1] >= B || fq[2] >= B;
+        if (outside || !exact) {
+            // Outside the grid the ring stages do not cover the boxes: below the grid a level's own cell
+            // (truncated toward zero) can hold points that no earlier box held, and the rings skip it. The
+            // exact list then comes from brute force.
+            TopK A(k);
+            for (int64_t j = 0; j < n; ++j) {
+                const float d = sqf(x, y, z, xs[j], ys[j], zs[j]);
+                if (d < thres_n) A.offer(d, (int32_t)j);
+            }
+            L = A;
+        }
+        // reference validity from the exact list
+        const int c = (int)L.e.size();
+        const float kth = c > 0 ? L.e.back().d : 0.0f;
+        int lox = 0, hix = 0, loy = 0, hiy = 0, loz = 0, hiz = 0;
+        if (c > 0) {
+            lox = loy = loz = B;
+            hix = hiy = hiz = -1;
+            for (const Entry& e : L.e) {
+                const int32_t kk = key[e.j];
+                const int mx_ = kk % B, my_ = (kk / B) % B, mz_ = kk / (B * B);
+                lox = std::min(lox, mx_); hix = std::max(hix, mx_);
+                loy = std::min(loy, my_); hiy = std::max(hiy, my_);
+                loz = std::min(loz, mz_); hiz = std::max(hiz, mz_);
+            }
+        }
+        bool valid = false, decided = false;
+        for (int s = 0; s <= S && !decided; ++s) {
+            const int lvl = s == 0 ? 0 : s - 1;
+            const int off = s == 0 ? 0 : 1;
+            const int Bl = B >> lvl;
+            const Level& Lv = lev[lvl];
+            const int cx = cell_of(x, g.a, g.ba, Bl), cy = cell_of(y, g.a, g.ba, Bl), cz = cell_of(z, g.a, g.ba, Bl);
+            for (int dz = -off; dz <= off && !decided; ++dz)
+                for (int dy = -off; dy <= off && !decided; ++dy)
+                    for (int dx = -off; dx <= off; ++dx) {
+                        if (s > 0 && dx == 0 && dy == 0 && dz == 0) continue;
+                        int32_t st, en;
+                        if (Lv.lookup(cx + dx, cy + dy, cz + dz, st, en) >= maxc) {
+                            decided = true;
+                            valid = false;
+                            break;
+                        }
+                    }
+            if (decided) break;
+            bool inside = true;
+            if (c > 0) {
+                inside = (lox >> lvl) >= cx - off && (hix >> lvl) <= cx + off && (loy >> lvl) >= cy - off &&
+                         (hiy >> lvl) <= cy + off && (loz >> lvl) >= cz - off && (hiz >> lvl) <= cz + off;
+            }
+            const int cells[3] = {cx, cy, cz};
+            if (inside && !ref_continue_query(g, s, x, y, z, cells, c, k, kth, thres_n)) {
+                decided = true;
+                valid = true;
+            }
+        }
+        int32_t* row = knn + (size_t)q * k;
+        if (valid) {
+            for (int i = 0; i < k; ++i) row[i] = i < c ? L.e[i].j : -1;
+            kth_out[q] = c > 0 ? kth : -1.0f;
+            invalid[q] = 0;
+        } else {
+            for (int i = 0; i < k; ++i) row[i] = -1;
+            kth_out[q] = -1.0f;
+            invalid[q] = 1;
+            ++n_invalid;
+        }
+        bool same = (lit_invalid == !valid);
+        if (same && valid) {
+            same = lit_list.size() == L.e.size();
+            for (size_t i = 0; same && i < L.e.size(); ++i) same = lit_list[i].j == L.e[i].j;
+        }
+        if (!same) ++n_literal_diff;
+    }
+    if (stats) {
+        stats[0] = n_invalid;
+        stats[1] = n_literal_diff;
+        stats[2] = n_evals;
+    }
+    return 0;
+}
+
+/* Brute-force query kNN (tests only): every reference point with d < thres_n (normalized query coordinates
+ * as orc_knn_query computes them), ordered by (d, sorted index). */
+int orc_knn_query_brute(const float* xs, const float* ys, const float* zs, int64_t n, const float* qx_raw,
+                        const float* qy_raw, const float* qz_raw, int64_t nq, float min_extent, float max_extent,
+                        int k, float thres_n, int32_t* knn) {
+    const float dext = max_extent - min_extent;
+    parallel_for(int64_t q = 0; q < nq; ++q) {
+        const float x = (qx_raw[q] - min_extent) / dext;
+        const float y = (qy_raw[q] - min_extent) / dext;
+        const float z = (qz_raw[q] - min_extent) / dext;
+        TopK L(k);
+        for (int64_t j = 0; j < n; ++j) {
+            const float d = sqf(x, y, z, xs[j], ys[j], zs[j]);
+            if (d < thres_n) L.offer(d, (int32_t)j);
+        }
+        for (int i = 0; i < k; ++i) knn[(size_t)q * k + i] = i < (int)L.e.size() ? L.e[i].j : -1;
     }
     return 0;
 }

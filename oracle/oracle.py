@@ -29,6 +29,9 @@ def lib():
         L.orc_grid_build.argtypes = [F32P, F32P, F32P, i64, f32, i32, F32P, F32P, F32P, I32P, I32P, F32P]
         L.orc_knn_self.argtypes = [F32P, F32P, F32P, I32P, i64, f32, i32, i32, f32, i32, I32P, F32P, I32P, I64P]
         L.orc_knn_brute.argtypes = [F32P, F32P, F32P, i64, i32, f32, I32P]
+        L.orc_knn_query.argtypes = [F32P, F32P, F32P, I32P, i64, F32P, F32P, F32P, i64, f32, f32, f32, i32, i32,
+                                    f32, i32, I32P, F32P, I32P, I64P]
+        L.orc_knn_query_brute.argtypes = [F32P, F32P, F32P, i64, F32P, F32P, F32P, i64, f32, f32, i32, f32, I32P]
         L.orc_egg.argtypes = [F32P, F32P, F32P, i64, i32, f32, I32P]
         L.orc_taubin_f32.argtypes = [F32P, F32P, F32P, I32P, i64, i32, i32, f32, f32]
         L.orc_taubin_f64.argtypes = [F64P, F64P, F64P, I32P, i64, i32, i32, ctypes.c_double, ctypes.c_double]
@@ -90,6 +93,34 @@ class Grid:
         knn = np.empty(self.n * k, np.int32)
         lib().orc_knn_brute(self.xs, self.ys, self.zs, self.n, k, self.thres_n(thres), knn)
         return knn.reshape(self.n, k)
+
+    def knn_query(self, qx, qy, qz, k, thres=1e8, maxc=100000):
+        """Query-vs-reference rows (GRIDCUDAKNNCompact) in original query order, reference sorted indices."""
+        qx, qy, qz = _f32(qx), _f32(qy), _f32(qz)
+        nq = qx.size
+        knn = np.empty(max(nq, 1) * k, np.int32)
+        kth = np.empty(max(nq, 1), np.float32)
+        inv = np.empty(max(nq, 1), np.int32)
+        stats = np.zeros(8, np.int64)
+        rc = lib().orc_knn_query(self.xs, self.ys, self.zs, self.key, self.n, qx, qy, qz, nq,
+                                 np.float32(self.min_extent), np.float32(self.max_extent), np.float32(self.eps),
+                                 self.B, k, self.thres_n(thres), maxc, knn, kth, inv, stats)
+        if rc != 0:
+            raise ValueError("orc_knn_query failed")
+        return knn[:nq * k].reshape(nq, k), kth[:nq], inv[:nq], stats
+
+    def knn_query_brute(self, qx, qy, qz, k, thres=1e8):
+        qx, qy, qz = _f32(qx), _f32(qy), _f32(qz)
+        nq = qx.size
+        knn = np.empty(max(nq, 1) * k, np.int32)
+        lib().orc_knn_query_brute(self.xs, self.ys, self.zs, self.n, qx, qy, qz, nq, np.float32(self.min_extent),
+                                  np.float32(self.max_extent), k, self.thres_n(thres), knn)
+        return knn[:nq * k].reshape(nq, k)
+
+    def ref_to_original(self, rows):
+        """Maps reference sorted indices in query rows to original reference indices (-1 kept)."""
+        rows = np.asarray(rows, np.int32)
+        return np.where(rows >= 0, self.perm[np.maximum(rows, 0)], -1).astype(np.int32)
 
     def to_original(self, knn):
         n, k = knn.shape

@@ -1,0 +1,131 @@
+"""GPU parity of the query-vs-reference search (GRIDCUDAKNNCompact, GridKNNCuda.cu:1645-1939) through the C ABI
+against the oracle, bit for bit: rows in original query order with original reference indices."""
+import numpy as np
+import pytest
+
+from multigpuegtsmoothing_amd import datasets as D
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def _queries_near(pts, n, seed, scale=1.0, noise=0.002):
+    rng = np.random.default_rng(seed)
+    x, y, z = pts
+    idx = rng.integers(0, x.size, n)
+    q = np.stack([x[idx], y[idx], z[idx]], 1).astype(np.float64) * scale
+    q += rng.normal(0.0, noise, q.shape)
+    return tuple(np.ascontiguousarray(q[:, i], np.float32) for i in range(3))
+
+
+def _check(E, O, ref, q, k, thres=1e8, maxc=None):
+    g = E.GridStructure(ref, 1, 1e-6, 1024, 4)
+    if maxc is not None:
+        g.set_option(E.OPT_MAX_PTS_IN_CUBE, maxc)
+    g.GRIDCUDAKNNCompact(q, k, thres)
+    got = g.GetKNNIndexes()
+    st = g.stats()
+    g.close()
+    og = O.Grid(*ref)
+    rows, _, inv, ost = og.knn_query(*q, k, thres=thres, maxc=maxc if maxc is not None else 100000)
+    want = og.ref_to_original(rows)
+    np.testing.assert_array_equal(got, want)
+    assert st["query_invalid_rows"] == int(inv.sum())
+    assert st["query_count"] == q[0].size and st["query_k"] == k
+    return got, st, inv, ost
+
+
+@pytest.mark.parametrize("case", ["sphere_k16", "torus_k40", "sphere_k64", "mixture_k8"])
+def test_query_rows_bit_exact(egt_mod, oracle_mod, case):
+    if case == "sphere_k16":
+        ref, k, seed = D.sphere(20_000, seed=41), 16, 51
+    elif case == "torus_k40":
+        ref, k, seed = D.torus(30_000, seed=42), 40, 52
+    elif case == "sphere_k64":
+        ref, k, seed = D.sphere(8_000, seed=43), 64, 53
+    else:
+        ref, k, seed = D.mixture(40_000, seed=44), 8, 54
+    q = _queries_near(ref, 12_000, seed)
+    _log("query case", case)
+    got, st, inv, ost = _check(egt_mod, oracle_mod, ref, q, k)
+    _log("  invalid", int(inv.sum()), "candidates/query", st["query_candidates"] / q[0].size, "ms", st["query_ms"])
+    assert (inv == 0).mean() > 0.5
+
+
+def test_queries_outside_the_grid(egt_mod, oracle_mod):
+    ref = D.sphere(15_000, seed=45)
+    near = _queries_near(ref, 4_000, 55, scale=1.08)
+    far = _queries_near(ref, 500, 56, scale=4.0)
+    q = tuple(np.concatenate([a, b]) for a, b in zip(near, far))
+    _log("outside, thres 1e8")
+    got, st, inv, ost = _check(egt_mod, oracle_mod, ref, q, 16)
+    _log("  literal reference schedule differs on", int(ost[1]), "rows")
+    assert not (inv[4000:] == 0).any()
+    _log("outside, thres 0.05")
+    got, st, inv, ost = _check(egt_mod, oracle_mod, ref, q, 16, thres=0.05)
+    assert (got[4000:] == -1).all() and (inv[4000:] == 0).all()
+
+
+def test_short_rows_duplicates_and_big_cells(egt_mod, oracle_mod):
+    ref = D.sphere(12_000, seed=46)
+    dup = tuple(np.ascontiguousarray(a[:2000]) for a in ref)  # queries on reference points
+    _log("duplicates")
+    got, _, inv, _ = _check(egt_mod, oracle_mod, ref, dup, 8)
+    v = inv == 0
+    np.testing.assert_array_equal(got[v, 0], np.arange(2000)[v])  # each finds itself first, at distance 0
+    _log("short rows")
+    got, _, inv, _ = _check(egt_mod, oracle_mod, ref, dup, 8, thres=0.01)
+    assert ((got[inv == 0] == -1).sum(1) > 0).mean() > 0.5
+    _log("big cells")
+    _, _, inv, _ = _check(egt_mod, oracle_mod, ref, _queries_near(ref, 3000, 57), 16, maxc=20)
+    assert 0 < (inv == 0).mean() < 1
+
+
+def test_device_input_and_self_rows_untouched(egt_mod, oracle_mod):
+    import torch
+
+    E, O = egt_mod, oracle_mod
+    ref = D.torus(20_000, seed=47)
+    q = _queries_near(ref, 5_000, 58)
+    g = E.GridStructure(ref, 1, 1e-6, 1024, 4)
+    g.GRIDCUDAKNNSELF_COMPACT(16, 1e8)
+    self_rows = g.GetKNNIndexes()
+    g.GRIDCUDAKNNCompact(q, 16)
+    host_rows = g.GetKNNIndexes()
+    dq = tuple(torch.from_numpy(a).cuda() for a in q)
+    g.GRIDCUDAKNNCompact(dq, 16)
+    dev_rows = g.GetKNNIndexes()
+    np.testing.assert_array_equal(host_rows, dev_rows)
+    assert g.pointsQuerCount() == q[0].size
+    # the self rows (used by smoothing) are kept beside the query rows
+    g.GRIDCUDAKNNSELF_COMPACT(16, 1e8)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), self_rows)
+    g.GRIDCUDAKNNCompact(q, 16)
+    E.EGT_CUDA(g).PerformSmoothing(5, 0.5, -0.53, 0.65)
+    out = np.stack(E.getOutputPoints(g), 1)
+    g2 = E.GridStructure(ref, 1, 1e-6, 1024, 4)
+    g2.GRIDCUDAKNNSELF_COMPACT(16, 1e8)
+    E.EGT_CUDA(g2).PerformSmoothing(5, 0.5, -0.53, 0.65)
+    np.testing.assert_array_equal(out, np.stack(E.getOutputPoints(g2), 1))
+    g.close()
+    g2.close()
+
+
+def test_query_argument_errors(egt_mod):
+    E = egt_mod
+    ref = D.sphere(2_000, seed=48)
+    g = E.GridStructure(ref, 1, 1e-6, 1024, 4)
+    e = np.zeros(0, np.float32)
+    g.GRIDCUDAKNNCompact((e, e, e), 8)  # empty query set
+    assert g.GetKNNIndexes().shape == (0, 8)
+    bad = (np.array([np.nan], np.float32), np.zeros(1, np.float32), np.zeros(1, np.float32))
+    with pytest.raises(E.EGTError) as ei:
+        g.GRIDCUDAKNNCompact(bad, 8)
+    assert ei.value.status == 1
+    with pytest.raises(E.EGTError) as ei:
+        g.GRIDCUDAKNNCompact(_queries_near(ref, 10, 1), 65)
+    assert ei.value.status == 4
+    g.close()
