@@ -37,6 +37,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample", type=int, default=250_000)
     p.add_argument("--no-learned", action="store_true", help="skip the learned-proposal vs plain comparison")
+    p.add_argument("--no-query", action="store_true", help="skip the query-vs-reference search section")
     p.add_argument("--no-graph", action="store_true", help="launch the Taubin passes one by one (profiling)")
     p.add_argument("--workload", choices=["torus", "batch"], default="torus",
                    help="torus: one 4M-point cloud (default); batch: 512 uneven superquadric clouds")
@@ -123,6 +124,47 @@ def cpu_baseline(args):
     return {"value": n / dt, "unit": "points/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"oracle (host C++/OpenMP restatement) on a {n}-point torus from the same generator, "
                       f"same k/alpha/{args.iters} iterations/normals; {dt:.2f} s"}
+
+
+def query_search(E, dev_xyz, args):
+    """Query-vs-reference search (GRIDCUDAKNNCompact) against the bench's reference cloud: args.n queries drawn
+    from the same torus generator with another seed (a second scan of the same surface, as in ICP), device
+    input, k = 1 and k = 16. Rates use the library's device time of the search (query normalization and
+    sort included). The CPU figure is the oracle's query search on a 200k-query sample against the same
+    reference cloud (its grid build is not timed)."""
+    import torch
+
+    from multigpuegtsmoothing_amd import datasets as D
+
+    qpts = D.torus(args.n, seed=3)
+    dq = tuple(torch.from_numpy(a).cuda() for a in qpts)
+    g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+    res = {"queries": f"torus {args.n} pts, seed 3 (same generator as the reference cloud)"}
+    for k in (1, 16):
+        times = []
+        for _ in range(3):
+            g.GRIDCUDAKNNCompact(dq, k, 1e8)
+            times.append(g.stats()["query_ms"])
+        st = g.stats()
+        ms = float(np.median(times))
+        res[f"k{k}"] = {"queries_per_s": args.n / (ms * 1e-3), "ms": ms,
+                        "candidates_per_query": st["query_candidates"] / args.n,
+                        "invalid_rows": st["query_invalid_rows"],
+                        "achieved_GBs": args.n * (12 + 4 * k) / (ms * 1e-3) / 1e9}
+    g.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        og = O.Grid(*D.torus(args.n, seed=2))
+        m = 200_000
+        sample = tuple(a[:m] for a in qpts)
+        t0 = time.perf_counter()
+        og.knn_query(*sample, 16)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline_k16"] = {"value": m / dt, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
+                                   "sample": f"oracle query search, first {m} queries against the same reference "
+                                             f"cloud; {dt:.2f} s"}
+    return res
 
 
 def batch_main(args):
@@ -272,6 +314,8 @@ def main():
     }
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain(E, dev, args)
+    if not args.no_query:
+        out["query_knn"] = query_search(E, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out), flush=True)
