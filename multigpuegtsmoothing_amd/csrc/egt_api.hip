@@ -324,7 +324,9 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
                 return fail(EGT_ERR_INVALID_ARG, "Taubin tile width must be 1024, 2048 or 0 (autotune)");
             h->taubin_tile = (int)value;
             return EGT_OK;
-
+        case EGT_OPT_TAUBIN_REGULARIZED:
+            h->taubin_regularized = value ? 1 : 0;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }

@@ -206,8 +206,17 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
 // w * (q - p) = (-w) * (p - q): q - p is exactly -(p - q), and an accumulator that starts at +0 never
 // becomes -0, so the sign of a zero difference cannot change a sum; the reuse saves three subtractions
 // per neighbour.
+// Neighbour weight: exp(-d^2), or with isRegularized (EGT_CUDA.cu:263, dormant in the reference) 1 in the
+// lambda pass and 1 / (d^2 + 1e-8) in the mu pass.
+template <bool REG>
+__device__ inline float taubin_weight(float dist, float scale) {
+    if (REG) return scale < 0.0f ? 1.0f / (dist + 1e-8f) : 1.0f;
+    return egt_expf_taubin(-dist);
+}
+
+template <bool REG>
 __device__ inline void taubin_sell_row(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ sc,
-                                       int W, float& cgx, float& cgy, float& cgz, float& sum) {
+                                       int W, float scale, float& cgx, float& cgy, float& cgz, float& sum) {
     constexpr int CH = 8;
     for (int u0 = 0; u0 < W; u0 += CH) {
         uint32_t c[CH];
@@ -224,7 +233,7 @@ __device__ inline void taubin_sell_row(const float4 p, const float4* __restrict_
                 const float4 q = qs[u];
                 const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                 const float dist = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const float w = egt_expf_taubin(-dist);
+                const float w = taubin_weight<REG>(dist, scale);
                 cgx = __builtin_fmaf(-w, dx, cgx);
                 cgy = __builtin_fmaf(-w, dy, cgy);
                 cgz = __builtin_fmaf(-w, dz, cgz);
@@ -242,7 +251,7 @@ __device__ inline float4 taubin_result(const float4 p, float cgx, float cgy, flo
 // One lambda or mu pass over one tile. Wave w of the block takes slices w and S - 1 - w (S = TILE / 64
 // slices in ascending width), so the waves of a block carry similar work; lane l of slice s handles the
 // row perm[64 s + l]. Results go back through LDS so the global writes are coalesced.
-template <int TILE>
+template <int TILE, bool REG>
 __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restrict__ in, float4* __restrict__ out,
                                                        const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ col,
@@ -271,7 +280,7 @@ __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restr
             for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
                 const float4 q = in[col[e]];
                 const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
-                const float w = egt_expf_taubin(-dist);
+                const float w = taubin_weight<REG>(dist, scale);
                 cgx = __builtin_fmaf(w, q.x - p.x, cgx);
                 cgy = __builtin_fmaf(w, q.y - p.y, cgy);
                 cgz = __builtin_fmaf(w, q.z - p.z, cgz);
@@ -308,7 +317,7 @@ __global__ void __launch_bounds__(TBLK_TILE) k_taubin_tile(const float4* __restr
         // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
         const uint32_t sb0 = __builtin_amdgcn_readfirstlane(sbase[s]);
         const int W = (int)((__builtin_amdgcn_readfirstlane(sbase[s + 1]) - sb0) >> 6);
-        taubin_sell_row(p, lds, tc + sb0 + lane, W, cgx, cgy, cgz, sum);
+        taubin_sell_row<REG>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum);
         res[r] = taubin_result(p, cgx, cgy, cgz, sum, scale);
     }
     __syncthreads();
@@ -391,17 +400,21 @@ TilePlan prepare_tiles(egt_ctx* h) {
     return tp;
 }
 
-void launch_pass(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
+template <int TILE, bool REG>
+void launch_tile(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
     const size_t lds_bytes = (size_t)(tp.tile + tp.maxH) * sizeof(float4);
     const dim3 tgrid(ceil_div(tp.ntiles, NXCD) * NXCD), tblk(TBLK_TILE);
+    k_taubin_tile<TILE, REG><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
+                                                                    tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
+                                                                    tp.ntiles);
+}
+
+void launch_pass(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
+    const bool reg = h->taubin_regularized != 0;
     if (tp.tile == 2048)
-        k_taubin_tile<2048><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
-                                                                   tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
-                                                                   tp.ntiles);
+        reg ? launch_tile<2048, true>(h, tp, in, out, scale) : launch_tile<2048, false>(h, tp, in, out, scale);
     else
-        k_taubin_tile<1024><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
-                                                                   tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
-                                                                   tp.ntiles);
+        reg ? launch_tile<1024, true>(h, tp, in, out, scale) : launch_tile<1024, false>(h, tp, in, out, scale);
 }
 
 TilePlan prepare(egt_ctx* h, int tile) { return tile == 2048 ? prepare_tiles<2048>(h) : prepare_tiles<1024>(h); }

@@ -25,6 +25,7 @@ LIB_PATH = os.environ.get("EGT_LIB_PATH") or os.path.join(_HERE, "libegt.so")
 EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
 OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT, OPT_TAUBIN_TILE = 0, 1, 2, 3, 4, 5
+OPT_TAUBIN_REGULARIZED = 6
 
 
 class EGTError(RuntimeError):

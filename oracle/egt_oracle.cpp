@@ -29,7 +29,9 @@
  *   orc_egg             -- pinned: closed forms (collinear drop, unit-square alpha>1 rule, alpha->0 keeps,
  *                          EGG subset of kNN, monotone in alpha) with an FP64 predicate
  *   orc_taubin_f32/f64  -- pinned: lambda=mu=0 identity, symmetric-neighbour fixed point, planar stays planar,
- *                          FP32 vs FP64 bound, closed-form ring shrink
+ *                          FP32 vs FP64 bound, closed-form ring shrink; regularized weights (isRegularized,
+ *                          EGT_CUDA.cu:263): ring closed form over both passes, two-neighbour closed form of
+ *                          the inverse-square weights, FP32 vs FP64 bound
  *   egt_expf            -- pinned: <=2 ulp against double exp over [-90, 0]
  *   orc_eig3            -- pinned: numpy.linalg.eigh on random symmetric matrices (angle/eigenvalue bounds)
  *   orc_pca_normals     -- pinned: plane -> plane normal, sphere -> radial
@@ -793,7 +795,7 @@ int orc_egg(const float* xs, const float* ys, const float* zs, int64_t n, int k,
 /* a7: Taubin lambda/mu Jacobi iterations, taubin_step_GKNN EGT_CUDA.cu:229-276 with the loop :545-634
  * (one iteration = lambda pass then mu pass, ping-pong). Weights w = exp(-|pn - pi|^2). */
 int orc_taubin_f32(float* x, float* y, float* z, const int32_t* knn, int64_t n, int k, int iters, float lambda,
-                   float mu) {
+                   float mu, int regularized) {
     std::vector<float> ox(n), oy(n), oz(n);
     float *ix = x, *iy = y, *iz = z, *px = ox.data(), *py = oy.data(), *pz = oz.data();
     for (int it = 0; it < 2 * iters; ++it) {
@@ -807,7 +809,8 @@ int orc_taubin_f32(float* x, float* y, float* z, const int32_t* knn, int64_t n, 
                 if (nb == -1) continue;
                 const float x2 = ix[nb], y2 = iy[nb], z2 = iz[nb];
                 const float dist = sqf(x1, y1, z1, x2, y2, z2);
-                const float w = egt_expf(-dist);
+                // isRegularized (EGT_CUDA.cu:263): w = 1 in the lambda pass, 1 / (d^2 + 1e-8) in the mu pass
+                const float w = regularized ? (scale < 0.0f ? 1.0f / (dist + 1e-8f) : 1.0f) : egt_expf(-dist);
                 cgx = std::fmaf(w, x2 - x1, cgx);
                 cgy = std::fmaf(w, y2 - y1, cgy);
                 cgz = std::fmaf(w, z2 - z1, cgz);
@@ -828,7 +831,7 @@ int orc_taubin_f32(float* x, float* y, float* z, const int32_t* knn, int64_t n, 
 
 /* FP64 twin of orc_taubin_f32 (error reference only). */
 int orc_taubin_f64(double* x, double* y, double* z, const int32_t* knn, int64_t n, int k, int iters, double lambda,
-                   double mu) {
+                   double mu, int regularized) {
     std::vector<double> ox(n), oy(n), oz(n);
     double *ix = x, *iy = y, *iz = z, *px = ox.data(), *py = oy.data(), *pz = oz.data();
     for (int it = 0; it < 2 * iters; ++it) {
@@ -841,7 +844,8 @@ int orc_taubin_f64(double* x, double* y, double* z, const int32_t* knn, int64_t 
                 const int32_t nb = row[s];
                 if (nb == -1) continue;
                 const double dx = x1 - ix[nb], dy = y1 - iy[nb], dz = z1 - iz[nb];
-                const double w = std::exp(-(dx * dx + dy * dy + dz * dz));
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double w = regularized ? (scale < 0.0 ? 1.0 / (d2 + 1e-8) : 1.0) : std::exp(-d2);
                 cgx += w * (ix[nb] - x1);
                 cgy += w * (iy[nb] - y1);
                 cgz += w * (iz[nb] - z1);

@@ -33,8 +33,8 @@ def lib():
                                     f32, i32, I32P, F32P, I32P, I64P]
         L.orc_knn_query_brute.argtypes = [F32P, F32P, F32P, i64, F32P, F32P, F32P, i64, f32, f32, i32, f32, I32P]
         L.orc_egg.argtypes = [F32P, F32P, F32P, i64, i32, f32, I32P]
-        L.orc_taubin_f32.argtypes = [F32P, F32P, F32P, I32P, i64, i32, i32, f32, f32]
-        L.orc_taubin_f64.argtypes = [F64P, F64P, F64P, I32P, i64, i32, i32, ctypes.c_double, ctypes.c_double]
+        L.orc_taubin_f32.argtypes = [F32P, F32P, F32P, I32P, i64, i32, i32, f32, f32, i32]
+        L.orc_taubin_f64.argtypes = [F64P, F64P, F64P, I32P, i64, i32, i32, ctypes.c_double, ctypes.c_double, i32]
         L.orc_output.argtypes = [F32P, F32P, F32P, I32P, i64, f32, f32, F32P, F32P, F32P]
         L.orc_knn_to_original.argtypes = [I32P, I32P, i64, i32, I32P]
         L.orc_mesh_adjacency.argtypes = [F32P, F32P, F32P, i64, U32P, i64, I32P, i32, I32P]
@@ -150,20 +150,21 @@ def egg(xs, ys, zs, knn, alpha):
     return out
 
 
-def taubin_f32(xs, ys, zs, knn, iters, lam, mu):
+def taubin_f32(xs, ys, zs, knn, iters, lam, mu, regularized=False):
     n, k = knn.shape
     x, y, z = _f32(xs).copy(), _f32(ys).copy(), _f32(zs).copy()
     lib().orc_taubin_f32(x, y, z, np.ascontiguousarray(knn, np.int32).ravel(), n, k, iters, np.float32(lam),
-                         np.float32(mu))
+                         np.float32(mu), int(regularized))
     return x, y, z
 
 
-def taubin_f64(xs, ys, zs, knn, iters, lam, mu):
+def taubin_f64(xs, ys, zs, knn, iters, lam, mu, regularized=False):
     n, k = knn.shape
     x = np.ascontiguousarray(xs, np.float64).copy()
     y = np.ascontiguousarray(ys, np.float64).copy()
     z = np.ascontiguousarray(zs, np.float64).copy()
-    lib().orc_taubin_f64(x, y, z, np.ascontiguousarray(knn, np.int32).ravel(), n, k, iters, float(lam), float(mu))
+    lib().orc_taubin_f64(x, y, z, np.ascontiguousarray(knn, np.int32).ravel(), n, k, iters, float(lam), float(mu),
+                         int(regularized))
     return x, y, z
 
 

@@ -113,19 +113,23 @@ def test_egg_and_taubin_bit_exact(egt_mod, oracle_mod):
 
 
 def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
-    """A uniform 3-D cloud at k=64 gives tiles whose halo exceeds the LDS budget: exercises the
-    global-gather path of the tiled Taubin pass next to LDS tiles."""
+    """A uniform 3-D cloud at k=64 with alpha 1e-8 (the EGG keeps nearly every neighbour) gives tiles whose
+    halo exceeds the LDS budget: exercises the global-gather path of the tiled Taubin pass next to LDS
+    tiles. (At alpha 0.65 the largest halo on this cloud is about 1600 points, inside the budget.)"""
     E, O = egt_mod, oracle_mod
     rng = np.random.default_rng(42)
     p = rng.uniform(0, 1, (40_000, 3)).astype(np.float32)
     pts = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy())
-    k, iters = 64, 4
+    k, iters, alpha = 64, 4, 1e-8
     g = _cloud(E, pts, k)
-    E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, 0.65)
+    E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, alpha)
     out = E.getOutputPoints(g)
-    ref = O.pipeline(*pts, k, iters, 0.5, -0.53, 0.65, normals=False)
+    st = g.stats()
+    ref = O.pipeline(*pts, k, iters, 0.5, -0.53, alpha, normals=False)
     for a, b in zip(out, ref["output"]):
         np.testing.assert_array_equal(a, b)
+    _log("fallback tiles", st["taubin_fallback_tiles"])
+    assert st["taubin_fallback_tiles"] > 0
 
 
 def test_taubin_vs_fp64(egt_mod, oracle_mod):
@@ -237,3 +241,32 @@ def test_taubin_tile_width_keeps_results(egt_mod, oracle_mod, tile):
     for a, b in zip(out, ref["output"]):
         np.testing.assert_array_equal(a, b)
     g.close()
+
+
+@pytest.mark.parametrize("case", ["sphere", "volumetric"])
+def test_regularized_taubin_bit_exact(egt_mod, oracle_mod, case):
+    """The reference's dormant isRegularized weights (EGT_CUDA.cu:263): 1 in the lambda pass, 1 / (d^2 + 1e-8)
+    in the mu pass, through LDS tiles and (volumetric, k=64) the global-gather tiles."""
+    E, O = egt_mod, oracle_mod
+    if case == "sphere":
+        pts, k, iters = D.sphere(20_000, seed=44), 40, 10
+    else:
+        rng = np.random.default_rng(45)
+        p = rng.uniform(0, 1, (30_000, 3)).astype(np.float32)
+        pts, k, iters = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()), 64, 3
+    alpha = 0.65 if case == "sphere" else 1e-8  # 1e-8: halos beyond the LDS budget (global-gather tiles)
+    g = _cloud(E, pts, k)
+    g.set_option(E.OPT_TAUBIN_REGULARIZED, 1)
+    E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, alpha)
+    out = E.getOutputPoints(g)
+    st = g.stats()
+    g.close()
+    og = O.Grid(*pts)
+    knn = og.knn(k)[0]
+    egg = O.egg(og.xs, og.ys, og.zs, knn, alpha)
+    sx, sy, sz = O.taubin_f32(og.xs, og.ys, og.zs, egg, iters, 0.5, -0.53, regularized=True)
+    for a, b in zip(out, og.output(sx, sy, sz)):
+        np.testing.assert_array_equal(a, b)
+    _log("regularized", case, "fallback tiles", st["taubin_fallback_tiles"])
+    if case == "volumetric":
+        assert st["taubin_fallback_tiles"] > 0

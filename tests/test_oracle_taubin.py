@@ -81,3 +81,52 @@ def test_output_round_trip(oracle_mod):
     ox, oy, oz = g.output(g.xs, g.ys, g.zs)
     tol = 2 * np.spacing(np.float32(g.max_extent - g.min_extent))
     assert np.abs(ox - pts[0]).max() <= tol and np.abs(oz - pts[2]).max() <= tol
+
+
+# Regularized variant (isRegularized = 1, EGT_CUDA.cu:263; dormant in the reference, which hard-codes 0 at
+# :544): w = 1 in the lambda pass (scale > 0) and w = 1 / (|q - p|^2 + 1e-8) in the mu pass (scale < 0).
+
+@pytest.mark.parametrize("lam,mu", [(0.5, -0.53), (0.63, -0.64)])
+def test_regularized_ring_closed_form(oracle_mod, lam, mu):
+    """On a regular ring both passes see equal weights, so one iteration scales the radius by
+    (1 - lam c)(1 - mu c), c = 1 - cos(2 pi / N)."""
+    N, r = 48, 0.2
+    t = 2 * np.pi * np.arange(N) / N
+    X, Y, Z = 0.5 + r * np.cos(t), 0.5 + r * np.sin(t), np.full(N, 0.5)
+    rows = np.stack([(np.arange(N) - 1) % N, (np.arange(N) + 1) % N], 1).astype(np.int32)
+    c = 1 - np.cos(2 * np.pi / N)
+    want = r * (1 - lam * c) * (1 - mu * c)
+    x, y, _ = oracle_mod.taubin_f64(X, Y, Z, rows, 1, lam, mu, regularized=True)
+    np.testing.assert_allclose(np.hypot(x - 0.5, y - 0.5), want, rtol=1e-12)
+    xf, yf, _ = oracle_mod.taubin_f32(X, Y, Z, rows, 1, lam, mu, regularized=True)
+    np.testing.assert_allclose(np.hypot(xf - 0.5, yf - 0.5), want, rtol=4e-6)
+
+
+def test_regularized_weights_closed_form(oracle_mod):
+    """One point with two neighbours on a line at distances a and b: the lambda pass moves it to
+    p + lam ((q1 - p) + (q2 - p)) / 2 (uniform weights); the mu pass uses w_i = 1 / (d_i^2 + 1e-8)."""
+    a, b, lam, mu = 0.01, 0.03, 0.5, -0.53
+    P = np.array([[0.5, 0.5, 0.5], [0.5 - a, 0.5, 0.5], [0.5 + b, 0.5, 0.5]])
+    rows = np.array([[1, 2], [-1, -1], [-1, -1]], np.int32)
+    # lambda pass only (mu = 0 second half has zero scale)
+    x, _, _ = oracle_mod.taubin_f64(P[:, 0], P[:, 1], P[:, 2], rows, 1, lam, 0.0, regularized=True)
+    np.testing.assert_allclose(x[0], 0.5 + lam * ((-a) + b) / 2, rtol=1e-14)
+    # mu pass only: lambda = 0 first, then mu with inverse-square weights
+    x, _, _ = oracle_mod.taubin_f64(P[:, 0], P[:, 1], P[:, 2], rows, 1, 0.0, mu, regularized=True)
+    w1, w2 = 1 / (a * a + 1e-8), 1 / (b * b + 1e-8)
+    np.testing.assert_allclose(x[0], 0.5 + mu * (w1 * (-a) + w2 * b) / (w1 + w2), rtol=1e-14)
+    xf, _, _ = oracle_mod.taubin_f32(P[:, 0], P[:, 1], P[:, 2], rows, 1, 0.0, mu, regularized=True)
+    assert abs(float(xf[0]) - x[0]) <= 4 * np.spacing(np.float32(0.5))
+
+
+def test_regularized_fp32_within_bound_and_differs(oracle_mod):
+    pts = D.sphere(10_000, seed=2)
+    g = oracle_mod.Grid(*pts)
+    knn, _, _, _ = g.knn(24)
+    egg = oracle_mod.egg(g.xs, g.ys, g.zs, knn, 0.65)
+    fx, fy, fz = oracle_mod.taubin_f32(g.xs, g.ys, g.zs, egg, 20, 0.5, -0.53, regularized=True)
+    dx, dy, dz = oracle_mod.taubin_f64(g.xs, g.ys, g.zs, egg, 20, 0.5, -0.53, regularized=True)
+    err = np.sqrt((fx - dx) ** 2 + (fy - dy) ** 2 + (fz - dz) ** 2)
+    assert err.max() / np.sqrt(3.0) <= 1e-5
+    ex, _, _ = oracle_mod.taubin_f32(g.xs, g.ys, g.zs, egg, 20, 0.5, -0.53)
+    assert np.abs(ex - fx).max() > 1e-6  # the flag changes the weights
