@@ -10,6 +10,7 @@
 // it has colours, then faces as a uchar count 3 followed by three uint32 indices.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -68,6 +69,34 @@ struct Element {
     }
 };
 
+inline bool little_endian_host() {
+    const uint16_t one = 1;
+    unsigned char c;
+    std::memcpy(&c, &one, 1);
+    return c == 1;
+}
+
+// Decodes one binary scalar of type t at src (file byte order: swap when it differs from the host's), as a
+// double.
+inline double decode(const unsigned char* src, Ty t, bool swap) {
+    unsigned char b[8];
+    const int sz = type_size(t);
+    std::memcpy(b, src, sz);
+    if (swap)
+        for (int i = 0; i < sz / 2; ++i) std::swap(b[i], b[sz - 1 - i]);
+    switch (t) {
+        case Ty::I8: return (double)(int8_t)b[0];
+        case Ty::U8: return (double)b[0];
+        case Ty::I16: { int16_t v; std::memcpy(&v, b, 2); return v; }
+        case Ty::U16: { uint16_t v; std::memcpy(&v, b, 2); return v; }
+        case Ty::I32: { int32_t v; std::memcpy(&v, b, 4); return v; }
+        case Ty::U32: { uint32_t v; std::memcpy(&v, b, 4); return v; }
+        case Ty::F32: { float v; std::memcpy(&v, b, 4); return v; }
+        case Ty::F64: { double v; std::memcpy(&v, b, 8); return v; }
+    }
+    return 0.0;
+}
+
 // Reads one scalar of type t from a binary stream or an ascii token stream, as a double.
 class ValueReader {
 public:
@@ -81,30 +110,10 @@ public:
         unsigned char b[8];
         const int sz = type_size(t);
         if (!in_.read(reinterpret_cast<char*>(b), sz)) throw std::runtime_error("PLY: unexpected end of binary data");
-        const bool host_le = little_endian_host();
-        const bool file_le = format_ == 1;
-        if (host_le != file_le)
-            for (int i = 0; i < sz / 2; ++i) std::swap(b[i], b[sz - 1 - i]);
-        switch (t) {
-            case Ty::I8: return (double)(int8_t)b[0];
-            case Ty::U8: return (double)b[0];
-            case Ty::I16: { int16_t v; std::memcpy(&v, b, 2); return v; }
-            case Ty::U16: { uint16_t v; std::memcpy(&v, b, 2); return v; }
-            case Ty::I32: { int32_t v; std::memcpy(&v, b, 4); return v; }
-            case Ty::U32: { uint32_t v; std::memcpy(&v, b, 4); return v; }
-            case Ty::F32: { float v; std::memcpy(&v, b, 4); return v; }
-            case Ty::F64: { double v; std::memcpy(&v, b, 8); return v; }
-        }
-        return 0.0;
+        return decode(b, t, little_endian_host() != (format_ == 1));
     }
 
 private:
-    static bool little_endian_host() {
-        const uint16_t one = 1;
-        unsigned char c;
-        std::memcpy(&c, &one, 1);
-        return c == 1;
-    }
     std::istream& in_;
     int format_;  // 0 ascii, 1 binary LE, 2 binary BE
 };
@@ -171,6 +180,50 @@ inline void read(const std::string& path, Mesh* mesh) {
             if (iface < 0) iface = e.find("vertex_index");
             if (iface >= 0) mesh->mF.resize(e.count);
         }
+        // Binary fast paths: records of fixed size are read in blocks of many records and decoded from memory
+        // (elements without list properties, and a face element whose only property is the index list with a
+        // one-byte count and 4-byte indices, where every face must be a triangle anyway).
+        if (format != 0) {
+            const bool swap = little_endian_host() != (format == 1);
+            bool fixed = true;
+            size_t rec = 0;
+            std::vector<size_t> off(e.props.size());
+            for (size_t pi = 0; pi < e.props.size(); ++pi) {
+                off[pi] = rec;
+                if (e.props[pi].is_list) fixed = false;
+                else rec += type_size(e.props[pi].type);
+            }
+            const bool tri_list = iface >= 0 && e.props.size() == 1 && type_size(e.props[0].count_type) == 1 &&
+                                  type_size(e.props[0].type) == 4;
+            if (fixed || tri_list) {
+                if (tri_list) rec = 1 + 3 * 4;
+                const uint64_t block = std::max<uint64_t>(1, (64u << 20) / std::max<size_t>(rec, 1));
+                std::vector<unsigned char> buf;
+                for (uint64_t r0 = 0; r0 < e.count; r0 += block) {
+                    const uint64_t nr = std::min<uint64_t>(block, e.count - r0);
+                    buf.resize(nr * rec);
+                    if (rec && !in.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(nr * rec)))
+                        throw std::runtime_error("PLY: unexpected end of binary data");
+                    for (uint64_t r = 0; r < nr; ++r) {
+                        const unsigned char* b = buf.data() + r * rec;
+                        if (tri_list) {
+                            if (decode(b, e.props[0].count_type, swap) != 3.0)
+                                throw std::runtime_error("PLY: only triangle faces are supported");
+                            uint32_t* f = &mesh->mF[r0 + r].v0;
+                            for (int q = 0; q < 3; ++q)
+                                f[q] = (uint32_t)(int64_t)decode(b + 1 + 4 * q, e.props[0].type, swap);
+                        } else if (is_vertex) {
+                            auto v = [&](int pi) { return decode(b + off[pi], e.props[pi].type, swap); };
+                            mesh->mV[r0 + r] = V3{(float)v(ix), (float)v(iy), (float)v(iz)};
+                            if (!mesh->mN.empty()) mesh->mN[r0 + r] = V3{(float)v(inx), (float)v(iny), (float)v(inz)};
+                            if (!mesh->mC.empty())
+                                mesh->mC[r0 + r] = C4{0, (uint8_t)v(ir), (uint8_t)v(ig), (uint8_t)v(ib)};
+                        }
+                    }
+                }
+                continue;
+            }
+        }
         std::vector<double> vals(e.props.size());
         for (uint64_t r = 0; r < e.count; ++r) {
             for (size_t pi = 0; pi < e.props.size(); ++pi) {
@@ -214,18 +267,40 @@ inline void write(const std::string& path, Mesh* mesh) {
         f << "property list uchar uint vertex_indices\n";
     }
     f << "end_header\n";
-    for (size_t i = 0; i < mesh->nV(); ++i) {
-        f.write(reinterpret_cast<const char*>(&mesh->mV[i]), sizeof(V3));
-        if (normals) f.write(reinterpret_cast<const char*>(&mesh->mN[i]), sizeof(V3));
-        if (colors) {
-            const C3 c{mesh->mC[i].r, mesh->mC[i].g, mesh->mC[i].b};
-            f.write(reinterpret_cast<const char*>(&c), sizeof(C3));
+    // records are assembled in memory and written in blocks (one stream write per block, not per value)
+    const size_t vrec = sizeof(V3) + (normals ? sizeof(V3) : 0) + (colors ? sizeof(C3) : 0);
+    const size_t frec = 1 + sizeof(I3);
+    std::vector<char> buf;
+    const size_t block = 1 << 20;
+    for (size_t i0 = 0; i0 < mesh->nV(); i0 += block) {
+        const size_t nr = std::min(block, mesh->nV() - i0);
+        buf.resize(nr * vrec);
+        char* o = buf.data();
+        for (size_t i = i0; i < i0 + nr; ++i) {
+            std::memcpy(o, &mesh->mV[i], sizeof(V3));
+            o += sizeof(V3);
+            if (normals) {
+                std::memcpy(o, &mesh->mN[i], sizeof(V3));
+                o += sizeof(V3);
+            }
+            if (colors) {
+                const C3 c{mesh->mC[i].r, mesh->mC[i].g, mesh->mC[i].b};
+                std::memcpy(o, &c, sizeof(C3));
+                o += sizeof(C3);
+            }
         }
+        f.write(buf.data(), (std::streamsize)buf.size());
     }
-    const unsigned char three = 3;
-    for (size_t i = 0; i < mesh->nF(); ++i) {
-        f.write(reinterpret_cast<const char*>(&three), 1);
-        f.write(reinterpret_cast<const char*>(&mesh->mF[i]), sizeof(I3));
+    for (size_t i0 = 0; i0 < mesh->nF(); i0 += block) {
+        const size_t nr = std::min(block, mesh->nF() - i0);
+        buf.resize(nr * frec);
+        char* o = buf.data();
+        for (size_t i = i0; i < i0 + nr; ++i) {
+            *o++ = 3;
+            std::memcpy(o, &mesh->mF[i], sizeof(I3));
+            o += sizeof(I3);
+        }
+        f.write(buf.data(), (std::streamsize)buf.size());
     }
     if (!f) throw std::runtime_error("PLY: write failed for " + path);
 }
