@@ -39,8 +39,9 @@ def parse():
     p.add_argument("--no-learned", action="store_true", help="skip the learned-proposal vs plain comparison")
     p.add_argument("--no-query", action="store_true", help="skip the query-vs-reference search section")
     p.add_argument("--no-graph", action="store_true", help="launch the Taubin passes one by one (profiling)")
-    p.add_argument("--workload", choices=["torus", "batch"], default="torus",
-                   help="torus: one 4M-point cloud (default); batch: 512 uneven superquadric clouds")
+    p.add_argument("--workload", choices=["torus", "batch", "query"], default="torus",
+                   help="torus: one 4M-point cloud (default); batch: 512 uneven superquadric clouds; query: only "
+                        "the query-vs-reference search against the torus")
     p.add_argument("--clouds", type=int, default=512)
     return p.parse_args()
 
@@ -167,6 +168,23 @@ def query_search(E, dev_xyz, args):
     return res
 
 
+def query_main(args):
+    """Only the query-vs-reference search (query_search) against the torus reference cloud; one JSON line."""
+    import torch
+
+    from multigpuegtsmoothing_amd import datasets as D
+    from multigpuegtsmoothing_amd import egt as E
+
+    torch.cuda.set_device(0)
+    dev = tuple(torch.from_numpy(a).cuda() for a in D.torus(args.n, seed=2))
+    res = query_search(E, dev, args)
+    out = {"metric": "query-vs-reference kNN queries/s (GRIDCUDAKNNCompact)", "value": res["k1"]["queries_per_s"],
+           "unit": "queries/s at k=1", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"reference torus {args.n} pts (seed 2), {res['queries']}", "k": [1, 16]},
+           "query_knn": res}
+    print(json.dumps(out), flush=True)
+
+
 def batch_main(args):
     """Batched uneven workload: args.clouds superquadric clouds, sizes log-uniform in [5k, 500k], k drawn
     from {16, 40, 64}, alpha 0.65, 100 iterations (lambda 0.5, mu -0.53), learned proposal on (model trained
@@ -242,6 +260,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         batch_main(args)
+        return
+    if args.workload == "query":
+        query_main(args)
         return
     import torch
 
