@@ -60,6 +60,8 @@ typedef struct {
     int64_t query_invalid_rows; /* its rows blanked by the reference's validity rule */
     float query_ms;            /* its device time (query sort + search) */
     int32_t query_k;           /* its neighbour count */
+    float taubin_pass_ms;      /* last smoothing call: device time per lambda or mu pass (the passes alone) */
+    float taubin_setup_ms;     /* its tile set-up and graph build (taubin_ms = setup + all passes) */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */
