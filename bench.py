@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=250_000)
     p.add_argument("--no-learned", action="store_true", help="skip the learned-proposal vs plain comparison")
     p.add_argument("--no-query", action="store_true", help="skip the query-vs-reference search section")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="start every kNN query at its own cell (the reference schedule) instead of the autotuned level")
     p.add_argument("--no-graph", action="store_true", help="launch the Taubin passes one by one (profiling)")
     p.add_argument("--workload", choices=["torus", "batch", "query"], default="torus",
                    help="torus: one 4M-point cloud (default); batch: 512 uneven superquadric clouds; query: only "
@@ -61,8 +63,10 @@ def measured_traffic():
     return None
 
 
-def gpu_step(E, torch, dev_xyz, args, stream_sync):
+def gpu_step(E, torch, dev_xyz, args, start_level):
     g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+    if start_level is not None:
+        g.set_option(E.OPT_START_LEVEL, start_level)
     if args.no_graph:
         g.set_option(E.OPT_TAUBIN_GRAPH, 0)
     g.GRIDCUDAKNNSELF_COMPACT(args.k, 1e8)
@@ -74,10 +78,10 @@ def gpu_step(E, torch, dev_xyz, args, stream_sync):
     return st
 
 
-def learned_vs_plain(E, dev_xyz, args):
-    """kNN stage of the same workload with the plain start (the query's own cell, as the reference) and
-    with the learned proposal (model trained on the GPU on a torus drawn with a different seed). Rows must
-    be identical; the learned time includes the proposal kernel."""
+def learned_vs_plain(E, dev_xyz, args, start_level=None):
+    """kNN stage of the same workload with the plain start (the query's own cell, as the reference), with the
+    learned proposal (model trained on the GPU on a torus drawn with a different seed) and, when given, with
+    the autotuned start level. Rows must be identical; the learned time includes the proposal kernel."""
     from multigpuegtsmoothing_amd import datasets as D
 
     k = args.k
@@ -89,8 +93,11 @@ def learned_vs_plain(E, dev_xyz, args):
     train.close()
     res = {}
     rows = {}
-    for mode in ("plain", "learned"):
+    modes = ("plain", "learned") + (("autotuned",) if start_level is not None else ())
+    for mode in modes:
         g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+        if mode == "autotuned":
+            g.set_option(E.OPT_START_LEVEL, start_level)
         best = None
         for _ in range(3):
             prop = 0.0
@@ -107,10 +114,14 @@ def learned_vs_plain(E, dev_xyz, args):
         rows[mode] = g.GetKNNIndexes()
         g.close()
     model.close()
-    return {"plain": res["plain"], "learned": res["learned"],
-            "knn_speedup": res["plain"]["knn_ms"] / res["learned"]["knn_ms"],
-            "bit_exact": bool(np.array_equal(rows["plain"], rows["learned"])),
-            "train": {"cloud": "torus 2M pts, seed 1002", "seconds": train_s, "mse": mse}}
+    out = {"plain": res["plain"], "learned": res["learned"],
+           "knn_speedup": res["plain"]["knn_ms"] / res["learned"]["knn_ms"],
+           "bit_exact": bool(np.array_equal(rows["plain"], rows["learned"])),
+           "train": {"cloud": "torus 2M pts, seed 1002", "seconds": train_s, "mse": mse}}
+    if "autotuned" in res:
+        out["autotuned"] = dict(res["autotuned"], start_level=start_level,
+                                bit_exact=bool(np.array_equal(rows["plain"], rows["autotuned"])))
+    return out
 
 
 def cpu_baseline(args):
@@ -295,13 +306,26 @@ def main():
     x, y, z = D.torus(args.n, seed=2)
     dev = tuple(torch.from_numpy(a).cuda() for a in (x, y, z))
     torch.cuda.synchronize()
+    # The search's start level (the cell size of the first box; rows do not depend on it) is chosen once for
+    # the workload by the library's autotuner, outside the timed steps. --no-autotune keeps the reference
+    # schedule's own-cell start.
+    start_level, tune = None, None
+    if not args.no_autotune:
+        g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
+        t0 = time.perf_counter()
+        lvl, ms, cost = g.autotune(args.k)
+        tune = {"start_level": int(lvl), "seconds": time.perf_counter() - t0,
+                "trial_ms": [round(float(v), 3) for v in ms if v >= 0],
+                "trial_cost": [round(float(v), 1) for v in cost if v >= 0]}
+        g.close()
+        start_level = int(lvl)
     for _ in range(args.warmup):
-        gpu_step(E, torch, dev, args, None)
+        gpu_step(E, torch, dev, args, start_level)
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(gpu_step(E, torch, dev, args, None))
+        stats.append(gpu_step(E, torch, dev, args, start_level))
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
 
@@ -348,7 +372,10 @@ def main():
         "config": {"workload": f"torus {n} pts (R=1, r=0.3, density 1-10x, sigma 0.003, seed 2), cloud mode, "
                                f"k={k}, alpha=0.65, {args.iters} Taubin its (lambda 0.5, mu -0.53)"
                                + ("" if args.no_normals else ", PCA normals + MST orientation"),
-                   "n_points": n, "k": k, "iterations": args.iters, "path": "plain"},
+                   "n_points": n, "k": k, "iterations": args.iters, "path": "plain",
+                   "knn_start_level": "own cell (reference schedule)" if start_level is None
+                   else f"{start_level} (autotuned once per workload, outside the timed steps)"},
+        "autotune": tune,
         "knn_queries_per_s": n / (knn_ms * 1e-3),
         "taubin_point_iters_per_s": n * args.iters / (taub_ms * 1e-3) if taub_ms > 0 else None,
         "taubin_setup_ms": med("taubin_setup_ms"),
@@ -359,7 +386,7 @@ def main():
         "roofline": roof,
     }
     if not args.no_learned:
-        out["learned_vs_plain"] = learned_vs_plain(E, dev, args)
+        out["learned_vs_plain"] = learned_vs_plain(E, dev, args, start_level)
     if not args.no_query:
         out["query_knn"] = query_search(E, dev, args)
     if not args.no_cpu_baseline:
