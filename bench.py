@@ -41,9 +41,10 @@ def parse():
     p.add_argument("--no-autotune", action="store_true",
                    help="start every kNN query at its own cell (the reference schedule) instead of the autotuned level")
     p.add_argument("--no-graph", action="store_true", help="launch the Taubin passes one by one (profiling)")
-    p.add_argument("--workload", choices=["torus", "batch", "query"], default="torus",
+    p.add_argument("--workload", choices=["torus", "batch", "query", "mesh", "mixture"], default="torus",
                    help="torus: one 4M-point cloud (default); batch: 512 uneven superquadric clouds; query: only "
-                        "the query-vs-reference search against the torus")
+                        "the query-vs-reference search against the torus; mesh: the 655k-vertex icosphere mesh; "
+                        "mixture: the 16M-point cluster mixture, plain vs learned vs autotuned at k=40 and k=64")
     p.add_argument("--clouds", type=int, default=512)
     return p.parse_args()
 
@@ -179,6 +180,143 @@ def query_search(E, dev_xyz, args):
     return res
 
 
+def _device_rows(torch, ptr, count):
+    """Device-to-device copy of a neighbour-row buffer into a torch int32 tensor (row comparison on the GPU)."""
+    import ctypes
+
+    t = torch.empty(count, dtype=torch.int32, device="cuda")
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipMemcpy(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(ptr), ctypes.c_size_t(count * 4), 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return t
+
+
+def mesh_main(args):
+    """Mesh mode: icosphere subdivided 8 times (655,362 vertices, 1,310,720 triangles), radial noise +-0.01,
+    seed 3; k=40, alpha=1e-8, 150 Taubin iterations on the mesh adjacency, then PCA normals with MST
+    orientation. The mesh provider takes host arrays (as the reference's GridMeshCuda does), so each step
+    includes uploading the coordinates and triangles (7.9 MB + 15.7 MB)."""
+    import torch
+
+    from multigpuegtsmoothing_amd import datasets as D
+    from multigpuegtsmoothing_amd import egt as E
+
+    torch.cuda.set_device(0)
+    x, y, z, tri = D.icosphere(8, seed=3)
+    n = x.size
+    iters = 150
+
+    def step():
+        m = E.GridMeshCuda((x, y, z), tri)
+        m.GetNeighborsFromTriangles(40)
+        E.EGT_CUDA(m).PerformSmoothing(iters, 0.5, -0.53, 1e-8)
+        E.Normal_CUDA(m).GetNormals()
+        st = m.stats()
+        m.close()
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = stats[-1]
+    out = {"metric": "mesh-mode points/s (grid + mesh adjacency + EGG + Taubin + normals)", "value": n / dt,
+           "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"icosphere 8 subdivisions ({n} vertices, {tri.shape[0]} triangles), radial noise "
+                                  "0.01, seed 3, k=40, alpha=1e-8, 150 Taubin its, PCA normals + MST orientation",
+                      "n_points": n},
+           "stage_ms": {"grid": st["grid_ms"], "mesh_adjacency": st["knn_ms"], "egg_csr": st["egg_ms"],
+                        "taubin": st["taubin_ms"], "normals": st["normals_ms"]},
+           "taubin_point_iters_per_s": n * iters / (st["taubin_ms"] * 1e-3), "egg_kept_per_point": st["egg_kept"] / n}
+    print(json.dumps(out), flush=True)
+
+
+def mixture_main(args):
+    """Learned radius proposal on 16,000,000 points from the 12-cluster / 4-plane mixture (seed 4), k=40 and
+    k=64, alpha 0.65, 150 Taubin iterations. The model is trained on the GPU (outside the timing) on the plain
+    search's radii of a 2M-point mixture drawn with seed 1000. One step = grid + (proposal) + kNN + EGG +
+    Taubin; plain = own-cell start, learned = proposal, autotuned = the tuner's level (tuned outside the
+    timing). Rows of the three are compared."""
+    import torch
+
+    from multigpuegtsmoothing_amd import datasets as D
+    from multigpuegtsmoothing_amd import egt as E
+
+    torch.cuda.set_device(0)
+    pts = D.mixture(16_000_000, seed=4)
+    n = pts[0].size
+    dev = tuple(torch.from_numpy(a).cuda() for a in pts)
+    del pts
+    train = D.mixture(2_000_000, seed=1000)
+    iters = 150
+    res = {}
+    for k in (40, 64):
+        model = E.RadiusModel(seed=17)
+        gt = E.GridStructure(train, 1, 1e-6, 1024, 4)
+        t0 = time.perf_counter()
+        mse = model.train(gt, k, epochs=6, batch=4096, max_samples=2_000_000, lr=2e-3, quantile=0.9)
+        train_s = time.perf_counter() - t0
+        gt.close()
+        g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
+        lvl = int(g.autotune(k)[0])
+        g.close()
+        per = {}
+        rows = {}
+        for mode in ("plain", "learned", "autotuned"):
+            def step():
+                g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
+                if mode == "learned":
+                    model.propose(g, k)
+                if mode == "autotuned":
+                    g.set_option(E.OPT_START_LEVEL, lvl)
+                g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
+                r = g.GetKNNIndexesDevice()
+                E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, 0.65)
+                st = g.stats()
+                return st, g, r
+            g0 = step()[1]
+            torch.cuda.synchronize()
+            g0.close()
+            ts, st = [], None
+            for rep in range(2):
+                t0 = time.perf_counter()
+                st, g, r = step()
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+                if rep == 1:
+                    rows[mode] = _device_rows(torch, r, n * k)
+                g.close()
+            dt = min(ts)
+            print(f"mixture k={k} {mode}: {dt * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+            per[mode] = {"ms_per_step": dt * 1e3, "points_per_s": n / dt,
+                         "knn_ms": st["knn_ms"] + st["propose_ms"], "egg_ms": st["egg_ms"], "taubin_ms": st["taubin_ms"],
+                         "grid_ms": st["grid_ms"], "candidates_per_query": st["knn_candidates"] / n,
+                         "invalid_rows": st["invalid_rows"],
+                         "fallback_rate": st["learned_fallbacks"] / n if mode == "learned" else None}
+        res[f"k{k}"] = dict(per, start_level_autotuned=lvl,
+                            rows_equal=bool(torch.equal(rows["plain"], rows["learned"])
+                                            and torch.equal(rows["plain"], rows["autotuned"])),
+                            learned_end_to_end_speedup=per["plain"]["ms_per_step"] / per["learned"]["ms_per_step"],
+                            learned_knn_speedup=per["plain"]["knn_ms"] / per["learned"]["knn_ms"],
+                            train={"cloud": "mixture 2M pts, seed 1000", "seconds": train_s, "mse": mse})
+        model.close()
+        del rows
+    out = {"metric": "learned-proposal vs plain, mixture 16M (points/s end to end, grid+kNN+EGG+Taubin)",
+           "value": res["k40"]["learned"]["points_per_s"], "unit": "points/s at k=40, learned", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"mixture {n} pts (12 anisotropic clusters + 4 planes, seed 4), k=40 and 64, "
+                                  f"alpha 0.65, {iters} Taubin its", "n_points": n},
+           "results": res}
+    print(json.dumps(out), flush=True)
+
+
 def query_main(args):
     """Only the query-vs-reference search (query_search) against the torus reference cloud; one JSON line."""
     import torch
@@ -296,6 +434,12 @@ def main():
         return
     if args.workload == "query":
         query_main(args)
+        return
+    if args.workload == "mesh":
+        mesh_main(args)
+        return
+    if args.workload == "mixture":
+        mixture_main(args)
         return
     import torch
 

@@ -54,7 +54,8 @@ typedef struct {
     int64_t knn_cells;         /* cells looked up by the kNN search */
     float taubin_tile_ms[2];   /* Taubin tile autotuner: ms per pass at tile width 1024 and 2048 (0 if untried) */
     int32_t taubin_tile;       /* tile width of the last smoothing call */
-    int32_t taubin_fallback_tiles; /* tiles of that call whose halo did not fit in LDS (global gathers) */
+    int32_t taubin_fallback_tiles; /* tiles of that call whose halo list exceeded its cap (CSR rows, global
+                                      gathers) */
     int64_t query_count;       /* queries of the last query-vs-reference search */
     int64_t query_candidates;  /* candidate distances it evaluated */
     int64_t query_invalid_rows; /* its rows blanked by the reference's validity rule */
@@ -62,6 +63,8 @@ typedef struct {
     int32_t query_k;           /* its neighbour count */
     float taubin_pass_ms;      /* last smoothing call: device time per lambda or mu pass (the passes alone) */
     float taubin_setup_ms;     /* its tile set-up and graph build (taubin_ms = setup + all passes) */
+    int32_t taubin_overflow_tiles; /* tiles of that call whose halo only partly fit in LDS (the rest is read
+                                      from global memory) */
 } egt_stats;
 
 /* Options set through egt_set_option (no reference counterpart). */

@@ -5,9 +5,9 @@
 //
 // Points live in Morton order as float4 (x, y, z, pad) in two buffers, so a wave's 64 points are spatial
 // neighbours and their neighbour gathers (one 16-byte load each) mostly hit the same cache lines. Blocks
-// are remapped so each of the 8 XCDs walks one contiguous Morton range (workgroups are dealt to XCDs round
-// robin), which keeps a block's neighbours in its own XCD's L2. The 2*iterations passes are recorded once
-// into a hipGraph and replayed. The exp is egt_expf, so results match the host oracle bit for bit; the
+// are remapped so each of the 8 XCDs walks chunks of consecutive Morton tiles (workgroups are dealt to XCDs
+// round robin), which keeps a block's neighbours in its own XCD's L2. The 2*iterations passes are recorded
+// into hipGraphs and replayed. The exp is egt_expf, so results match the host oracle bit for bit; the
 // per-point arithmetic and neighbour order are those of the reference, only the memory layout differs.
 #include "egt_device.h"
 #include "egt_internal.h"
@@ -17,23 +17,29 @@ namespace {
 
 constexpr int TBLK = 256;
 constexpr int NXCD = 8;
-
-__device__ inline int64_t xcd_block(int nblocks) {
+// Block -> tile map of a pass. Workgroups are dealt to the 8 XCDs round robin; XCD x takes the chunks x, x + 8,
+// x + 16, ... of XCHUNK consecutive tiles. Neighbouring tiles share halo points, so a chunk mostly reads its
+// halo from its own XCD's L2, and every XCD gets a share of every region of the cloud: a dense region (slower
+// per tile) is spread over all XCDs instead of holding back the one whose contiguous range it falls in.
+// The grid is padded to a multiple of NXCD * XCHUNK blocks, over which the map is a bijection.
+constexpr int XCHUNK = 16;
+__device__ inline int64_t xcd_tile() {
     const int b = blockIdx.x;
-    const int per = (nblocks + NXCD - 1) / NXCD;
-    return (int64_t)(b % NXCD) * per + b / NXCD;
+    const int x = b % NXCD, j = b / NXCD;
+    return ((int64_t)(j / XCHUNK) * NXCD + x) * XCHUNK + j % XCHUNK;
 }
 
 // ---------------------------------------------------------------------------------------------------
 // LDS-tiled pass. A tile is TILE consecutive Morton positions (TILE = 1024 or 2048, two rows per thread); its
-// 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most TILE entries) is listed once per
-// smoothing call, and every CSR column is rewritten as a 16-bit local index
+// 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most halo_max<TILE>() entries) is
+// listed once per smoothing call, and every CSR column is rewritten as a 16-bit local index
 // (< TILE: inside the tile; TILE + h: halo entry h). The columns are then laid out as sliced ELLPACK: the
 // tile's rows sorted by length, 64 rows (one wave) per slice, slot-major within a slice. A pass stages tile
 // + halo coordinates in LDS and reads all neighbours from LDS; the lanes of a wave walk rows of nearly equal
 // length and each slot load is one coalesced 128-byte line. The pass is limited by VALU issue (the weight's
-// exp per neighbour) and by the latency of its dependent loads; equal-length lanes cut the issued VALU. Tiles
-// whose halo exceeds TILE use CSR rows with global gathers.
+// exp per neighbour) and by the latency of its dependent loads; equal-length lanes cut the issued VALU. Only
+// the first halo_cap<TILE>() halo entries fit in LDS; a tile with more reads the rest from global memory
+// through its halo list. Tiles with more than halo_max<TILE>() halo entries use CSR rows with global gathers.
 // Tile preparation runs one 1024-thread block per tile: its sorts are barrier-bound, so four times the threads
 // of a 256-thread block take the set-up of a 4M-point torus from 2.9 to 1.1 ms.
 constexpr int PBLK = 1024;
@@ -41,8 +47,16 @@ constexpr int PBLK = 1024;
 // per CU instead of two: 64.2 vs 67.7 ms for 300 iterations on a 4M-point torus.
 template <int TILE>
 constexpr int tile_threads() { return TILE == 1024 ? 512 : 1024; }
-constexpr int SORT_CAP = 8192;
-
+// Largest halo a tile keeps in LDS: (TILE + cap) float4 plus the slice offsets stay 1 KiB under a half (2048)
+// or a quarter (1024) of the CU's 160 KiB, so two 1024-thread or four 512-thread blocks still fit per CU. In
+// volumetric regions a 2048-point Morton range has a halo of 2000-3000 points at 16 kept neighbours a point.
+template <int TILE>
+constexpr int halo_cap() { return TILE == 1024 ? 1456 : 2992; }
+// Longest halo list of a tile (the list's stride per tile); beyond it the tile uses CSR rows.
+template <int TILE>
+constexpr int halo_max() { return TILE == 1024 ? 4096 : 8192; }
+static_assert((2048 + halo_cap<2048>()) * 16 + 4 * 33 <= 160 * 1024 / 2 - 1024, "two tile blocks per CU");
+static_assert((1024 + halo_cap<1024>()) * 16 + 4 * 17 <= 160 * 1024 / 4 - 1024, "four tile blocks per CU");
 __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -62,14 +76,23 @@ __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
     }
 }
 
+// LDS hash set of a tile's external columns: open addressing, linear probing, 2 * halo_max slots. An insert
+// is only started while the count is at most halo_max, so the set never holds more than halo_max + PBLK
+// entries and every probe sequence reaches an empty slot.
+template <int TILE>
+constexpr int hset_size() { return 2 * halo_max<TILE>(); }
+constexpr uint32_t HS_EMPTY = 0xFFFFFFFFu;
+
 template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                      int64_t n, uint16_t* __restrict__ col16,
                                                      int32_t* __restrict__ halo, int32_t* __restrict__ hcount,
                                                      uint16_t* __restrict__ perm, uint32_t* __restrict__ width,
                                                      uint32_t* __restrict__ padded) {
-    __shared__ uint32_t raw[SORT_CAP];
-    __shared__ uint32_t uniq[SORT_CAP];
+    constexpr int HS = hset_size<TILE>(), HMAX = halo_max<TILE>();
+    static_assert((HS & (HS - 1)) == 0 && HS >= TILE && HS >= HMAX + PBLK, "hash set size");
+    __shared__ uint32_t hs[HS];      // hash set, then the row-sort keys
+    __shared__ uint32_t uniq[HMAX];  // the halo, sorted
     __shared__ uint32_t cnt, wsum[PBLK / 64];
     __shared__ uint32_t swid[TILE / 64];
     const int64_t t = blockIdx.x;
@@ -77,33 +100,37 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     const int np = (int)(end - base);
     const int32_t eb = rowptr[base], ee = rowptr[end];
     if (threadIdx.x == 0) cnt = 0;
+    for (int i = threadIdx.x; i < HS; i += PBLK) hs[i] = HS_EMPTY;
     __syncthreads();
     for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
-        const int32_t c = col[e];
-        if (c < base || c >= end) {
-            const uint32_t p = atomicAdd(&cnt, 1u);
-            if (p < SORT_CAP) raw[p] = (uint32_t)c;
+        const uint32_t c = (uint32_t)col[e];
+        if ((int64_t)c >= base && (int64_t)c < end) continue;
+        if (__atomic_load_n(&cnt, __ATOMIC_RELAXED) > (uint32_t)HMAX) continue;  // over the cap: CSR tile
+        uint32_t slot = (c * 0x9E3779B1u) & (uint32_t)(HS - 1);
+        for (;;) {
+            const uint32_t old = atomicCAS(&hs[slot], HS_EMPTY, c);
+            if (old == HS_EMPTY) {
+                atomicAdd(&cnt, 1u);
+                break;
+            }
+            if (old == c) break;
+            slot = (slot + 1) & (uint32_t)(HS - 1);
         }
     }
     __syncthreads();
-    const uint32_t total = cnt;
-    if (total > SORT_CAP) {
+    const uint32_t H = cnt;
+    if (H > (uint32_t)HMAX) {
         if (threadIdx.x == 0) {
             hcount[t] = -1;
             padded[t] = 0;
         }
         return;
     }
-    int n2 = 1;
-    while (n2 < (int)total) n2 <<= 1;
-    for (int i = total + threadIdx.x; i < n2; i += PBLK) raw[i] = 0xFFFFFFFFu;
-    __syncthreads();
-    block_bitonic_sort(raw, n2);
-    // unique: each thread owns a contiguous chunk
-    const int per = (n2 + PBLK - 1) / PBLK;
-    const int c0 = threadIdx.x * per, c1 = min(c0 + per, (int)total);
+    // compact the set into uniq: each thread owns a contiguous chunk of slots
+    constexpr int PER = HS / PBLK;
+    const int c0 = threadIdx.x * PER;
     uint32_t mine = 0;
-    for (int i = c0; i < c1; ++i) mine += (i == 0 || raw[i] != raw[i - 1]);
+    for (int i = c0; i < c0 + PER; ++i) mine += hs[i] != HS_EMPTY;
     uint32_t v = mine;
     const int lane = __lane_id(), wid = threadIdx.x / 64;
 #pragma unroll
@@ -113,22 +140,17 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     }
     if (lane == 63) wsum[wid] = v;
     __syncthreads();
-    uint32_t off = v - mine, H = 0;
-    for (int w = 0; w < PBLK / 64; ++w) {
-        if (w < wid) off += wsum[w];
-        H += wsum[w];
-    }
-    for (int i = c0; i < c1; ++i)
-        if (i == 0 || raw[i] != raw[i - 1]) uniq[off++] = raw[i];
+    uint32_t off = v - mine;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    for (int i = c0; i < c0 + PER; ++i)
+        if (hs[i] != HS_EMPTY) uniq[off++] = hs[i];
+    int n2 = 1;
+    while (n2 < (int)H) n2 <<= 1;
     __syncthreads();
-    if (H > TILE) {
-        if (threadIdx.x == 0) {
-            hcount[t] = -1;
-            padded[t] = 0;
-        }
-        return;
-    }
-    for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * TILE + h] = (int32_t)uniq[h];
+    for (int i = (int)H + threadIdx.x; i < n2; i += PBLK) uniq[i] = HS_EMPTY;
+    __syncthreads();
+    block_bitonic_sort(uniq, n2);
+    for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * halo_max<TILE>() + h] = (int32_t)uniq[h];
     for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
         const uint32_t c = (uint32_t)col[e];
         uint16_t loc;
@@ -151,13 +173,13 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     for (int i = threadIdx.x; i < TILE; i += PBLK) {
         uint32_t key = 0xFFFFFFFFu;
         if (i < np) key = ((uint32_t)min(rowptr[base + i + 1] - rowptr[base + i], (1 << 20) - 1) << 12) | (uint32_t)i;
-        raw[i] = key;
+        hs[i] = key;
     }
     for (int s = threadIdx.x; s < TILE / 64; s += PBLK) swid[s] = 0;
     __syncthreads();
-    block_bitonic_sort(raw, TILE);
+    block_bitonic_sort(hs, TILE);
     for (int j = threadIdx.x; j < np; j += PBLK) {
-        const uint32_t li = raw[j] & 0xFFFu;
+        const uint32_t li = hs[j] & 0xFFFu;
         perm[t * TILE + j] = (uint16_t)li;
         atomicMax(&swid[j >> 6], (uint32_t)(rowptr[base + li + 1] - rowptr[base + li]));
     }
@@ -219,19 +241,28 @@ __device__ inline float taubin_weight(float dist, float scale) {
     return egt_expf_taubin(-dist);
 }
 
-template <bool REG>
+// With OVF, local columns at or past lim (TILE + the LDS part of the halo) are halo entries kept in global
+// memory: their coordinates are gathered from gin through the tile's halo list hal (indexed by column - TILE).
+template <bool REG, bool OVF>
 __device__ inline void taubin_sell_row(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ sc,
-                                       int W, float scale, float& cgx, float& cgy, float& cgz, float& sum) {
+                                       int W, float scale, float& cgx, float& cgy, float& cgz, float& sum,
+                                       const float4* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
+                                       uint32_t tile) {
     constexpr int CH = 8;
     for (int u0 = 0; u0 < W; u0 += CH) {
         uint32_t c[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) c[u] = u0 + u < W ? (uint32_t)sc[(u0 + u) * 64] : 0xFFFFu;
-        // every LDS read of the chunk is issued before the first use (padding slots read entry 0)
+        // every read of the chunk is issued before the first use (padding slots read entry 0)
         float4 qs[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u)
-            if (u0 + u < W) qs[u] = src[c[u] != 0xFFFFu ? c[u] : 0u];
+        for (int u = 0; u < CH; ++u) {
+            if (u0 + u < W) {
+                const uint32_t cc = c[u] != 0xFFFFu ? c[u] : 0u;
+                if (!OVF || cc < lim) qs[u] = src[cc];
+                else qs[u] = gin[hal[cc - tile]];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             if (c[u] != 0xFFFFu) {
@@ -273,7 +304,7 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     static_assert(S <= 64 && S % NW == 0 && (S / NW) <= 2, "slices per wave");
     extern __shared__ float4 lds[];
     __shared__ uint32_t sbase[S + 1];
-    const int64_t t = xcd_block(ntiles);
+    const int64_t t = xcd_tile();
     if (t >= ntiles) return;
     const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
     const int np = (int)(end - base);
@@ -308,7 +339,9 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
         if (lane == 0) sbase[0] = 0;
     }
     for (int i = threadIdx.x; i < np; i += TBLK_TILE) lds[i] = in[base + i];
-    for (int h = threadIdx.x; h < H; h += TBLK_TILE) lds[TILE + h] = in[halo[t * TILE + h]];
+    const int32_t* hal = halo + t * halo_max<TILE>();
+    const int HL = min(H, halo_cap<TILE>());  // halo entries staged in LDS; the rest are read from global
+    for (int h = threadIdx.x; h < HL; h += TBLK_TILE) lds[TILE + h] = in[hal[h]];
     __syncthreads();
     const uint16_t* tc = scol + toff[t];
     float4 res[S / NW];
@@ -323,7 +356,10 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
         // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
         const uint32_t sb0 = __builtin_amdgcn_readfirstlane(sbase[s]);
         const int W = (int)((__builtin_amdgcn_readfirstlane(sbase[s + 1]) - sb0) >> 6);
-        taubin_sell_row<REG>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum);
+        if (H > halo_cap<TILE>())
+            taubin_sell_row<REG, true>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE);
+        else
+            taubin_sell_row<REG, false>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE);
         res[r] = taubin_result(p, cgx, cgy, cgz, sum, scale);
     }
     __syncthreads();
@@ -355,7 +391,7 @@ namespace {
 // Tile preparation for tile width TILE (once per smoothing call): halo lists and 16-bit local columns.
 struct TilePlan {
     int tile = 0, ntiles = 0, maxH = 0;
-    int64_t fallback_tiles = 0;
+    int64_t fallback_tiles = 0, overflow_tiles = 0;  // CSR tiles; tiles with part of the halo in global memory
     uint16_t* col16 = nullptr;   // CSR-order local columns
     int32_t* halo = nullptr;
     int32_t* hcount = nullptr;
@@ -374,7 +410,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
     tp.tile = TILE;
     tp.ntiles = ceil_div(n, TILE);
     tp.col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
-    tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * TILE);
+    tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * halo_max<TILE>());
     tp.hcount = h->scratch.get<int32_t>(21, tp.ntiles);
     tp.perm = h->scratch.get<uint16_t>(28, (size_t)tp.ntiles * TILE);
     tp.width = h->scratch.get<uint32_t>(29, (size_t)tp.ntiles * (TILE / 64));
@@ -392,7 +428,8 @@ TilePlan prepare_tiles(egt_ctx* h) {
     uint64_t tot = 0;
     for (int t = 0; t < tp.ntiles; ++t) {
         if (hc[t] < 0) ++tp.fallback_tiles;
-        else tp.maxH = std::max(tp.maxH, hc[t]);
+        else tp.maxH = std::max(tp.maxH, std::min(hc[t], halo_cap<TILE>()));
+        if (hc[t] > halo_cap<TILE>()) ++tp.overflow_tiles;
         off[t] = tot;  // padded sizes are multiples of 64 slots, so every slice starts 128-byte aligned
         tot += pad[t];
     }
@@ -409,7 +446,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
 template <int TILE, bool REG>
 void launch_tile(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
     const size_t lds_bytes = (size_t)(tp.tile + tp.maxH) * sizeof(float4);
-    const dim3 tgrid(ceil_div(tp.ntiles, NXCD) * NXCD), tblk(tile_threads<TILE>());
+    const dim3 tgrid(ceil_div(tp.ntiles, NXCD * XCHUNK) * NXCD * XCHUNK), tblk(tile_threads<TILE>());
     k_taubin_tile<TILE, REG><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
                                                                     tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
                                                                     tp.ntiles);
@@ -481,6 +518,7 @@ void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     h->taubin_fallback_tiles = tp.fallback_tiles;
     h->stats.taubin_tile = tile;
     h->stats.taubin_fallback_tiles = (int32_t)tp.fallback_tiles;
+    h->stats.taubin_overflow_tiles = (int32_t)tp.overflow_tiles;
     auto pass = [&](const float4* in, float4* out, float scale) { launch_pass(h, tp, in, out, scale); };
     if (iters > 0) {
         if (h->taubin_graph) {

@@ -45,7 +45,8 @@ class Stats(ctypes.Structure):
                 ("taubin_tile", ctypes.c_int32), ("taubin_fallback_tiles", ctypes.c_int32),
                 ("query_count", ctypes.c_int64), ("query_candidates", ctypes.c_int64),
                 ("query_invalid_rows", ctypes.c_int64), ("query_ms", ctypes.c_float), ("query_k", ctypes.c_int32),
-                ("taubin_pass_ms", ctypes.c_float), ("taubin_setup_ms", ctypes.c_float)]
+                ("taubin_pass_ms", ctypes.c_float), ("taubin_setup_ms", ctypes.c_float),
+                ("taubin_overflow_tiles", ctypes.c_int32)]
 
     def as_dict(self):
         d = {f[0]: getattr(self, f[0]) for f in self._fields_}

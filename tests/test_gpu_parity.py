@@ -113,9 +113,8 @@ def test_egg_and_taubin_bit_exact(egt_mod, oracle_mod):
 
 
 def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
-    """A uniform 3-D cloud at k=64 with alpha 1e-8 (the EGG keeps nearly every neighbour) gives tiles whose
-    halo exceeds the LDS budget: exercises the global-gather path of the tiled Taubin pass next to LDS
-    tiles. (At alpha 0.65 the largest halo on this cloud is about 1600 points, inside the budget.)"""
+    """A uniform 3-D cloud at k=64 with alpha 1e-8 (the EGG keeps nearly every neighbour): 2048-point tiles
+    with halos of up to about 2700 points, all inside the LDS budget of a tile."""
     E, O = egt_mod, oracle_mod
     rng = np.random.default_rng(42)
     p = rng.uniform(0, 1, (40_000, 3)).astype(np.float32)
@@ -128,8 +127,41 @@ def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
     ref = O.pipeline(*pts, k, iters, 0.5, -0.53, alpha, normals=False)
     for a, b in zip(out, ref["output"]):
         np.testing.assert_array_equal(a, b)
-    _log("fallback tiles", st["taubin_fallback_tiles"])
-    assert st["taubin_fallback_tiles"] > 0
+    _log("fallback tiles", st["taubin_fallback_tiles"], "overflow tiles", st["taubin_overflow_tiles"])
+    assert st["taubin_fallback_tiles"] == 0 and st["taubin_overflow_tiles"] == 0
+
+
+def _crowded_blob(n=30_000, sigma=0.002, seed=91):
+    """A tight Gaussian blob inside a unit cube (its 8 corners are points too): with 64 cells per axis the blob
+    falls into 8 cells of about 3,800 points, whose points keep the input order, so a Morton tile's halo spans
+    most of a cell. With k=40, alpha 0.65: 2048-point tiles have halos of up to about 6,900 points (5 in LDS,
+    10 partly in global memory); 1024-point tiles have no LDS-only tile, 23 partly global and 7 CSR tiles."""
+    rng = np.random.default_rng(seed)
+    p = (0.5 + sigma * rng.standard_normal((n, 3))).astype(np.float32)
+    p[:8] = np.array([[i & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], np.float32)
+    return p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+
+
+@pytest.mark.parametrize("tile", [2048, 1024])
+def test_taubin_crowded_cells_bit_exact(egt_mod, oracle_mod, tile):
+    """All three kinds of tile in one pass: halo in LDS, halo partly in global memory, CSR rows."""
+    E, O = egt_mod, oracle_mod
+    pts = _crowded_blob()
+    k, iters, alpha = 40, 6, 0.65
+    g = _cloud(E, pts, k, bucket=64)
+    g.set_option(E.OPT_TAUBIN_TILE, tile)
+    E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, alpha)
+    out = E.getOutputPoints(g)
+    st = g.stats()
+    g.close()
+    ref = O.pipeline(*pts, k, iters, 0.5, -0.53, alpha, normals=False, B=64)
+    for a, b in zip(out, ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    _log("crowded", tile, "overflow tiles", st["taubin_overflow_tiles"], "fallback tiles",
+         st["taubin_fallback_tiles"])
+    assert st["taubin_tile"] == tile and st["taubin_overflow_tiles"] > 0
+    if tile == 1024:
+        assert st["taubin_fallback_tiles"] > 0
 
 
 def test_taubin_vs_fp64(egt_mod, oracle_mod):
@@ -243,30 +275,33 @@ def test_taubin_tile_width_keeps_results(egt_mod, oracle_mod, tile):
     g.close()
 
 
-@pytest.mark.parametrize("case", ["sphere", "volumetric"])
+@pytest.mark.parametrize("case", ["sphere", "crowded"])
 def test_regularized_taubin_bit_exact(egt_mod, oracle_mod, case):
     """The reference's dormant isRegularized weights (EGT_CUDA.cu:263): 1 in the lambda pass, 1 / (d^2 + 1e-8)
-    in the mu pass, through LDS tiles and (volumetric, k=64) the global-gather tiles."""
+    in the mu pass, through LDS tiles and (crowded cells, 1024-point tiles) the tiles whose halo is partly in
+    global memory and the CSR tiles."""
     E, O = egt_mod, oracle_mod
+    B = 1024
     if case == "sphere":
         pts, k, iters = D.sphere(20_000, seed=44), 40, 10
     else:
-        rng = np.random.default_rng(45)
-        p = rng.uniform(0, 1, (30_000, 3)).astype(np.float32)
-        pts, k, iters = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()), 64, 3
-    alpha = 0.65 if case == "sphere" else 1e-8  # 1e-8: halos beyond the LDS budget (global-gather tiles)
-    g = _cloud(E, pts, k)
+        pts, k, iters, B = _crowded_blob(), 40, 4, 64
+    alpha = 0.65
+    g = _cloud(E, pts, k, bucket=B)
     g.set_option(E.OPT_TAUBIN_REGULARIZED, 1)
+    if case == "crowded":
+        g.set_option(E.OPT_TAUBIN_TILE, 1024)
     E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, alpha)
     out = E.getOutputPoints(g)
     st = g.stats()
     g.close()
-    og = O.Grid(*pts)
+    og = O.Grid(*pts, B=B)
     knn = og.knn(k)[0]
     egg = O.egg(og.xs, og.ys, og.zs, knn, alpha)
     sx, sy, sz = O.taubin_f32(og.xs, og.ys, og.zs, egg, iters, 0.5, -0.53, regularized=True)
     for a, b in zip(out, og.output(sx, sy, sz)):
         np.testing.assert_array_equal(a, b)
-    _log("regularized", case, "fallback tiles", st["taubin_fallback_tiles"])
-    if case == "volumetric":
-        assert st["taubin_fallback_tiles"] > 0
+    _log("regularized", case, "overflow tiles", st["taubin_overflow_tiles"], "fallback tiles",
+         st["taubin_fallback_tiles"])
+    if case == "crowded":
+        assert st["taubin_overflow_tiles"] > 0 and st["taubin_fallback_tiles"] > 0
