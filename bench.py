@@ -459,8 +459,8 @@ def main():
         t0 = time.perf_counter()
         lvl, ms, cost = g.autotune(args.k)
         tune = {"start_level": int(lvl), "seconds": time.perf_counter() - t0,
-                "trial_ms": [round(float(v), 3) for v in ms if v >= 0],
-                "trial_cost": [round(float(v), 1) for v in cost if v >= 0]}
+                "trial_ms": {str(i - E.SUB_LEVELS): round(float(v), 3) for i, v in enumerate(ms) if v >= 0},
+                "trial_cost": {str(i - E.SUB_LEVELS): round(float(v), 1) for i, v in enumerate(cost) if v >= 0}}
         g.close()
         start_level = int(lvl)
     for _ in range(args.warmup):

@@ -67,10 +67,17 @@ typedef struct {
                                       from global memory) */
 } egt_stats;
 
+/* EGT_OPT_START_LEVEL value: choose the level from the mean cell occupancy. */
+#define EGT_START_LEVEL_OCCUPANCY 127
+/* Entries of the autotuner's trial arrays: start levels -3 .. 10, entry i = level i - 3. */
+#define EGT_AUTOTUNE_SLOTS 14
+
 /* Options set through egt_set_option (no reference counterpart). */
 typedef enum {
     EGT_OPT_START_LEVEL = 0,     /* kNN: pyramid level of the first search box; 0 = the query's own cell, as
-                                    in the reference (default); -1 = from the mean cell occupancy */
+                                    in the reference (default); > 0 coarser cells; -1, -2, -3 = cells of 1/2,
+                                    1/4, 1/8 of a grid cell; EGT_START_LEVEL_OCCUPANCY = one level from the
+                                    mean cell occupancy */
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
@@ -184,8 +191,9 @@ egt_status egt_release_cached_memory(uint64_t* released, uint64_t* cached);
  * (input units, as egt_grid_knn_self) on an evenly spread sample of queries for each candidate start level
  * (the cell size of the first search box), scores each level by its measured work per query (candidates
  * tested + 3 x cells looked up) and sets EGT_OPT_START_LEVEL to the cheapest. Rows do not depend on the
- * choice. trial_ms and trial_cost (optional, 11 floats each) receive each level's sample time in ms and its
- * score, -1 where untried. The handle's neighbour rows are not valid afterwards: run egt_grid_knn_self. */
+ * choice. trial_ms and trial_cost (optional, EGT_AUTOTUNE_SLOTS floats each) receive each level's sample time
+ * in ms and its score at index level + 3, -1 where untried. The handle's neighbour rows are not valid
+ * afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms,
                         float* trial_cost);
 

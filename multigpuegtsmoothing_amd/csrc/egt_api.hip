@@ -303,7 +303,8 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
     switch (opt) {
         case EGT_OPT_START_LEVEL:
-            if (value < -1 || value >= h->nlev) return fail(EGT_ERR_INVALID_ARG, "start level out of range");
+            if (value != EGT_START_LEVEL_OCCUPANCY && (value < -egt::SUB || value >= h->nlev))
+                return fail(EGT_ERR_INVALID_ARG, "start level out of range");
             h->start_level = (int)value;
             return EGT_OK;
         case EGT_OPT_LEARNED:
@@ -342,13 +343,13 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
         float t = thres / (h->max_extent - h->min_extent);
         t *= t;
         if (h->search_stale) egt::refresh_search_structure(h);
-        float ms[EGT_MAX_LEVELS], cost[EGT_MAX_LEVELS];
+        float ms[EGT_AUTOTUNE_SLOTS], cost[EGT_AUTOTUNE_SLOTS];
         const int best = egt::autotune_start_level(h, k, t, ms, cost);
         h->csr_valid = false;
         if (start_level) *start_level = best;
-        for (int l = 0; l < EGT_MAX_LEVELS; ++l) {
-            if (trial_ms) trial_ms[l] = ms[l];
-            if (trial_cost) trial_cost[l] = cost[l];
+        for (int i = 0; i < EGT_AUTOTUNE_SLOTS; ++i) {
+            if (trial_ms) trial_ms[i] = ms[i];
+            if (trial_cost) trial_cost[i] = cost[i];
         }
     });
 }

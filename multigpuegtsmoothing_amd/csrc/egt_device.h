@@ -26,12 +26,33 @@ __host__ __device__ inline uint32_t morton_code(uint32_t x, uint32_t y, uint32_t
     return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
 }
 
+// Sub-cell levels of the kNN search: each fine cell of the reference grid is cut into 2^SUB cells per axis,
+// so search boxes can be smaller than a cell. A point's virtual cell (per axis: fine cell << SUB | its
+// position inside the fine cell) is packed with the Morton code of that position as
+// x | y << 16 | z << 32 | sub code << 48. Inside a fine cell, points are in sub-code order, so every
+// sub-cell at every sub level is a contiguous range.
+constexpr int SUB = 3;
+__host__ __device__ inline uint64_t vcell_pack(uint32_t vx, uint32_t vy, uint32_t vz) {
+    constexpr uint32_t m = (1u << SUB) - 1u;
+    return (uint64_t)vx | ((uint64_t)vy << 16) | ((uint64_t)vz << 32) |
+           ((uint64_t)morton_code(vx & m, vy & m, vz & m) << 48);
+}
+__host__ __device__ inline int vcell_axis(uint64_t v, int ax) { return (int)((v >> (16 * ax)) & 0xFFFFu); }
+__host__ __device__ inline uint32_t vcell_sub(uint64_t v) { return (uint32_t)(v >> 48); }
+
 __device__ inline float sqd(float px, float py, float pz, float qx, float qy, float qz) {
     const float dx = px - qx, dy = py - qy, dz = pz - qz;
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
 __device__ inline int cell_of(float v, float a, float ba, int B) { return (int)(((v - a) / ba) * (float)B); }
+// Virtual coordinate (resolution 2^SUB B) of normalized coordinate v whose fine cell is c. Scaling by 2^SUB is
+// exact, so for a point whose cell is cell_of(v) the result lies inside cell c; the clamp keeps it there for
+// cells assigned otherwise (queries clamped to the grid).
+__device__ inline uint32_t vcoord(float v, float a, float ba, int B, int c) {
+    const int t = (int)(((v - a) / ba) * (float)(B << SUB));
+    return (uint32_t)min(max(t, c << SUB), (c << SUB) + (1 << SUB) - 1);
+}
 
 // exp(x) near 0: for |x| <= 2^-6 the cubic Taylor polynomial in Horner form (truncation error
 // x^4/24 <= 2.5e-9, a small fraction of an ulp of the result).

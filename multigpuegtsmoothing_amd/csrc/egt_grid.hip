@@ -185,20 +185,36 @@ __global__ void __launch_bounds__(BLK) k_gather_sorted(const float* __restrict__
     z[s] = nz[o];
 }
 
-__global__ void __launch_bounds__(BLK) k_morton_keys(const int32_t* __restrict__ key, int64_t n, int log2B,
-                                                      uint32_t* __restrict__ code, uint32_t* __restrict__ idx) {
+// Sort keys of the Morton order: the fine cell's Morton code and the Morton code of the point's position
+// inside the cell (sorted first, stably, so that points of a cell end up in sub-code order).
+__global__ void __launch_bounds__(BLK) k_morton_keys(const float* __restrict__ x, const float* __restrict__ y,
+                                                      const float* __restrict__ z, const int32_t* __restrict__ key,
+                                                      int64_t n, float a, float ba, int B, int log2B,
+                                                      uint32_t* __restrict__ code, uint32_t* __restrict__ sub,
+                                                      uint32_t* __restrict__ idx) {
     const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (s >= n) return;
     const uint32_t k = (uint32_t)key[s], m = (1u << log2B) - 1u;
-    code[s] = morton_code(k & m, (k >> log2B) & m, k >> (2 * log2B));
+    const uint32_t cx = k & m, cy = (k >> log2B) & m, cz = k >> (2 * log2B);
+    code[s] = morton_code(cx, cy, cz);
+    sub[s] = vcell_sub(vcell_pack(vcoord(x[s], a, ba, B, (int)cx), vcoord(y[s], a, ba, B, (int)cy),
+                                  vcoord(z[s], a, ba, B, (int)cz)));
     idx[s] = (uint32_t)s;
+}
+
+__global__ void __launch_bounds__(BLK) k_gather_u32(const uint32_t* __restrict__ src,
+                                                     const uint32_t* __restrict__ idx, int64_t n,
+                                                     uint32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
 }
 
 __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ z, const int32_t* __restrict__ key,
                                                         const int32_t* __restrict__ scloud,
-                                                        const uint32_t* __restrict__ order, int64_t n, int log2B,
-                                                        float4* __restrict__ mpts, uint32_t* __restrict__ mcell,
+                                                        const uint32_t* __restrict__ order, int64_t n, float a,
+                                                        float ba, int B, int log2B, float4* __restrict__ mpts,
+                                                        uint32_t* __restrict__ mcell, unsigned long long* __restrict__ mvcell,
                                                         uint32_t* __restrict__ mcode, int32_t* __restrict__ mcloud,
                                                         int32_t* __restrict__ morder, int32_t* __restrict__ minv) {
     const int64_t m = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -206,10 +222,12 @@ __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__
     const uint32_t s = order[m];
     morder[m] = (int32_t)s;
     minv[s] = (int32_t)m;
-    mpts[m] = make_float4(x[s], y[s], z[s], __int_as_float((int)s));
+    const float px = x[s], py = y[s], pz = z[s];
+    mpts[m] = make_float4(px, py, pz, __int_as_float((int)s));
     const uint32_t k = (uint32_t)key[s], msk = (1u << log2B) - 1u;
     const uint32_t cx = k & msk, cy = (k >> log2B) & msk, cz = k >> (2 * log2B);
     mcell[m] = cx | (cy << 10) | (cz << 20);
+    mvcell[m] = vcell_pack(vcoord(px, a, ba, B, (int)cx), vcoord(py, a, ba, B, (int)cy), vcoord(pz, a, ba, B, (int)cz));
     mcode[m] = morton_code(cx, cy, cz);
     if (mcloud) mcloud[m] = scloud[s];
 }
@@ -498,8 +516,19 @@ void build_search_structure(egt_ctx* h) {
     DBuf<uint32_t>& order = sc.bufs[13];
     code.alloc(n);
     order.alloc(n);
-    k_morton_keys<<<ceil_div(n, BLK), BLK, 0, s>>>(h->key.p, n, log2B, code.p, order.p);
-    LAUNCH_CHECK();
+    const float a = -h->eps, ba = (1.0f + h->eps) - a;
+    {
+        // Morton order of the fine cells, points of a cell in sub-code order: stable sort by sub code, then
+        // stable sort by cell code
+        uint32_t* sub = sc.get<uint32_t>(22, n);
+        uint32_t* cell = sc.get<uint32_t>(8, n);
+        k_morton_keys<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, n, a, ba, h->B, log2B,
+                                                       cell, sub, order.p);
+        LAUNCH_CHECK();
+        radix_sort_pairs(sub, order.p, n, 3 * SUB, s, sc);
+        k_gather_u32<<<ceil_div(n, BLK), BLK, 0, s>>>(cell, order.p, n, code.p);
+        LAUNCH_CHECK();
+    }
     radix_sort_pairs(code.p, order.p, n, 3 * log2B, s, sc);
     if (batch) {
         // second stable pass by cloud: Morton order within each cloud, clouds in input order
@@ -511,11 +540,13 @@ void build_search_structure(egt_ctx* h) {
     }
     h->mpts.alloc(n);
     h->mcell.alloc(n);
+    h->mvcell.alloc(n);
     h->mcode.alloc(n);
     h->morder.alloc(n);
     h->minv.alloc(n);
     k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, batch ? h->scloud.p : nullptr,
-                                                      order.p, n, log2B, h->mpts.p, h->mcell.p, h->mcode.p,
+                                                      order.p, n, a, ba, h->B, log2B, h->mpts.p, h->mcell.p,
+                                                      reinterpret_cast<unsigned long long*>(h->mvcell.p), h->mcode.p,
                                                       batch ? h->mcloud.p : nullptr, h->morder.p, h->minv.p);
     LAUNCH_CHECK();
     const int32_t* mcl = batch ? h->mcloud.p : nullptr;

@@ -8,6 +8,7 @@
 #include "egt_device.h"
 
 #define EGT_MAX_LEVELS 11
+static_assert(EGT_AUTOTUNE_SLOTS == EGT_MAX_LEVELS + egt::SUB, "autotuner slots: sub-cell levels + pyramid levels");
 
 namespace egt {
 
@@ -124,6 +125,7 @@ struct egt_ctx {
     // search structure (Morton order)
     egt::DBuf<float4> mpts;      // x, y, z, bits(sorted index)
     egt::DBuf<uint32_t> mcell;   // packed fine cell coords cx | cy<<10 | cz<<20
+    egt::DBuf<uint64_t> mvcell;  // virtual cell and sub code (vcell_pack)
     egt::DBuf<uint32_t> mcode;   // fine Morton code
     egt::DBuf<int32_t> morder;   // Morton position -> sorted index
     egt::DBuf<int32_t> minv;     // sorted index -> Morton position
@@ -160,7 +162,7 @@ struct egt_ctx {
 
     // query-vs-reference search (its rows live beside the self rows, which smoothing keeps using)
     egt::DBuf<float4> qpts;       // queries in Morton order of their cell: x', y', z', bits(original index)
-    egt::DBuf<uint32_t> qcell;    // packed fine cell, clamped to the grid
+    egt::DBuf<uint64_t> qvcell;   // virtual cell (vcell_pack), clamped to the grid
     egt::DBuf<int32_t> qknn;      // nq*kq rows in original query order, reference sorted indices
     egt::DBuf<egt::CloudDev> dqcloud;
     int64_t nq = 0;
@@ -168,7 +170,8 @@ struct egt_ctx {
     bool have_query = false;
 
     // options
-    int start_level = 0;  // reference schedule: every query starts at its own level-0 cell
+    int start_level = 0;  // reference schedule: every query starts at its own level-0 cell; -SUB..-1 sub-cell
+                          // boxes; EGT_START_LEVEL_OCCUPANCY = from the mean cell occupancy
     int learned = 0;
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;

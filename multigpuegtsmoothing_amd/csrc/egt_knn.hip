@@ -7,7 +7,10 @@
 // One thread per query, queries walked in Morton order so a wave's 64 queries are spatial neighbours and
 // share cache lines of candidate tiles. The search box at pyramid level l is the 3x3x3 block of level-l
 // cells around the query's cell (cells of level l are contiguous ranges of the Morton-ordered point array,
-// found through a per-level hash). Each level scans only cells and points outside the previous box, keeps
+// found through a per-level hash). Below the reference's fine cells the pyramid has SUB sub-cell levels
+// (cells of 1/2, 1/4, 1/8 of a fine cell), whose ranges are found by binary search of the points' sub codes
+// inside their fine cell's range; the search runs on internal levels il = l + SUB. Each level scans only
+// cells and points outside the previous box, keeps
 // a register-resident sorted top-K of 64-bit keys (distance bits << 32 | sorted index) and stops when the
 // k-th distance plus a 2e-6 margin (double) is below every interior face distance, so rows are the exact
 // k nearest neighbours ordered by (distance, sorted index). The reference's float certificate is then
@@ -36,7 +39,8 @@ constexpr int knn_fetch() { return K <= 16 ? 2 : 4; }
 
 struct KnnParams {
     const float4* __restrict__ mpts;
-    const uint32_t* __restrict__ mcell;
+    const uint32_t* __restrict__ mcell;    // fine cell cx | cy << 10 | cz << 20 of each Morton position
+    const unsigned long long* __restrict__ mvcell;  // virtual cell + sub code of each Morton position
     const uint64_t* __restrict__ hkeys;
     const uint2* __restrict__ hvals;
     const int32_t* __restrict__ key;       // sorted-space fine key (member cell coordinates)
@@ -52,10 +56,13 @@ struct KnnParams {
                                                 // [4] cells looked up
     const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
     const float4* __restrict__ qpts;       // query search: queries in Morton order (x', y', z', bits(index))
-    const uint32_t* __restrict__ qcell;    // query search: packed fine cell of each query, clamped to the grid
+    const unsigned long long* __restrict__ qvcell;  // query search: virtual cell, clamped to the grid
     int64_t nchunks;   // chunks in the work list (after sampling)
     int stride;        // 1 = every chunk; s > 1 = every s-th chunk of the list (sampled trial runs)
-    int B, log2B, nlev, start_level, stages, maxc;
+    int B, log2B, stages, maxc;
+    int Bv;            // cells per axis at internal level 0 (B << SUB)
+    int nlev;          // internal levels: SUB sub-cell levels + the reference pyramid
+    int start_level;   // internal level of the first box (SUB = the query's own fine cell)
     uint32_t bigmask;  // levels holding a cell with >= maxc points (any cloud)
     float a, ba;
 };
@@ -109,6 +116,43 @@ __device__ inline uint32_t cell_count(const KnnParams& P, uint32_t cloud, int lv
     return level_lookup(P.hkeys, P.hvals, P.lev[lvl], cell_key(cloud, morton_code(cx, cy, cz))).y;
 }
 
+// Range of the cell c (coordinates at internal level il < SUB) inside the range [lo, lo + cnt) of an enclosing
+// cell: the points whose sub code, shifted down by 3 il, equals the Morton code of c's position inside its
+// fine cell. Two binary searches; probes counts their steps.
+__device__ inline uint2 sub_range(const KnnParams& P, uint32_t lo, uint32_t cnt, int il, const int (&c)[3],
+                                  uint32_t& probes) {
+    const uint32_t m = (1u << (SUB - il)) - 1u;
+    const uint32_t want = morton_code((uint32_t)c[0] & m, (uint32_t)c[1] & m, (uint32_t)c[2] & m);
+    const int sh = 3 * il;
+    uint32_t a = lo, b = lo + cnt;  // first position with prefix >= want
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        ++probes;
+        if ((vcell_sub(P.mvcell[mid]) >> sh) < want) a = mid + 1;
+        else b = mid;
+    }
+    uint32_t e = lo + cnt;  // first position with prefix > want
+    b = a;
+    while (b < e) {
+        const uint32_t mid = (b + e) >> 1;
+        ++probes;
+        if ((vcell_sub(P.mvcell[mid]) >> sh) <= want) b = mid + 1;
+        else e = mid;
+    }
+    return make_uint2(a, b - a);
+}
+
+// Range of the cell c at internal level il: the level table for il >= SUB, else the fine cell's range (level 0
+// table) narrowed by sub_range.
+__device__ inline uint2 cell_range(const KnnParams& P, uint32_t cloud, int il, const int (&c)[3], uint32_t& probes) {
+    if (il >= SUB)
+        return level_lookup(P.hkeys, P.hvals, P.lev[il - SUB], cell_key(cloud, morton_code(c[0], c[1], c[2])));
+    const int sh = SUB - il;
+    const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[0], cell_key(cloud, morton_code(c[0] >> sh, c[1] >> sh, c[2] >> sh)));
+    if (r.y == 0) return r;
+    return sub_range(P, r.x, r.y, il, c, probes);
+}
+
 // The 27 cells of a 3x3x3 block (t = dx + 1 + 3 (dy + 1) + 9 (dz + 1)) nearest first: centre, faces, edges,
 // corners, so the bound tightens early and the farther cells can be pruned.
 __constant__ int8_t c_shell_order[27] = {13, 4, 10, 12, 14, 16, 22, 1, 3, 5, 7, 9, 11, 15, 17,
@@ -128,7 +172,7 @@ __device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
 // construct_neighbors :340-440), which differs in four places: no point is excluded, candidates need
 // d < thres (strict), the reference computes a query's cell at every level from its coordinates (it can lie
 // outside the grid, and (int) truncates toward zero) and its short-list certificate uses >=. Queries come
-// from qpts / qcell (Morton order of their cell clamped to the grid); rows go to knn at the query's original
+// from qpts / qvcell (Morton order of their cell clamped to the grid); rows go to knn at the query's original
 // index. A query outside the grid skips the early exit below: its search boxes (around the clamped cell) are
 // not the reference's stage boxes, so it searches until the list is exact.
 template <int K, bool PERSIST, bool QUERY = false>
@@ -150,7 +194,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
     int own_level = P.start_level;
     int wave_level = own_level;
     if (P.qlevel) {
-        own_level = active ? (int)P.qlevel[m] : 127;
+        own_level = active ? (int)P.qlevel[m] + SUB : 127;
         wave_level = own_level;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) wave_level = min(wave_level, __shfl_xor(wave_level, d, 64));
@@ -159,8 +203,8 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
     if (active) {
         const float4 q = QUERY ? P.qpts[m] : P.mpts[m];
         const int32_t jq = __float_as_int(q.w);
-        const uint32_t qc = QUERY ? P.qcell[m] : P.mcell[m];
-        const int fx = qc & 1023, fy = (qc >> 10) & 1023, fz = qc >> 20;
+        const uint64_t qvc = QUERY ? P.qvcell[m] : P.mvcell[m];
+        const int vq[3] = {vcell_axis(qvc, 0), vcell_axis(qvc, 1), vcell_axis(qvc, 2)};
         bool outside = false;  // query search: the query's own (unclamped) cell lies outside the grid
         if (QUERY) {
             const int ux = cell_of(q.x, P.a, P.ba, P.B), uy = cell_of(q.y, P.a, P.ba, P.B),
@@ -192,22 +236,24 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
         level = level < 0 ? 0 : (level >= P.nlev ? P.nlev - 1 : level);
         const int first_level = level;
         uint32_t fallback_q = 0;
-        int plo[3] = {P.B, P.B, P.B}, phi[3] = {-1, -1, -1};  // previous box (fine coords), empty
+        int plo[3] = {P.Bv, P.Bv, P.Bv}, phi[3] = {-1, -1, -1};  // previous box (virtual coords), empty
         for (;;) {
-            const int Bl = P.B >> level;
-            const int cq[3] = {fx >> level, fy >> level, fz >> level};
-            // Early exit (exact): the box of level l is what the reference's stage l + 1 scans. Once the box
-            // of level l - 1 is scanned, the list answers every earlier stage exactly as the reference's own
-            // list would. If stage l + 1 does not exist (l >= stages) or would meet a cell of >= maxc points
-            // (the reference then invalidates the query), nothing beyond the scanned box can change the
-            // outcome: the replay below either stops at an earlier stage with this list or blanks the row.
-            if (level > first_level && !outside) {
-                if (level >= P.stages) break;
-                if ((P.bigmask >> level) & 1u) {
+            const int Bl = P.Bv >> level;
+            const int cq[3] = {vq[0] >> level, vq[1] >> level, vq[2] >> level};
+            // Early exit (exact): the box of reference level l = level - SUB is what the reference's stage
+            // l + 1 scans. Once the box of level l - 1 is scanned, the list answers every earlier stage
+            // exactly as the reference's own list would. If stage l + 1 does not exist (l >= stages) or would
+            // meet a cell of >= maxc points (the reference then invalidates the query), nothing beyond the
+            // scanned box can change the outcome: the replay below either stops at an earlier stage with this
+            // list or blanks the row.
+            if (level > first_level && level > SUB && !outside) {
+                const int rl = level - SUB;
+                if (rl >= P.stages) break;
+                if ((P.bigmask >> rl) & 1u) {
                     bool big = false;
                     for (int t = 0; t < 27 && !big; ++t) {
                         if (t == 13) continue;
-                        big = cell_count(P, cloud, level, cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1,
+                        big = cell_count(P, cloud, rl, cq[0] + (t % 3) - 1, cq[1] + ((t / 3) % 3) - 1,
                                          cq[2] + (t / 9) - 1) >= (uint32_t)P.maxc;
                     }
                     if (big) break;
@@ -215,9 +261,10 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
             }
             const float qv[3] = {q.x, q.y, q.z};
             // Offers the points of Morton range [start, pend) (skipping those inside the previous box when the
-            // range's cell overlaps it). Candidates are fetched CH at a time before any is used (one memory
-            // latency per chunk); the top-K under the (distance, index) order does not depend on the offering
-            // order.
+            // range's cell overlaps it; that only happens above the fine level, where the previous box is made of
+            // whole fine cells, so the point's fine cell decides). Candidates are fetched CH at a time before any
+            // is used (one memory latency per chunk); the top-K under the (distance, index) order does not depend
+            // on the offering order.
             auto scan = [&](uint32_t start, uint32_t pend, bool overlap) {
                 constexpr int CH = knn_fetch<K>();
                 for (uint32_t p0 = start; p0 < pend; p0 += CH) {
@@ -235,8 +282,9 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                         const float4 e = ev[u];
                         const int32_t j = __float_as_int(e.w);
                         if (!QUERY && j == jq) continue;
-                        if (overlap) {
-                            const int ex = cc[u] & 1023, ey = (cc[u] >> 10) & 1023, ez = cc[u] >> 20;
+                        if (overlap) {  // fine cell scaled to virtual coordinates (box bounds are multiples of 2^SUB)
+                            const int ex = (int)(cc[u] & 1023u) << SUB, ey = (int)((cc[u] >> 10) & 1023u) << SUB,
+                                      ez = (int)(cc[u] >> 20) << SUB;
                             if (ex >= plo[0] && ex <= phi[0] && ey >= plo[1] && ey <= phi[1] && ez >= plo[2] &&
                                 ez <= phi[2])
                                 continue;
@@ -269,7 +317,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                     overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
                 }
                 if (inside) return 0;
-                const float hc = P.ba / (float)(P.B >> lv);
+                const float hc = P.ba / (float)(P.Bv >> lv);
                 const float bnd = L[K - 1] != KEY_EMPTY ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
                 float dd = 0.0f;
 #pragma unroll
@@ -288,15 +336,16 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                 const int ct = cell_test(level, c3);
                 if (!ct) continue;
                 ++cells;
-                const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[level],
-                                             cell_key(cloud, morton_code(c3[0], c3[1], c3[2])));
+                const uint2 r = cell_range(P, cloud, level, c3, cells);
                 // A populous cell is tested and scanned child by child, nearest child first (the child on the
                 // query's side along each axis, then across one, two and three mid-planes), so the list fills
-                // with near points early and the farther children can be pruned.
-                const bool split = level > 0 && r.y > SPLIT_COUNT;
+                // with near points early and the farther children can be pruned. At or below the fine level a
+                // cell that overlaps the previous box (one level finer) is split too: each child then lies wholly
+                // inside the previous box (skipped) or wholly outside it, so no point needs the box filter.
+                const bool split = level > 0 && (r.y > SPLIT_COUNT || (ct == 2 && level <= SUB));
                 int near = 0;
                 if (split) {
-                    const float hc = P.ba / (float)(P.B >> (level - 1));
+                    const float hc = P.ba / (float)(P.Bv >> (level - 1));
 #pragma unroll
                     for (int ax = 0; ax < 3; ++ax)
                         near |= (qv[ax] >= P.a + (float)(2 * c3[ax] + 1) * hc ? 1 : 0) << ax;
@@ -310,8 +359,11 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
                         const int cct = cell_test(level - 1, cc3);
                         if (!cct) continue;
                         ++cells;
-                        const uint2 rc = level_lookup(P.hkeys, P.hvals, P.lev[level - 1],
-                                                      cell_key(cloud, morton_code(cc3[0], cc3[1], cc3[2])));
+                        const uint2 rc =
+                            level - 1 >= SUB
+                                ? level_lookup(P.hkeys, P.hvals, P.lev[level - 1 - SUB],
+                                               cell_key(cloud, morton_code(cc3[0], cc3[1], cc3[2])))
+                                : sub_range(P, r.x, r.y, level - 1, cc3, cells);
                         start = rc.x;
                         count = rc.y;
                         ov = cct == 2;
@@ -338,7 +390,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 plo[ax] = max(0, (cq[ax] - 1) << level);
-                phi[ax] = min(P.B - 1, ((cq[ax] + 2) << level) - 1);
+                phi[ax] = min(P.Bv - 1, ((cq[ax] + 2) << level) - 1);
             }
             ++level;
         }
@@ -363,7 +415,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
         for (int s = 0; s <= P.stages; ++s) {
             const int lvl = s == 0 ? 0 : s - 1;
             const int off = s == 0 ? 0 : 1;
-            int cx = fx >> lvl, cy = fy >> lvl, cz = fz >> lvl;
+            int cx = vq[0] >> (SUB + lvl), cy = vq[1] >> (SUB + lvl), cz = vq[2] >> (SUB + lvl);
             if (QUERY) {  // the reference's cell of the query at this level (GridKNNCuda.cu:351-353)
                 cx = cell_of(q.x, P.a, P.ba, P.B >> lvl);
                 cy = cell_of(q.y, P.a, P.ba, P.B >> lvl);
@@ -426,13 +478,13 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 
 }
 
 // Query preparation: each query normalized with the reference cloud's extent (normalize_points
-// GridKNNCuda.cu:841-849: x' = (x - min) / (max - min)), its fine cell clamped to the grid (the search boxes
-// start there) and the Morton code of that cell as the sort key, so a wave's 64 queries are spatial
-// neighbours.
+// GridKNNCuda.cu:841-849: x' = (x - min) / (max - min)), its fine cell clamped to the grid and its virtual
+// cell inside that (the search boxes start there), and the Morton code of the fine cell as the sort key, so
+// a wave's 64 queries are spatial neighbours.
 __global__ void __launch_bounds__(KBLK) k_query_prep(const float* __restrict__ qx, const float* __restrict__ qy,
                                                       const float* __restrict__ qz, int64_t nq, float mn, float mx,
                                                       float a, float ba, int B, float4* __restrict__ qn,
-                                                      uint32_t* __restrict__ cellp, uint32_t* __restrict__ code,
+                                                      uint64_t* __restrict__ cellp, uint32_t* __restrict__ code,
                                                       uint32_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x;
     if (i >= nq) return;
@@ -442,14 +494,14 @@ __global__ void __launch_bounds__(KBLK) k_query_prep(const float* __restrict__ q
     const int cy = min(max(cell_of(y, a, ba, B), 0), B - 1);
     const int cz = min(max(cell_of(z, a, ba, B), 0), B - 1);
     qn[i] = make_float4(x, y, z, 0.0f);
-    cellp[i] = (uint32_t)cx | ((uint32_t)cy << 10) | ((uint32_t)cz << 20);
+    cellp[i] = vcell_pack(vcoord(x, a, ba, B, cx), vcoord(y, a, ba, B, cy), vcoord(z, a, ba, B, cz));
     code[i] = morton_code(cx, cy, cz);
     idx[i] = (uint32_t)i;
 }
 
-__global__ void __launch_bounds__(KBLK) k_query_gather(const float4* __restrict__ qn, const uint32_t* __restrict__ cellp,
+__global__ void __launch_bounds__(KBLK) k_query_gather(const float4* __restrict__ qn, const uint64_t* __restrict__ cellp,
                                                         const uint32_t* __restrict__ idx, int64_t nq,
-                                                        float4* __restrict__ qpts, uint32_t* __restrict__ qcell) {
+                                                        float4* __restrict__ qpts, uint64_t* __restrict__ qcell) {
     const int64_t m = (int64_t)blockIdx.x * KBLK + threadIdx.x;
     if (m >= nq) return;
     const uint32_t i = idx[m];
@@ -517,6 +569,7 @@ KnnParams knn_params(egt_ctx* h) {
     KnnParams P{};
     P.mpts = h->mpts.p;
     P.mcell = h->mcell.p;
+    P.mvcell = reinterpret_cast<const unsigned long long*>(h->mvcell.p);
     P.hkeys = h->hkeys.p;
     P.hvals = h->hvals.p;
     P.key = h->key.p;
@@ -535,9 +588,10 @@ KnnParams knn_params(egt_ctx* h) {
     int log2B = 0;
     while ((1 << log2B) < h->B) ++log2B;
     P.log2B = log2B;
-    P.nlev = h->nlev;
-    P.start_level = h->start_level >= 0 ? std::min(h->start_level, h->nlev - 1)
-                                        : auto_start_level(h, h->clouds[0].k);
+    P.Bv = h->B << SUB;
+    P.nlev = h->nlev + SUB;
+    P.start_level = SUB + (h->start_level == EGT_START_LEVEL_OCCUPANCY ? auto_start_level(h, h->clouds[0].k)
+                                                                         : std::min(h->start_level, h->nlev - 1));
     int stages = 0;
     for (int b = h->B; b >= 8; b >>= 1) ++stages;
     P.stages = std::max(stages, 1);
@@ -651,7 +705,7 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     h->kq = k;
     h->qknn.alloc((size_t)std::max<int64_t>(nq * k, 1));
     h->qpts.alloc((size_t)std::max<int64_t>(nq, 1));
-    h->qcell.alloc((size_t)std::max<int64_t>(nq, 1));
+    h->qvcell.alloc((size_t)std::max<int64_t>(nq, 1));
     KnnParams P = knn_params(h);
     HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
     hipEvent_t e0, e1;
@@ -666,7 +720,7 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     qcd.thres_n = thres_n;
     if (nq > 0) {
         float4* qn = h->scratch.get<float4>(32, nq);
-        uint32_t* cellp = h->scratch.get<uint32_t>(33, nq);
+        uint64_t* cellp = h->scratch.get<uint64_t>(33, nq);
         uint32_t* code = h->scratch.get<uint32_t>(34, nq);
         uint32_t* idx = h->scratch.get<uint32_t>(35, nq);
         const int g = ceil_div(nq, KBLK);
@@ -674,12 +728,12 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
                                         code, idx);
         LAUNCH_CHECK();
         radix_sort_pairs(code, idx, nq, 30, s, h->scratch);
-        k_query_gather<<<g, KBLK, 0, s>>>(qn, cellp, idx, nq, h->qpts.p, h->qcell.p);
+        k_query_gather<<<g, KBLK, 0, s>>>(qn, cellp, idx, nq, h->qpts.p, h->qvcell.p);
         LAUNCH_CHECK();
         h->dqcloud.alloc(1);
         HIP_CHECK(hipMemcpyAsync(h->dqcloud.p, &qcd, sizeof(CloudDev), hipMemcpyHostToDevice, s));
         P.qpts = h->qpts.p;
-        P.qcell = h->qcell.p;
+        P.qvcell = reinterpret_cast<const unsigned long long*>(h->qvcell.p);
         P.knn = h->qknn.p;
         P.kth = nullptr;
         P.clouds = h->dqcloud.p;
@@ -720,9 +774,11 @@ void run_knn_query_original(egt_ctx* h, int32_t* out) {
 // it runs the plain search on an evenly spread sample of chunks (about 8192 chunks of 64 Morton-consecutive
 // queries) and reads back the work counters. The score per sampled query is candidates + 3 x cells looked up
 // (a cell lookup costs about three candidate tests on gfx950: a dependent hash probe and range fetch against a
-// coalesced distance test). Counters, not the sample's wall time, rank the levels: a sample's time is set by
-// its slowest wave (isolated queries next to dense clusters), not by the bulk of the work. Rows do not depend
-// on the choice. trial_ms / trial_cost get each level's sample time and score (-1 for untried levels).
+// coalesced distance test; cells looked up include the binary-search steps of sub-cell ranges). Counters,
+// not the sample's wall time, rank the levels: a sample's time is set by its slowest wave (isolated queries
+// next to dense clusters), not by the bulk of the work. Rows do not depend on the choice. Levels -SUB .. top
+// are tried; trial_ms / trial_cost get each level's sample time and score at index level + SUB (-1 for
+// untried levels).
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost) {
     hipStream_t s = h->stream;
     h->clouds[0].k = k;
@@ -737,14 +793,14 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     P.nchunks = (total + P.stride - 1) / P.stride;
     const int K = kclass(k);
     const int top = std::min(h->nlev - 1, std::max(3, auto_start_level(h, k) + 1));
-    for (int l = 0; l < EGT_MAX_LEVELS; ++l) trial_ms[l] = trial_cost[l] = -1.0f;
+    for (int i = 0; i < EGT_AUTOTUNE_SLOTS; ++i) trial_ms[i] = trial_cost[i] = -1.0f;
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
     HIP_CHECK(hipEventCreate(&e1));
     int best = 0;
     double best_cost = 0.0;
-    for (int l = 0; l <= top; ++l) {
-        P.start_level = l;
+    for (int l = -SUB; l <= top; ++l) {
+        P.start_level = l + SUB;
         HIP_CHECK(hipMemsetAsync(P.counters, 0, 5 * sizeof(uint64_t), s));
         HIP_CHECK(hipEventRecord(e0, s));
         launch_any(P, K, s);
@@ -756,9 +812,9 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
         const double q = (double)std::min<int64_t>(h->n, P.nchunks * 64);
         const double cost = ((double)hc[0] + 3.0 * (double)hc[4]) / q;
-        trial_ms[l] = ms;
-        trial_cost[l] = (float)cost;
-        if (l == 0 || cost < best_cost) {
+        trial_ms[l + SUB] = ms;
+        trial_cost[l + SUB] = (float)cost;
+        if (l == -SUB || cost < best_cost) {
             best = l;
             best_cost = cost;
         }

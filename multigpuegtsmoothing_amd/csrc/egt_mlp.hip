@@ -344,13 +344,14 @@ __global__ void __launch_bounds__(MBLK) k_mlp_adam(float* __restrict__ P, float*
 
 struct FeatParams {
     const float4* mpts;
-    const uint32_t* mcell;
+    const unsigned long long* mvcell;  // virtual cell of each Morton position (fine cell = coordinate >> SUB)
     const uint64_t* hkeys;
     const uint2* hvals;
     const LevelDev* lev;     // level descriptors (device)
     const CloudDev* clouds;  // per cloud: reference level, neighbour count
     const int32_t* mcloud;   // cloud of each Morton position, null for one cloud
     int64_t n;
+    int B;  // fine cells per axis
 };
 
 // Feature t of the query at Morton position m (its cloud's reference level and k).
@@ -360,9 +361,10 @@ __device__ inline float feature(const FeatParams& F, int64_t m, int t) {
     const CloudDev& cd = F.clouds[cloud];
     if (t == NF - 1) return log2f((float)cd.k) * (1.0f / 8.0f);
     const int ref = cd.ref_level;
-    const uint32_t qc = F.mcell[m];
-    const int Bl = 1024 >> ref;
-    const int cx = (int)(qc & 1023u) >> ref, cy = (int)((qc >> 10) & 1023u) >> ref, cz = (int)(qc >> 20) >> ref;
+    const uint64_t qc = F.mvcell[m];
+    const int Bl = F.B >> ref;
+    const int cx = vcell_axis(qc, 0) >> (SUB + ref), cy = vcell_axis(qc, 1) >> (SUB + ref),
+              cz = vcell_axis(qc, 2) >> (SUB + ref);
     const int x = cx + (t % 3) - 1, y = cy + ((t / 3) % 3) - 1, z = cz + (t / 9) - 1;
     uint32_t c = 0;
     if (x >= 0 && y >= 0 && z >= 0 && x < Bl && y < Bl && z < Bl)
@@ -385,20 +387,21 @@ __global__ void __launch_bounds__(256) k_features(FeatParams F, int by_sorted, f
 // invalid or r_k == 0.
 __global__ void __launch_bounds__(256) k_targets(const float4* __restrict__ mpts, const float* __restrict__ kth,
                                                   const CloudDev* __restrict__ clouds,
-                                                  const int32_t* __restrict__ mcloud, int64_t n, float ba,
+                                                  const int32_t* __restrict__ mcloud, int64_t n, float ba, int B,
                                                   float* __restrict__ target, uint32_t* __restrict__ ok) {
     const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (m >= n) return;
     const float d = kth[__float_as_int(mpts[m].w)];
     const bool good = d > 0.0f;
-    const float inv_href = (float)(1024 >> clouds[mcloud ? mcloud[m] : 0].ref_level) / ba;
+    const float inv_href = (float)(B >> clouds[mcloud ? mcloud[m] : 0].ref_level) / ba;
     ok[m] = good ? 1u : 0u;
     target[m] = good ? log2f(sqrtf(d) * inv_href) : 0.0f;
 }
 
 // Proposal: features -> MLP -> per-query start level, fused (no feature matrix in HBM). The level is the
-// smallest whose 3x3x3 block around the query reaches beyond r = h_ref * 2^(y + safety) on every side
-// (reach along an axis = h_l * (1 + distance of the query to the nearer face of its cell in cell units)).
+// smallest (from the finest sub-cell level -SUB up) whose 3x3x3 block around the query reaches beyond
+// r = h_ref * 2^(y + safety) on every side (reach along an axis = h_l * (1 + distance of the query to the
+// nearer face of its cell in cell units)).
 __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ P, FeatParams F, float a, float ba,
                                                        int nlev, float safety, int8_t* __restrict__ qlevel) {
     __shared__ WaveLds lds[WPB];
@@ -418,13 +421,13 @@ __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ 
         tile_forward(L, P, o);
         const int64_t m = first + l;
         if (l < TS && m < F.n) {
-            const float href = ba / (float)(1024 >> F.clouds[F.mcloud ? F.mcloud[m] : 0].ref_level);
+            const float href = ba / (float)(F.B >> F.clouds[F.mcloud ? F.mcloud[m] : 0].ref_level);
             const float r = href * exp2f(L.y[l] + safety);
             const float4 p = F.mpts[m];
             const float v[3] = {p.x, p.y, p.z};
             int lvl = nlev - 1;
-            for (int lv = 0; lv < nlev; ++lv) {
-                const float h = ba / (float)(1024 >> lv);
+            for (int lv = -SUB; lv < nlev; ++lv) {
+                const float h = ba / (float)((F.B << SUB) >> (lv + SUB));
                 float reach = 3.0f * h;
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
@@ -446,7 +449,8 @@ __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ 
 FeatParams feat_params(const egt_ctx* h) {
     FeatParams F{};
     F.mpts = h->mpts.p;
-    F.mcell = h->mcell.p;
+    F.mvcell = reinterpret_cast<const unsigned long long*>(h->mvcell.p);
+    F.B = h->B;
     F.hkeys = h->hkeys.p;
     F.hvals = h->hvals.p;
     F.lev = h->dlev.p;
@@ -595,7 +599,7 @@ void mlp_features(egt_ctx* h, int k, float* Xh, float* Th, int* ref_level_out) {
             for (int f = 0; f < NF; ++f) Xh[(size_t)i * NF + f] = buf[(size_t)i * NIN + f];
     if (Th) {
         // targets of the handle's last search with this k, sorted order; NaN where there is none
-        const float inv_href = (float)(1024 >> ref) / ((1.0f + h->eps) + h->eps);
+        const float inv_href = (float)(h->B >> ref) / ((1.0f + h->eps) + h->eps);
         if (h->have_knn && h->k == k && !h->smoothed) {
             std::vector<float> kth(n);
             HIP_CHECK(hipMemcpyAsync(kth.data(), h->scratch.get<float>(15, n), n * sizeof(float),
@@ -633,7 +637,7 @@ void mlp_train(egt_model_s* m, egt_ctx* h, const int* ks, int epochs, int batch,
     k_features<<<ceil_div(n * NIN, 256), 256, 0, s>>>(feat_params(h), 0, Xall);
     LAUNCH_CHECK();
     k_targets<<<ceil_div(n, 256), 256, 0, s>>>(h->mpts.p, h->scratch.get<float>(15, n), h->dclouds.p,
-                                               h->nclouds > 1 ? h->mcloud.p : nullptr, n, ba, Tall, ok);
+                                               h->nclouds > 1 ? h->mcloud.p : nullptr, n, ba, h->B, Tall, ok);
     LAUNCH_CHECK();
     std::vector<uint32_t> okh(n);
     HIP_CHECK(hipMemcpyAsync(okh.data(), ok, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

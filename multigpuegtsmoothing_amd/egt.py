@@ -26,6 +26,9 @@ EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
 OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT, OPT_TAUBIN_TILE = 0, 1, 2, 3, 4, 5
 OPT_TAUBIN_REGULARIZED = 6
+START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
+SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
+AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
 
 
 class EGTError(RuntimeError):
@@ -276,11 +279,11 @@ class GridStructure(_Provider):
 
     def autotune(self, k, thres=1e8):
         """Picks the start level with the least measured work per query for this cloud and k, on a sample of
-        queries (no reference counterpart). Returns (level, per-level sample times in ms, per-level scores),
-        -1 where untried."""
+        queries (no reference counterpart). Returns (level, per-level sample times in ms, per-level scores):
+        entry i of the arrays is level i - 3, -1 where untried."""
         lvl = ctypes.c_int32()
-        ms = np.empty(11, np.float32)
-        cost = np.empty(11, np.float32)
+        ms = np.empty(AUTOTUNE_SLOTS, np.float32)
+        cost = np.empty(AUTOTUNE_SLOTS, np.float32)
         _check(_lib.egt_autotune(self.handle, k, thres, ctypes.byref(lvl), ms, cost))
         return lvl.value, ms, cost
 

@@ -57,18 +57,19 @@ def test_autotuned_start_level_keeps_rows(egt_mod, oracle_mod):
     g = E.GridStructure(pts, 1, 1e-6, 1024, 4)
     lvl, ms, cost = g.autotune(40)
     _log("autotune level", lvl, "trial ms", np.round(ms, 4).tolist(), "cost", np.round(cost, 1).tolist())
-    assert 0 <= lvl <= 10 and cost[lvl] > 0 and cost[lvl] == cost[cost > 0].min()
+    i = lvl + E.SUB_LEVELS
+    assert -E.SUB_LEVELS <= lvl <= 10 and cost[i] > 0 and cost[i] == cost[cost > 0].min()
     g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
     og = O.Grid(*pts)
     np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(og.knn(40)[0]))
     g.close()
 
 
-@pytest.mark.parametrize("start_level", [-1, 0, 2, 5, 9])
+@pytest.mark.parametrize("start_level", ["occupancy", -3, -2, -1, 0, 2, 5, 9])
 def test_knn_independent_of_start_level(egt_mod, oracle_mod, start_level):
     E, O = egt_mod, oracle_mod
     pts = D.sphere(8_000, seed=21)
-    g = _cloud(E, pts, 24, start_level=start_level)
+    g = _cloud(E, pts, 24, start_level=E.START_LEVEL_OCCUPANCY if start_level == "occupancy" else start_level)
     og = O.Grid(*pts)
     knn, _, _, _ = og.knn(24)
     np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(knn))
@@ -132,10 +133,12 @@ def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
 
 
 def _crowded_blob(n=30_000, sigma=0.002, seed=91):
-    """A tight Gaussian blob inside a unit cube (its 8 corners are points too): with 64 cells per axis the blob
-    falls into 8 cells of about 3,800 points, whose points keep the input order, so a Morton tile's halo spans
-    most of a cell. With k=40, alpha 0.65: 2048-point tiles have halos of up to about 6,900 points (5 in LDS,
-    10 partly in global memory); 1024-point tiles have no LDS-only tile, 23 partly global and 7 CSR tiles."""
+    """A tight Gaussian blob inside a unit cube (its 8 corners are points too). With 64 cells per axis and
+    sigma 0.002 the blob falls into 8 cells of about 3,800 points, which the search cuts into sub-cells. With
+    sigma 0.00025 it falls into the 8 finest sub-cells around the centre, whose points keep the input order,
+    so a Morton tile's halo spans most of a sub-cell: with k=40, alpha 0.65, 2048-point tiles have halos of up
+    to about 6,900 points (4 in LDS, 11 partly in global memory); 1024-point tiles have no LDS-only tile, 23
+    partly global and 7 CSR tiles."""
     rng = np.random.default_rng(seed)
     p = (0.5 + sigma * rng.standard_normal((n, 3))).astype(np.float32)
     p[:8] = np.array([[i & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], np.float32)
@@ -146,7 +149,7 @@ def _crowded_blob(n=30_000, sigma=0.002, seed=91):
 def test_taubin_crowded_cells_bit_exact(egt_mod, oracle_mod, tile):
     """All three kinds of tile in one pass: halo in LDS, halo partly in global memory, CSR rows."""
     E, O = egt_mod, oracle_mod
-    pts = _crowded_blob()
+    pts = _crowded_blob(sigma=0.00025)
     k, iters, alpha = 40, 6, 0.65
     g = _cloud(E, pts, k, bucket=64)
     g.set_option(E.OPT_TAUBIN_TILE, tile)
@@ -162,6 +165,28 @@ def test_taubin_crowded_cells_bit_exact(egt_mod, oracle_mod, tile):
     assert st["taubin_tile"] == tile and st["taubin_overflow_tiles"] > 0
     if tile == 1024:
         assert st["taubin_fallback_tiles"] > 0
+
+
+@pytest.mark.parametrize("start_level", [0, -1, -3])
+@pytest.mark.parametrize("maxc", [None, 3750])
+def test_knn_crowded_cells_sub_levels(egt_mod, oracle_mod, start_level, maxc):
+    """Cells of about 3,800 points: the search splits them into sub-cells (binary-searched ranges inside the
+    cell) or starts from boxes of 1/2 or 1/8 cells; with the population limit lowered to 3750 (between the
+    sizes of those cells) the reference's rule blanks the rows that meet a cell over the limit."""
+    E, O = egt_mod, oracle_mod
+    pts = _crowded_blob()
+    g = _cloud(E, pts, 40, bucket=64, maxc=maxc, start_level=start_level)
+    got = g.GetKNNIndexes()
+    st = g.stats()
+    g.close()
+    og = O.Grid(*pts, B=64)
+    knn, _, inv, _ = og.knn(40, maxc=maxc or 100000)
+    np.testing.assert_array_equal(got, og.to_original(knn))
+    assert st["invalid_rows"] == int(inv.sum())
+    _log("crowded knn start", start_level, "maxc", maxc, "invalid", int(inv.sum()), "candidates/query",
+         st["knn_candidates"] / pts[0].size, "cells/query", st["knn_cells"] / pts[0].size)
+    if maxc is not None:
+        assert 0 < inv.sum() < pts[0].size
 
 
 def test_taubin_vs_fp64(egt_mod, oracle_mod):
@@ -285,7 +310,7 @@ def test_regularized_taubin_bit_exact(egt_mod, oracle_mod, case):
     if case == "sphere":
         pts, k, iters = D.sphere(20_000, seed=44), 40, 10
     else:
-        pts, k, iters, B = _crowded_blob(), 40, 4, 64
+        pts, k, iters, B = _crowded_blob(sigma=0.00025), 40, 4, 64
     alpha = 0.65
     g = _cloud(E, pts, k, bucket=B)
     g.set_option(E.OPT_TAUBIN_REGULARIZED, 1)

@@ -21,15 +21,17 @@ def _queries_near(pts, n, seed, scale=1.0, noise=0.002):
     return tuple(np.ascontiguousarray(q[:, i], np.float32) for i in range(3))
 
 
-def _check(E, O, ref, q, k, thres=1e8, maxc=None):
-    g = E.GridStructure(ref, 1, 1e-6, 1024, 4)
+def _check(E, O, ref, q, k, thres=1e8, maxc=None, bucket=1024, start_level=None):
+    g = E.GridStructure(ref, 1, 1e-6, bucket, 4)
     if maxc is not None:
         g.set_option(E.OPT_MAX_PTS_IN_CUBE, maxc)
+    if start_level is not None:
+        g.set_option(E.OPT_START_LEVEL, start_level)
     g.GRIDCUDAKNNCompact(q, k, thres)
     got = g.GetKNNIndexes()
     st = g.stats()
     g.close()
-    og = O.Grid(*ref)
+    og = O.Grid(*ref, B=bucket)
     rows, _, inv, ost = og.knn_query(*q, k, thres=thres, maxc=maxc if maxc is not None else 100000)
     want = og.ref_to_original(rows)
     np.testing.assert_array_equal(got, want)
@@ -129,3 +131,18 @@ def test_query_argument_errors(egt_mod):
         g.GRIDCUDAKNNCompact(_queries_near(ref, 10, 1), 65)
     assert ei.value.status == 4
     g.close()
+
+
+@pytest.mark.parametrize("start_level", [0, -2])
+def test_queries_crowded_cells(egt_mod, oracle_mod, start_level):
+    """A tight blob in 64 cells per axis (cells of about 3,800 points, searched through sub-cell ranges), with
+    queries inside the blob and outside the grid, from the own cell and from quarter-cell boxes."""
+    rng = np.random.default_rng(57)
+    p = (0.5 + 0.002 * rng.standard_normal((30_000, 3))).astype(np.float32)
+    p[:8] = np.array([[i & 1, (i >> 1) & 1, (i >> 2) & 1] for i in range(8)], np.float32)
+    ref = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy())
+    qin = (0.5 + 0.003 * rng.standard_normal((6_000, 3))).astype(np.float32)
+    qout = rng.uniform(-0.3, 1.3, (2_000, 3)).astype(np.float32)
+    q = tuple(np.ascontiguousarray(np.concatenate([qin[:, i], qout[:, i]])) for i in range(3))
+    got, st, inv, ost = _check(egt_mod, oracle_mod, ref, q, 24, bucket=64, start_level=start_level)
+    _log("crowded queries, start", start_level, "candidates/query", st["query_candidates"] / q[0].size)
