@@ -175,8 +175,13 @@ __device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
 // from qpts / qvcell (Morton order of their cell clamped to the grid); rows go to knn at the query's original
 // index. A query outside the grid skips the early exit below: its search boxes (around the clamped cell) are
 // not the reference's stage boxes, so it searches until the list is exact.
+// Occupancy: the list (2 K registers) dominates the register budget; small lists are held to 128 VGPRs
+// (four waves per SIMD) and K <= 40 to 168 (three), which the measured runs favour over fewer waves without
+// spills.
 template <int K, bool PERSIST, bool QUERY = false>
-__global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(K <= 40 && !PERSIST ? 3 : 1))) k_knn(KnnParams P) {
+__global__ void __launch_bounds__(KBLK)
+    __attribute__((amdgpu_waves_per_eu(K <= 8 || (K <= 16 && !PERSIST) ? 4 : (K <= 40 && !PERSIST ? 3 : 1))))
+    k_knn(KnnParams P) {
     uint64_t cand = 0;
     uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0;
     // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
