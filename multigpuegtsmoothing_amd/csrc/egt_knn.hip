@@ -25,10 +25,11 @@ namespace egt {
 namespace {
 
 constexpr int KBLK = 256;
-#ifndef EGT_KNN_SPLIT
-#define EGT_KNN_SPLIT 48
-#endif
-constexpr uint32_t SPLIT_COUNT = EGT_KNN_SPLIT;  // cells with more points are scanned child by child
+// Cells with more points than this are scanned child by child. Measured on a 4M-point torus at start level 2
+// (k=40: 14.0 ms at 48, 12.2 at 64, 11.9 at 96, 12.0 at 128; k=64: 31.7 at 48, 26.5 at 96) and on a
+// 16M-point cluster mixture from the own cell (76.2 ms at 48, 77.1 at 96; 78.6 with 96 above the fine level
+// and 48 below).
+constexpr uint32_t SPLIT_COUNT = 96;
 constexpr uint64_t KEY_EMPTY = ~0ull;
 // Per-lane buffer of keys awaiting insertion and candidates fetched per step, by list length (measured on
 // a 4M-point torus: K = 16 runs fastest with 4 and 2, K = 40 and 64 with 8 and 4).
