@@ -340,7 +340,7 @@ def batch_main(args):
     """Batched uneven workload: args.clouds superquadric clouds, sizes log-uniform in [5k, 500k], k drawn
     from {16, 40, 64}, alpha 0.65, 100 iterations (lambda 0.5, mu -0.53), learned proposal on (model trained
     on the GPU on 64 clouds of other seeds). One step = batch build (cube, sort, cell tables for all clouds)
-    + learned proposal + kNN (one persistent work-queue launch per neighbour-count class) + EGG + Taubin
+    + learned proposal + kNN (one launch per neighbour-count class over 64-query chunks) + EGG + Taubin
     (one captured graph over all clouds). Coordinates are resident in HBM before timing. Beside it: the same
     batch with the plain start, and the plain GPU path (one handle per cloud, plain search, EGG, Taubin,
     cloud after cloud) timed once after a warm-up pass."""

@@ -81,8 +81,8 @@ typedef enum {
     EGT_OPT_LEARNED = 1,         /* kNN: 1 = start each query from the learned radius proposal */
     EGT_OPT_TAUBIN_GRAPH = 2,    /* Taubin: 1 = replay the lambda/mu loop as a captured hipGraph (default) */
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
-    EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class
-                                    (default for batch handles), 0 = one workgroup per 256 queries */
+    EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class,
+                                    0 = one workgroup per 256 queries (default) */
     EGT_OPT_TAUBIN_TILE = 5,     /* Taubin: points per LDS tile, 1024 or 2048 (default); 0 = autotune on the
                                     next smoothing call (times both on copies, keeps the faster; the
                                     per-pass times land in egt_stats.taubin_tile_ms) */

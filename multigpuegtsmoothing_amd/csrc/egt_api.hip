@@ -376,7 +376,7 @@ egt_status batch_create(const float* x, const float* y, const float* z, const in
         h->n = n;
         h->B = bucket_count;
         h->eps = eps;
-        h->knn_persistent = 1;
+        h->knn_persistent = 0;  // one wave per chunk: 221 ms against 300 ms persistent (512 clouds, 54.7M points)
         h->stats.n_points = n;
         h->stats.bucket_count = bucket_count;
         if (device_input) {

@@ -24,12 +24,15 @@ def _clouds():
     return clouds, np.array(ks, np.int32)
 
 
-def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod):
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent):
+    """persistent: the kNN as one wave per 64-query chunk (default) or as a work-queue launch."""
     E, O = egt_mod, oracle_mod
     clouds, ks = _clouds()
     lam, mu, alpha, its = 0.5, -0.53, 0.65, 12
-    _log("batch of", len(clouds), "clouds, sizes", [c[0].size for c in clouds])
+    _log("batch of", len(clouds), "clouds, sizes", [c[0].size for c in clouds], "persistent", persistent)
     b = E.BatchGrid(clouds)
+    b.set_option(E.OPT_KNN_PERSISTENT, persistent)
     b.knn(ks)
     refs = []
     for c, pts in enumerate(clouds):
