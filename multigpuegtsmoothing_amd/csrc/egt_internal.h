@@ -201,6 +201,9 @@ namespace egt {
 // scan / sort / reduce utilities (egt_prims.hip)
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
                         uint32_t* total_host);
+// Same, with the total written to device memory and no synchronization.
+void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
+                            uint32_t* total_dev);
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, Scratch& sc);
 uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
 

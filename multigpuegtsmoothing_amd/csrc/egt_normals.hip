@@ -64,13 +64,15 @@ __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, c
     nz[i] = vz / nrm;
 }
 
-// pass 0: count per owner u; pass 1: write v into u's bucket
-// Walks the Morton-order CSR (the same rows as the sorted-space neighbour lists, holes removed); the
-// ownership test scans the neighbour's compact CSR row.
+// pass 0: count per owner u and record which CSR entries emit an edge; pass 1: write v into u's bucket.
+// Walks the Morton-order CSR (the same rows as the sorted-space neighbour lists, holes removed). Entry (i, nb)
+// emits the edge {i, nb} when i < nb, or when i > nb and row nb does not list i (pass 0 scans nb's compact
+// CSR row for that and stores the answer in own[], which pass 1 reads).
 template <int PASS>
 __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                  const int32_t* __restrict__ morder, int64_t n,
-                                                 uint32_t* __restrict__ cnt, int32_t* __restrict__ ev) {
+                                                 uint32_t* __restrict__ cnt, uint8_t* __restrict__ own,
+                                                 int32_t* __restrict__ ev) {
     const int64_t m = (int64_t)blockIdx.x * NBLK + threadIdx.x;
     if (m >= n) return;
     const int32_t i = morder[m];
@@ -78,18 +80,20 @@ __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowp
     for (int32_t e = e0; e < e1; ++e) {
         const int32_t m2 = col[e];
         const int32_t nb = morder[m2];
-        if (nb == i) continue;
-        int32_t u, v;
-        if (i < nb) {
-            u = i;
-            v = nb;
+        bool emit;
+        if (PASS == 0) {
+            emit = nb != i;
+            if (emit && i > nb) {
+                bool listed = false;  // does row nb list i? then row nb owns the edge
+                for (int32_t f = rowptr[m2], f1 = rowptr[m2 + 1]; f < f1; ++f) listed = listed || col[f] == (int32_t)m;
+                emit = !listed;
+            }
+            own[e] = emit ? 1 : 0;
         } else {
-            bool listed = false;  // does row nb list i? then row nb owns the edge
-            for (int32_t f = rowptr[m2], f1 = rowptr[m2 + 1]; f < f1; ++f) listed = listed || col[f] == (int32_t)m;
-            if (listed) continue;
-            u = nb;
-            v = i;
+            emit = own[e] != 0;
         }
+        if (!emit) continue;
+        const int32_t u = i < nb ? i : nb, v = i < nb ? nb : i;
         const uint32_t p = atomicAdd(&cnt[u], 1u);
         if (PASS == 1) ev[p] = v;
     }
@@ -215,11 +219,14 @@ __device__ inline void block_atomic_min(uint64_t* base, int32_t idx, uint64_t va
     }
 }
 
+// The alive-edge count is read from device memory (*mcount); the grid covers an upper bound of it.
 __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
                                                    const uint64_t* __restrict__ ekey, const uint32_t* __restrict__ alive,
-                                                   uint32_t m, const int32_t* __restrict__ comp,
+                                                   const uint32_t* __restrict__ mcount, const int32_t* __restrict__ comp,
                                                    uint64_t* __restrict__ best) {
     const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
+    const uint32_t m = *mcount;
+    if (blockIdx.x * NBLK >= m) return;  // whole block past the count (uniform)
     int32_t cu = 0, cv = 0;
     uint64_t key = 0;
     bool act = t < m;
@@ -234,21 +241,28 @@ __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu
     wave_atomic_min(best, cv, key, act);
 }
 
-// Edges still joining two components after this round's merge (the rest can never be chosen again).
+// Edges still joining two components after this round's merge (the rest can never be chosen again). Flags
+// are written for every position below the bound (0 past the device count), which the scan then covers.
 __global__ void __launch_bounds__(NBLK) k_bv_alive(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
-                                                    const uint32_t* __restrict__ alive, uint32_t m,
+                                                    const uint32_t* __restrict__ alive,
+                                                    const uint32_t* __restrict__ mcount, uint32_t bound,
                                                     const int32_t* __restrict__ comp, uint32_t* __restrict__ flag) {
     const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
-    if (t >= m) return;
-    const uint32_t e = alive ? alive[t] : t;
-    flag[t] = comp[eu[e]] != comp[ev[e]] ? 1u : 0u;
+    if (t >= bound) return;
+    uint32_t f = 0;
+    if (t < *mcount) {
+        const uint32_t e = alive ? alive[t] : t;
+        f = comp[eu[e]] != comp[ev[e]] ? 1u : 0u;
+    }
+    flag[t] = f;
 }
 
-__global__ void __launch_bounds__(NBLK) k_bv_compact(const uint32_t* __restrict__ alive, uint32_t m,
+__global__ void __launch_bounds__(NBLK) k_bv_compact(const uint32_t* __restrict__ alive,
+                                                      const uint32_t* __restrict__ mcount,
                                                       const uint32_t* __restrict__ flag,
                                                       const uint32_t* __restrict__ pos, uint32_t* __restrict__ out) {
     const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
-    if (t >= m || !flag[t]) return;
+    if (t >= *mcount || !flag[t]) return;
     out[pos[t]] = alive ? alive[t] : t;
 }
 
@@ -392,7 +406,6 @@ T* dalloc(DBuf<uint8_t>& b, size_t& off, size_t count) {
 
 void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     const int64_t n = h->n;
-    const int k = h->k;
     hipStream_t s = h->stream;
     hipEvent_t ev0, ev1;
     HIP_CHECK(hipEventCreate(&ev0));
@@ -409,8 +422,9 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     // canonical undirected edge list
     uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
     uint32_t* off = h->scratch.get<uint32_t>(10, n);
+    uint8_t* own = h->scratch.get<uint8_t>(11, std::max<int64_t>(h->nnz, 1));
     HIP_CHECK(hipMemsetAsync(cnt, 0, n * sizeof(uint32_t), s));
-    k_edges<0><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, nullptr);
+    k_edges<0><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, nullptr);
     LAUNCH_CHECK();
     uint32_t m = 0;
     exclusive_scan_u32(cnt, off, n, s, h->scratch, &m);
@@ -436,11 +450,12 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     uint32_t* aflag = dalloc<uint32_t>(pool, po, mm);
     uint32_t* apos = dalloc<uint32_t>(pool, po, mm);
     uint32_t* flags = dalloc<uint32_t>(pool, po, 4);
+    uint32_t* dcount = dalloc<uint32_t>(pool, po, 2);  // alive-edge counts, current and next round
     double* part_w = dalloc<double>(pool, po, 1024);
     uint32_t* part_c = dalloc<uint32_t>(pool, po, 1024);
     if (po > pool.n) throw ArgError{EGT_ERR_STATE, "normals pool overflow"};
 
-    k_edges<1><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, ev);
+    k_edges<1><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, ev);
     LAUNCH_CHECK();
     k_edge_sort<<<gn, blk, 0, s>>>(off, n, m, ev, eu);
     LAUNCH_CHECK();
@@ -451,33 +466,48 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     }
     HIP_CHECK(hipMemsetAsync(msf, 0, mm, s));
 
-    // Boruvka with parity
+    // Boruvka with parity. The alive-edge count lives in device memory, so rounds follow one another without
+    // a host round trip; the host reads the count (to shrink the grids) and the convergence flags every
+    // second round. A round after convergence finds no edge and changes nothing.
     k_bv_init<<<gn, blk, 0, s>>>(n, comp, par);
     LAUNCH_CHECK();
+    HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
+    HIP_CHECK(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(dcount), m, 1, s));
     uint32_t* alive = nullptr;  // null = all edges
-    uint32_t malive = m;
-    for (int round = 0; round < 64 && malive > 0; ++round) {
-        const dim3 ga(std::max(1, ceil_div(malive, NBLK)));
+    uint32_t bound = m;         // host-side upper bound of the alive count
+    int cur = 0;
+    for (int round = 0; round < 64 && bound > 0; ++round) {
+        const dim3 ga(std::max(1, ceil_div(bound, NBLK)));
         k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
-        HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
-        k_bv_best<<<ga, blk, 0, s>>>(eu, ev, ekey, alive, malive, comp, best);
+        HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(uint32_t), s));  // hooks of this round; flags[1] accumulates
+        k_bv_best<<<ga, blk, 0, s>>>(eu, ev, ekey, alive, dcount + cur, comp, best);
         if (round == 0) HIP_CHECK(hipMemcpyAsync(best1, best, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
         k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf, flags);
         k_bv_jump<<<gn, blk, 0, s>>>(n, comp, H, flags + 1);
         k_bv_relabel<<<gn, blk, 0, s>>>(n, comp, par, H);
-        k_bv_alive<<<ga, blk, 0, s>>>(eu, ev, alive, malive, comp, aflag);
+        k_bv_alive<<<ga, blk, 0, s>>>(eu, ev, alive, dcount + cur, bound, comp, aflag);
         LAUNCH_CHECK();
-        uint32_t hf[2] = {0, 0};
-        HIP_CHECK(hipMemcpyAsync(hf, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        uint32_t next = 0;
-        exclusive_scan_u32(aflag, apos, malive, s, h->scratch, &next);  // synchronizes the stream
-        if (hf[1]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
-        if (hf[0] == 0 || next == 0) break;
+        exclusive_scan_u32_dev(aflag, apos, bound, s, h->scratch, dcount + (cur ^ 1));
         uint32_t* out = alive == alive_a ? alive_b : alive_a;
-        k_bv_compact<<<ga, blk, 0, s>>>(alive, malive, aflag, apos, out);
+        k_bv_compact<<<ga, blk, 0, s>>>(alive, dcount + cur, aflag, apos, out);
         LAUNCH_CHECK();
         alive = out;
-        malive = next;
+        cur ^= 1;
+        if (round & 1) {
+            uint32_t hf[3] = {0, 0, 0};
+            HIP_CHECK(hipMemcpyAsync(hf, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(hf + 2, dcount + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (hf[1]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
+            if (hf[0] == 0 || hf[2] == 0) break;
+            bound = hf[2];
+        }
+    }
+    {
+        uint32_t of = 0;
+        HIP_CHECK(hipMemcpyAsync(&of, flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (of) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
     }
 
     // orientation

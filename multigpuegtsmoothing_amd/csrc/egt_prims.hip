@@ -210,6 +210,15 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_
     }
 }
 
+void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
+                            uint32_t* total_dev) {
+    if (n <= 0) {
+        HIP_CHECK(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), s));
+        return;
+    }
+    scan_rec(in, out, n, s, &part_buf(sc, 0), 0, total_dev);
+}
+
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, Scratch& sc) {
     if (n <= 1) return;
     const int nb = ceil_div(n, SORT_TILE);
