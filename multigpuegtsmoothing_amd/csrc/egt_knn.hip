@@ -30,7 +30,13 @@ constexpr int KBLK = 256;
 // 16M-point cluster mixture from the own cell (76.2 ms at 48, 77.1 at 96; 78.6 with 96 above the fine level
 // and 48 below).
 constexpr uint32_t SPLIT_COUNT = 96;
-constexpr uint64_t KEY_EMPTY = ~0ull;
+// Keys (distance bits << 32 | sorted index) of non-negative distances, read as IEEE doubles, are positive
+// and order exactly as the integers do, so the list is held as doubles and merged with FP64 min / max (full
+// rate on gfx950; keys of zero distance are denormal doubles, which compute kernels keep). The empty key is
+// the largest finite double, above every real key (a real key's high word is below 0x7F800000).
+constexpr uint64_t KEY_EMPTY = 0x7FEFFFFFFFFFFFFFull;
+__device__ inline double key_d(uint64_t k) { return __builtin_bit_cast(double, k); }
+__device__ inline uint64_t key_u(double d) { return __builtin_bit_cast(uint64_t, d); }
 // Per-lane buffer of keys awaiting insertion and candidates fetched per step, by list length (measured on
 // a 4M-point torus: K = 16 runs fastest with 4 and 2, K = 40 and 64 with 8 and 4).
 template <int K>
@@ -68,15 +74,28 @@ struct KnnParams {
     float a, ba;
 };
 
+// FP64 min / max as single instructions. The keys are never NaN, so the quieting the compiler adds to keep
+// IEEE minNum semantics for arbitrary inputs (one more v_max_f64 per operand) is not needed.
+__device__ inline double dmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ inline double dmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Inserts key into the ascending list L (the largest entry drops out): position i becomes
+// max(L[i-1], min(key, L[i])), which is L[i-1] when key < L[i-1], key when L[i-1] <= key < L[i], else L[i].
+// Two FP64 operations per position, against a 64-bit compare, a hazard wait and four selects as integers.
 template <int K>
-__device__ inline void topk_insert(uint64_t (&L)[K], uint64_t key) {
+__device__ inline void topk_insert(double (&L)[K], uint64_t key) {
+    const double kd = key_d(key);
 #pragma unroll
-    for (int i = K - 1; i > 0; --i) {
-        const bool lt_prev = key < L[i - 1];
-        const bool lt_cur = key < L[i];
-        L[i] = lt_prev ? L[i - 1] : (lt_cur ? key : L[i]);
-    }
-    L[0] = key < L[0] ? key : L[0];
+    for (int i = K - 1; i > 0; --i) L[i] = dmax(L[i - 1], dmin(kd, L[i]));
+    L[0] = dmin(kd, L[0]);
 }
 
 // Reference certificate at stage s (self search GridKNNCuda.cu:694-741, query search :383-433) for a query
@@ -217,9 +236,9 @@ __global__ void __launch_bounds__(KBLK)
                       uz = cell_of(q.z, P.a, P.ba, P.B);
             outside = ux < 0 || uy < 0 || uz < 0 || ux >= P.B || uy >= P.B || uz >= P.B;
         }
-        uint64_t L[K];
+        double L[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) L[i] = (i < K - kq) ? 0ull : KEY_EMPTY;
+        for (int i = 0; i < K; ++i) L[i] = key_d((i < K - kq) ? 0ull : KEY_EMPTY);
         // Keys that beat the list's bound wait in a small per-lane buffer and are inserted in batches: the
         // wave executes an insertion step only as often as its busiest lane needs one, instead of at every
         // candidate where any lane inserts. The list (and so the bound used for pruning) lags by at most
@@ -299,7 +318,7 @@ __global__ void __launch_bounds__(KBLK)
                         const float d = sqd(q.x, q.y, q.z, e.x, e.y, e.z);
                         if (QUERY ? d < thres_n : d <= thres_n) {
                             const uint64_t kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
-                            if (kk < L[K - 1]) {
+                            if (kk < key_u(L[K - 1])) {
 #pragma unroll
                                 for (int b = 0; b < KNB; ++b) pbuf[b] = b == npend ? kk : pbuf[b];
                                 ++npend;
@@ -324,7 +343,8 @@ __global__ void __launch_bounds__(KBLK)
                 }
                 if (inside) return 0;
                 const float hc = P.ba / (float)(P.Bv >> lv);
-                const float bnd = L[K - 1] != KEY_EMPTY ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
+                const uint64_t lk = key_u(L[K - 1]);
+                const float bnd = lk != KEY_EMPTY ? __uint_as_float((uint32_t)(lk >> 32)) : thres_n;
                 float dd = 0.0f;
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) {
@@ -380,8 +400,9 @@ __global__ void __launch_bounds__(KBLK)
             flush();
             if (level >= P.nlev - 1) break;
             // exactness: every unscanned point is farther than the current bound
-            const bool full = L[K - 1] != KEY_EMPTY;
-            const float bound = full ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : thres_n;
+            const uint64_t lk = key_u(L[K - 1]);
+            const bool full = lk != KEY_EMPTY;
+            const float bound = full ? __uint_as_float((uint32_t)(lk >> 32)) : thres_n;
             const double R = __builtin_sqrt((double)bound) + 2e-6;
             const double hl = (double)P.ba / (double)Bl;
             const double qd[3] = {q.x, q.y, q.z};
@@ -407,16 +428,16 @@ __global__ void __launch_bounds__(KBLK)
         const uint32_t bm = (uint32_t)P.B - 1u;
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-            if (i >= K - kq && L[i] != KEY_EMPTY) {
+            if (i >= K - kq && key_u(L[i]) != KEY_EMPTY) {
                 ++c;
-                const uint32_t kk = (uint32_t)P.key[(uint32_t)L[i]];
+                const uint32_t kk = (uint32_t)P.key[(uint32_t)key_u(L[i])];
                 const int mx = kk & bm, my = (kk >> P.log2B) & bm, mz = kk >> (2 * P.log2B);
                 lo[0] = min(lo[0], mx); hi[0] = max(hi[0], mx);
                 lo[1] = min(lo[1], my); hi[1] = max(hi[1], my);
                 lo[2] = min(lo[2], mz); hi[2] = max(hi[2], mz);
             }
         }
-        const float kthd = c > 0 ? __uint_as_float((uint32_t)(L[K - 1] >> 32)) : 0.0f;
+        const float kthd = c > 0 ? __uint_as_float((uint32_t)(key_u(L[K - 1]) >> 32)) : 0.0f;
         bool valid = false;
         for (int s = 0; s <= P.stages; ++s) {
             const int lvl = s == 0 ? 0 : s - 1;
@@ -454,7 +475,7 @@ __global__ void __launch_bounds__(KBLK)
 #pragma unroll
         for (int i = 0; i < K; ++i) {
             if (i >= K - kq) {
-                const int32_t val = (valid && L[i] != KEY_EMPTY) ? (int32_t)(uint32_t)L[i] - base : -1;
+                const int32_t val = (valid && key_u(L[i]) != KEY_EMPTY) ? (int32_t)(uint32_t)key_u(L[i]) - base : -1;
                 row[i - (K - kq)] = val;
             }
         }
