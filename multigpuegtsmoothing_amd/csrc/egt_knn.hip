@@ -37,12 +37,14 @@ constexpr uint32_t SPLIT_COUNT = 96;
 constexpr uint64_t KEY_EMPTY = 0x7FEFFFFFFFFFFFFFull;
 __device__ inline double key_d(uint64_t k) { return __builtin_bit_cast(double, k); }
 __device__ inline uint64_t key_u(double d) { return __builtin_bit_cast(uint64_t, d); }
-// Per-lane buffer of keys awaiting insertion and candidates fetched per step, by list length (measured on
-// a 4M-point torus: K = 16 runs fastest with 4 and 2, K = 40 and 64 with 8 and 4).
+// Per-lane buffer of keys awaiting insertion (4) and candidates fetched per step (4). Measured on a 4M-point
+// torus at start level 2 and on a 16M-point mixture with the FP64 insertion: k=40 8.3 ms with a buffer of 4
+// against 8.7 with 8, 9.0 with 2 and 13.9 with 16 (register pressure); fetching 2 or 8 per step runs slower
+// (9.0, 14.0 ms); k=16 4.4 ms fetching 4 against 4.7 fetching 2.
 template <int K>
-constexpr int knn_buffer() { return K <= 16 ? 4 : 8; }
+constexpr int knn_buffer() { return 4; }
 template <int K>
-constexpr int knn_fetch() { return K <= 16 ? 2 : 4; }
+constexpr int knn_fetch() { return 4; }
 
 struct KnnParams {
     const float4* __restrict__ mpts;
