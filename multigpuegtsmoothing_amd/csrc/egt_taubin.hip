@@ -195,7 +195,8 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
 
 // Sliced-ELLPACK columns of one tile: slice s (sorted positions 64 s .. 64 s + 63) stores its width W_s
 // slots slot-major, slot u of lane l at 64 u + l, so a wave's load of one slot is 128 contiguous bytes.
-// Slots past a row's end hold 0xFFFF. Slot order within a row is the CSR (neighbour) order.
+// Slots past a row's end hold the row's own local index (a row never lists itself), lanes past the tile's
+// last row hold 0. Slot order within a row is the CSR (neighbour) order.
 template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ rowptr,
                                                      const uint16_t* __restrict__ col16, int64_t n,
@@ -214,11 +215,11 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
         for (uint32_t x = threadIdx.x; x < 64u * W; x += PBLK) {
             const uint32_t u = x >> 6;
             const int j = s * 64 + (int)(x & 63u);
-            uint16_t v = 0xFFFFu;
+            uint16_t v = 0;
             if (j < np) {
                 const int li = perm[t * TILE + j];
                 const int32_t e0 = rowptr[base + li];
-                if ((int32_t)u < rowptr[base + li + 1] - e0) v = col16[e0 + u];
+                v = (int32_t)u < rowptr[base + li + 1] - e0 ? col16[e0 + u] : (uint16_t)li;
             }
             dst[sb + x] = v;
         }
@@ -229,7 +230,9 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
 // Neighbour loop of one row over LDS coordinates, its local columns at sc[64 u] (sliced ELLPACK, W slots,
 // W uniform over the wave). The columns of CH slots, then their LDS coordinates, are fetched before use, so
 // a chunk costs one global and one LDS latency (CH = 8: 16 slots in flight cost more in VGPRs, and so in
-// occupancy, than they save); accumulation stays in slot order. The weighted difference uses
+// occupancy, than they save); accumulation stays in slot order. A padding slot holds the row's own index
+// (self): its distance is 0, its weight is selected to 0 and it adds fma(-0, 0, acc) = acc and sum + 0 = sum,
+// so every lane runs every slot without a branch on the padding. The weighted difference uses
 // w * (q - p) = (-w) * (p - q): q - p is exactly -(p - q), and an accumulator that starts at +0 never
 // becomes -0, so the sign of a zero difference cannot change a sum; the reuse saves three subtractions
 // per neighbour.
@@ -247,29 +250,29 @@ template <bool REG, bool OVF>
 __device__ inline void taubin_sell_row(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ sc,
                                        int W, float scale, float& cgx, float& cgy, float& cgz, float& sum,
                                        const float4* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
-                                       uint32_t tile) {
+                                       uint32_t tile, uint32_t self) {
     constexpr int CH = 8;
     for (int u0 = 0; u0 < W; u0 += CH) {
         uint32_t c[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) c[u] = u0 + u < W ? (uint32_t)sc[(u0 + u) * 64] : 0xFFFFu;
-        // every read of the chunk is issued before the first use (padding slots read entry 0)
+        for (int u = 0; u < CH; ++u) c[u] = u0 + u < W ? (uint32_t)sc[(u0 + u) * 64] : 0u;
+        // every read of the chunk is issued before the first use
         float4 qs[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (u0 + u < W) {
-                const uint32_t cc = c[u] != 0xFFFFu ? c[u] : 0u;
-                if (!OVF || cc < lim) qs[u] = src[cc];
-                else qs[u] = gin[hal[cc - tile]];
+            if (u0 + u < W) {  // wave-uniform
+                if (!OVF || c[u] < lim) qs[u] = src[c[u]];
+                else qs[u] = gin[hal[c[u] - tile]];
             }
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            if (c[u] != 0xFFFFu) {
+            if (u0 + u < W) {  // wave-uniform
                 const float4 q = qs[u];
                 const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                 const float dist = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const float w = taubin_weight<REG>(dist, scale);
+                float w = taubin_weight<REG>(dist, scale);
+                w = c[u] != self ? w : 0.0f;
                 cgx = __builtin_fmaf(-w, dx, cgx);
                 cgy = __builtin_fmaf(-w, dy, cgy);
                 cgz = __builtin_fmaf(-w, dz, cgz);
@@ -356,10 +359,13 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
         // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
         const uint32_t sb0 = __builtin_amdgcn_readfirstlane(sbase[s]);
         const int W = (int)((__builtin_amdgcn_readfirstlane(sbase[s + 1]) - sb0) >> 6);
+        const uint32_t self = row[r] < 0 ? 0xFFFFFFFFu : (uint32_t)row[r];  // lanes past the last row hold 0
         if (H > halo_cap<TILE>())
-            taubin_sell_row<REG, true>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE);
+            taubin_sell_row<REG, true>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE,
+                                       self);
         else
-            taubin_sell_row<REG, false>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE);
+            taubin_sell_row<REG, false>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE,
+                                        self);
         res[r] = taubin_result(p, cgx, cgy, cgz, sum, scale);
     }
     __syncthreads();
