@@ -28,6 +28,90 @@ __device__ inline float rowt(float a1, float a2, float a3, float t, float x, flo
     return __builtin_fmaf(a3, z, __builtin_fmaf(a2, y, a1 * x)) + t;
 }
 
+// Ellipsoid frame of slot j: rows a, b, c, translation t, squared semi-axis dd, ball pre-test radius^2 rb2
+// around the midpoint o.
+struct EggFrame {
+    float ax, ay, az, bx, by, bz, cx, cy, cz, tx, ty, tz, xo, yo, zo, dd, rb2, lo, hi;
+};
+
+// True when one valid slot m < j (coordinates in LDS at sx[m * stride + col], ...) lies strictly inside the
+// ellipsoid: v = ex^2 + ey^2 / aa + ez^2 / aa < dd. Slots are read four at a time, their LDS loads issued
+// together. The two correctly rounded divisions are only evaluated near the boundary: the estimate
+// vf = ex^2 + (ey^2 + ez^2) * (1 / aa) sums non-negative terms, so it is within about 7 units in the last
+// place (4.2e-7 relative) of v as written; outside dd (1 -+ 1e-5) the estimate decides as v would.
+__device__ inline bool egg_inside(const float* __restrict__ sx, const float* __restrict__ sy,
+                                  const float* __restrict__ sz, int stride, int col, int j, int kmax,
+                                  uint64_t valid, const EggFrame& F, float aa, float inv_aa) {
+    for (int m0 = 0; m0 < j; m0 += 4) {
+        float px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int mm = min(m0 + u, kmax - 1);
+            px[u] = sx[mm * stride + col];
+            py[u] = sy[mm * stride + col];
+            pz[u] = sz[mm * stride + col];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int mm = m0 + u;
+            if (mm >= j || !((valid >> mm) & 1ull)) continue;
+            if (sqd(px[u], py[u], pz[u], F.xo, F.yo, F.zo) >= F.rb2) continue;
+            const float ex = rowt(F.ax, F.ay, F.az, F.tx, px[u], py[u], pz[u]);
+            const float ey = rowt(F.bx, F.by, F.bz, F.ty, px[u], py[u], pz[u]);
+            const float ez = rowt(F.cx, F.cy, F.cz, F.tz, px[u], py[u], pz[u]);
+            const float e2 = ex * ex, f2 = ey * ey, g2 = ez * ez;
+            const float vf = e2 + (f2 + g2) * inv_aa;
+            if (vf < F.lo) return true;
+            if (vf > F.hi) continue;
+            if (e2 + f2 / aa + g2 / aa < F.dd) return true;
+        }
+    }
+    return false;
+}
+
+// Frame of slot j (coordinates xi, yi, zi) for the point (x, y, z); false when |pq|/2 < 1e-6 (always kept).
+__device__ inline bool egg_frame(float x, float y, float z, float xi, float yi, float zi, float alpha,
+                                 EggFrame& F) {
+    F.xo = 0.5f * (xi + x);
+    F.yo = 0.5f * (yi + y);
+    F.zo = 0.5f * (zi + z);
+    const float d = __builtin_sqrtf(sqd(xi, yi, zi, x, y, z)) / 2.0f;
+    if ((double)d < 1e-6) return false;
+    float ax = xi - x, ay = yi - y, az = zi - z;
+    normalize3(ax, ay, az);
+    float mxv = __builtin_fabsf(ax);
+    int ci = 0;
+    if (mxv < __builtin_fabsf(ay)) {
+        ci = 1;
+        mxv = __builtin_fabsf(ay);
+    }
+    if (mxv < __builtin_fabsf(az)) ci = 2;
+    float bx = 1.0f, by = 1.0f, bz = 1.0f;
+    if (ci == 0) bx = (-ay - az) / ax;
+    else if (ci == 1) by = (-ax - az) / ay;
+    else bz = (-ax - ay) / az;
+    normalize3(bx, by, bz);
+    float cx = ay * bz - by * az;
+    float cy = bx * az - ax * bz;
+    float cz = ax * by - bx * ay;
+    normalize3(cx, cy, cz);
+    F.ax = ax; F.ay = ay; F.az = az;
+    F.bx = bx; F.by = by; F.bz = bz;
+    F.cx = cx; F.cy = cy; F.cz = cz;
+    F.tx = -rowt(ax, ay, az, 0.0f, F.xo, F.yo, F.zo);
+    F.ty = -rowt(bx, by, bz, 0.0f, F.xo, F.yo, F.zo);
+    F.tz = -rowt(cx, cy, cz, 0.0f, F.xo, F.yo, F.zo);
+    F.dd = d * d;
+    F.lo = F.dd * (1.0f - 1e-5f);
+    F.hi = F.dd * (1.0f + 1e-5f);
+    // The ellipsoid lies in the ball of radius d*max(1, alpha) around the midpoint. A candidate outside that
+    // ball padded by a relative 1e-4 and an absolute 4e-6 (well above the ~1e-6 rounding of the frame
+    // transform on unit-cube coordinates) cannot pass the exact test, so it is skipped.
+    const float rb = d * fmaxf(1.0f, alpha) * 1.0001f + 4e-6f;
+    F.rb2 = rb * rb;
+    return true;
+}
+
 // mpts: current points in Morton order (w = sorted index). knn: sorted-space rows, updated in place.
 // mrow: per Morton position, the k slots mapped to Morton positions (-1 for empty). keepm: kept-slot mask.
 // One workgroup per 64-point chunk of one cloud: chunks[b] is the chunk's first Morton position (null: 64 b).
@@ -70,55 +154,14 @@ __global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, c
     }
     const float x = p.x, y = p.y, z = p.z;
     const float aa = alpha * alpha;
+    const float inv_aa = 1.0f / aa;
     uint64_t keep = valid;
     for (int j = k - 1; j >= 0; --j) {
         if (!((valid >> j) & 1ull)) continue;
-        const float xi = sx[j][tid], yi = sy[j][tid], zi = sz[j][tid];
-        const float xo = 0.5f * (xi + x), yo = 0.5f * (yi + y), zo = 0.5f * (zi + z);
-        const float d = __builtin_sqrtf(sqd(xi, yi, zi, x, y, z)) / 2.0f;
-        if ((double)d < 1e-6) continue;  // kept
-        float ax = xi - x, ay = yi - y, az = zi - z;
-        normalize3(ax, ay, az);
-        float mxv = __builtin_fabsf(ax);
-        int ci = 0;
-        if (mxv < __builtin_fabsf(ay)) {
-            ci = 1;
-            mxv = __builtin_fabsf(ay);
-        }
-        if (mxv < __builtin_fabsf(az)) ci = 2;
-        float bx = 1.0f, by = 1.0f, bz = 1.0f;
-        if (ci == 0) bx = (-ay - az) / ax;
-        else if (ci == 1) by = (-ax - az) / ay;
-        else bz = (-ax - ay) / az;
-        normalize3(bx, by, bz);
-        float cx = ay * bz - by * az;
-        float cy = bx * az - ax * bz;
-        float cz = ax * by - bx * ay;
-        normalize3(cx, cy, cz);
-        const float tx = -rowt(ax, ay, az, 0.0f, xo, yo, zo);
-        const float ty = -rowt(bx, by, bz, 0.0f, xo, yo, zo);
-        const float tz = -rowt(cx, cy, cz, 0.0f, xo, yo, zo);
-        const float dd = d * d;
-        // The ellipsoid lies in the ball of radius d*max(1, alpha) around the midpoint. A candidate outside
-        // that ball padded by a relative 1e-4 and an absolute 4e-6 (well above the ~1e-6 rounding of the frame
-        // transform on unit-cube coordinates) cannot pass the exact test below, so it is skipped.
-        const float rb = d * fmaxf(1.0f, alpha) * 1.0001f + 4e-6f;
-        const float rb2 = rb * rb;
-        uint64_t mset = valid & ((1ull << j) - 1ull);
-        while (mset) {
-            const int mm = __builtin_ctzll(mset);
-            mset &= mset - 1ull;
-            const float px = sx[mm][tid], py = sy[mm][tid], pz = sz[mm][tid];
-            if (sqd(px, py, pz, xo, yo, zo) >= rb2) continue;
-            const float ex = rowt(ax, ay, az, tx, px, py, pz);
-            const float ey = rowt(bx, by, bz, ty, px, py, pz);
-            const float ez = rowt(cx, cy, cz, tz, px, py, pz);
-            const float v = ex * ex + ey * ey / aa + ez * ez / aa;
-            if (v < dd) {
-                keep &= ~(1ull << j);
-                break;
-            }
-        }
+        EggFrame F;
+        if (!egg_frame(x, y, z, sx[j][tid], sy[j][tid], sz[j][tid], alpha, F)) continue;  // kept
+        if (egg_inside(&sx[0][0], &sy[0][0], &sz[0][0], EBLK, tid, j, K, valid, F, aa, inv_aa))
+            keep &= ~(1ull << j);
     }
     uint64_t dropped = valid & ~keep;
     while (dropped) {
@@ -192,52 +235,15 @@ __global__ void __launch_bounds__(EBLK) k_egg_shared(const float4* __restrict__ 
     if (act) {
         const float x = p.x, y = p.y, z = p.z;
         const float aa = alpha * alpha;
+        const float inv_aa = 1.0f / aa;
         for (int u = NJ - 1; u >= 0; --u) {
             const int j = r + u * LPP;
             if (j >= K || !((valid >> j) & 1ull)) continue;
-            const float xi = sx[j][pi], yi = sy[j][pi], zi = sz[j][pi];
-            const float xo = 0.5f * (xi + x), yo = 0.5f * (yi + y), zo = 0.5f * (zi + z);
-            const float d = __builtin_sqrtf(sqd(xi, yi, zi, x, y, z)) / 2.0f;
-            if ((double)d < 1e-6) continue;  // kept
-            float ax = xi - x, ay = yi - y, az = zi - z;
-            normalize3(ax, ay, az);
-            float mxv = __builtin_fabsf(ax);
-            int ci = 0;
-            if (mxv < __builtin_fabsf(ay)) {
-                ci = 1;
-                mxv = __builtin_fabsf(ay);
-            }
-            if (mxv < __builtin_fabsf(az)) ci = 2;
-            float bx = 1.0f, by = 1.0f, bz = 1.0f;
-            if (ci == 0) bx = (-ay - az) / ax;
-            else if (ci == 1) by = (-ax - az) / ay;
-            else bz = (-ax - ay) / az;
-            normalize3(bx, by, bz);
-            float cx = ay * bz - by * az;
-            float cy = bx * az - ax * bz;
-            float cz = ax * by - bx * ay;
-            normalize3(cx, cy, cz);
-            const float tx = -rowt(ax, ay, az, 0.0f, xo, yo, zo);
-            const float ty = -rowt(bx, by, bz, 0.0f, xo, yo, zo);
-            const float tz = -rowt(cx, cy, cz, 0.0f, xo, yo, zo);
-            const float dd = d * d;
-            const float rb = d * fmaxf(1.0f, alpha) * 1.0001f + 4e-6f;
-            const float rb2 = rb * rb;
-            uint64_t mset = valid & ((1ull << j) - 1ull);
-            while (mset) {
-                const int mm = __builtin_ctzll(mset);
-                mset &= mset - 1ull;
-                const float px = sx[mm][pi], py = sy[mm][pi], pz = sz[mm][pi];
-                if (sqd(px, py, pz, xo, yo, zo) >= rb2) continue;
-                const float ex = rowt(ax, ay, az, tx, px, py, pz);
-                const float ey = rowt(bx, by, bz, ty, px, py, pz);
-                const float ez = rowt(cx, cy, cz, tz, px, py, pz);
-                const float v = ex * ex + ey * ey / aa + ez * ez / aa;
-                if (v < dd) {
-                    drop |= 1ull << j;
-                    row[j] = -1;
-                    break;
-                }
+            EggFrame F;
+            if (!egg_frame(x, y, z, sx[j][pi], sy[j][pi], sz[j][pi], alpha, F)) continue;  // kept
+            if (egg_inside(&sx[0][0], &sy[0][0], &sz[0][0], PPB, pi, j, K, valid, F, aa, inv_aa)) {
+                drop |= 1ull << j;
+                row[j] = -1;
             }
         }
     }
