@@ -108,25 +108,49 @@ __device__ inline int f2o(float f) {
 // Per-cloud min/max of x, y, z (fold starting at +-1e8, as the reference's thrust reductions
 // GridKNNCuda.cu:1042-1059) plus a count of non-finite inputs. A wave whose 64 points share one cloud
 // reduces first and issues one atomic per value.
+// Each thread folds MM_PER points (stride BLK, so loads stay coalesced) into running extremes of the cloud it
+// is in, flushing them with atomics only when the cloud changes (clouds are contiguous in the input); the
+// final extremes are reduced over the wave when the wave ends in one cloud. Every point of a wave used to
+// hit the same six addresses of its cloud: 5.5 ms on 54.7M points against a fraction of that now.
+constexpr int MM_PER = 16;
+__device__ inline void minmax_flush(int* __restrict__ mm, int c, const float (&v)[6]) {
+    for (int j = 0; j < 6; j += 2) {
+        atomicMin(&mm[c * 6 + j], f2o(v[j]));
+        atomicMax(&mm[c * 6 + j + 1], f2o(v[j + 1]));
+    }
+}
 __global__ void __launch_bounds__(BLK) k_cloud_minmax(const float* __restrict__ x, const float* __restrict__ y,
                                                        const float* __restrict__ z, const int32_t* __restrict__ icloud,
                                                        int64_t n, int* __restrict__ mm, uint32_t* __restrict__ bad) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const bool act = i < n;
-    const int c = act ? icloud[i] : -1;
+    const int64_t first = (int64_t)blockIdx.x * BLK * MM_PER + threadIdx.x;
+    int c = -1;
     float v[6] = {1e8f, -1e8f, 1e8f, -1e8f, 1e8f, -1e8f};
     uint32_t nb = 0;
-    if (act) {
+    for (int r = 0; r < MM_PER; ++r) {
+        const int64_t i = first + (int64_t)r * BLK;
+        if (i >= n) break;
+        const int ci = icloud[i];
+        if (ci != c) {
+            if (c >= 0) minmax_flush(mm, c, v);
+            c = ci;
+#pragma unroll
+            for (int j = 0; j < 6; j += 2) {
+                v[j] = 1e8f;
+                v[j + 1] = -1e8f;
+            }
+        }
         const float p[3] = {x[i], y[i], z[i]};
-        nb = !(__builtin_isfinite(p[0]) && __builtin_isfinite(p[1]) && __builtin_isfinite(p[2]));
+        nb += !(__builtin_isfinite(p[0]) && __builtin_isfinite(p[1]) && __builtin_isfinite(p[2]));
         for (int ax = 0; ax < 3; ++ax) {
             v[2 * ax] = fminf(v[2 * ax], p[ax]);
             v[2 * ax + 1] = fmaxf(v[2 * ax + 1], p[ax]);
         }
     }
+    const bool act = c >= 0;
     const int c0 = __shfl(c, 0, 64);
+    // uniform with c0 < 0: a wave past the end with no point (nothing to add)
     const bool uniform = __ballot(c == c0) == ~0ull;
-    if (uniform) {
+    if (uniform && c0 >= 0) {
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
 #pragma unroll
@@ -143,11 +167,8 @@ __global__ void __launch_bounds__(BLK) k_cloud_minmax(const float* __restrict__ 
             }
             if (nb) atomicAdd(bad, nb);
         }
-    } else if (act) {
-        for (int j = 0; j < 6; j += 2) {
-            atomicMin(&mm[c * 6 + j], f2o(v[j]));
-            atomicMax(&mm[c * 6 + j + 1], f2o(v[j + 1]));
-        }
+    } else if (!uniform && act) {
+        minmax_flush(mm, c, v);
         if (nb) atomicAdd(bad, nb);
     }
 }
@@ -257,22 +278,28 @@ __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict
                                                        uint2* __restrict__ hvals, uint64_t offset, uint32_t log2cap,
                                                        int* __restrict__ maxc) {
     const int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (c >= cells) return;
-    const uint32_t st = starts[c];
-    const uint32_t en = c + 1 < cells ? starts[c + 1] : (uint32_t)n;
-    const unsigned long long cc = cell_key(mcloud ? (uint32_t)mcloud[st] : 0u, code[st] >> shift);
-    const uint32_t mask = (1u << log2cap) - 1u;
-    uint32_t s = hash_slot(cc, log2cap);
-    // a table has at least twice as many slots as cells, so an empty slot is always found
-    for (;;) {
-        const unsigned long long prev = atomicCAS(&hkeys[offset + s], (unsigned long long)HASH_EMPTY, cc);
-        if (prev == HASH_EMPTY) {
-            hvals[offset + s] = make_uint2(st, en - st);
-            break;
+    int cnt = 0;
+    if (c < cells) {
+        const uint32_t st = starts[c];
+        const uint32_t en = c + 1 < cells ? starts[c + 1] : (uint32_t)n;
+        const unsigned long long cc = cell_key(mcloud ? (uint32_t)mcloud[st] : 0u, code[st] >> shift);
+        const uint32_t mask = (1u << log2cap) - 1u;
+        uint32_t s = hash_slot(cc, log2cap);
+        // a table has at least twice as many slots as cells, so an empty slot is always found
+        for (;;) {
+            const unsigned long long prev = atomicCAS(&hkeys[offset + s], (unsigned long long)HASH_EMPTY, cc);
+            if (prev == HASH_EMPTY) {
+                hvals[offset + s] = make_uint2(st, en - st);
+                break;
+            }
+            s = (s + 1u) & mask;
         }
-        s = (s + 1u) & mask;
+        cnt = (int)(en - st);
     }
-    atomicMax(maxc, (int)(en - st));
+    // largest cell of the level: one atomic per wave (every cell used to hit the one address)
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cnt = max(cnt, __shfl_xor(cnt, d, 64));
+    if (__lane_id() == 0) atomicMax(maxc, cnt);
 }
 
 // seg value at each cloud's first Morton position (running cell count), plus the total at the end.
@@ -471,7 +498,7 @@ void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float*
     k_minmax_init<<<ceil_div(nclouds * 6, BLK), BLK, 0, s>>>(mm, nclouds);
     LAUNCH_CHECK();
     HIP_CHECK(hipMemsetAsync(dbad, 0, sizeof(uint32_t), s));
-    k_cloud_minmax<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, icloud, n, mm, dbad);
+    k_cloud_minmax<<<std::max(1, ceil_div(n, (int64_t)BLK * MM_PER)), BLK, 0, s>>>(dx, dy, dz, icloud, n, mm, dbad);
     LAUNCH_CHECK();
     std::vector<int> hm((size_t)nclouds * 6 + 1);
     HIP_CHECK(hipMemcpyAsync(hm.data(), mm, hm.size() * sizeof(int), hipMemcpyDeviceToHost, s));
