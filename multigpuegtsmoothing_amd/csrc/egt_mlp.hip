@@ -402,20 +402,47 @@ __global__ void __launch_bounds__(256) k_targets(const float4* __restrict__ mpts
 // smallest (from the finest sub-cell level -SUB up) whose 3x3x3 block around the query reaches beyond
 // r = h_ref * 2^(y + safety) on every side (reach along an axis = h_l * (1 + distance of the query to the
 // nearer face of its cell in cell units)).
+// Key of the query's cell at its cloud's reference level (the features depend on nothing else).
+__device__ inline uint64_t feature_cell(const FeatParams& F, int64_t m) {
+    const uint32_t cloud = F.mcloud ? (uint32_t)F.mcloud[m] : 0u;
+    const int sh = SUB + F.clouds[cloud].ref_level;
+    const uint64_t qc = F.mvcell[m];
+    return ((uint64_t)cloud << 30) | ((uint64_t)(vcell_axis(qc, 0) >> sh) << 20) |
+           ((uint64_t)(vcell_axis(qc, 1) >> sh) << 10) | (uint64_t)(vcell_axis(qc, 2) >> sh);
+}
+
 __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ P, FeatParams F, float a, float ba,
                                                        int nlev, float safety, int8_t* __restrict__ qlevel) {
     __shared__ WaveLds lds[WPB];
+    __shared__ int lead_s[WPB][TS];
     WaveLds& L = lds[threadIdx.x / 64];
+    int* lead = lead_s[threadIdx.x / 64];
     FwdOps o;
     load_fwd(P, o);
     const int l = __lane_id();
     const int64_t ntile = (F.n + TS - 1) / TS;
     for (int64_t t = (int64_t)blockIdx.x * WPB + threadIdx.x / 64; t < ntile; t += (int64_t)gridDim.x * WPB) {
         const int64_t first = t * TS;
+        // Morton-consecutive queries in one reference-level cell have the same feature row: only the first
+        // of each run (its leader) does the 27 cell lookups, the others copy its row.
+        {
+            const int64_t m = first + l;
+            const uint64_t key = (l < TS && m < F.n) ? feature_cell(F, m) : ~0ull;
+            const uint64_t prev = __shfl_up(key, 1, 64);
+            const uint64_t leaders = __ballot(l < TS && (l == 0 || key != prev));
+            const uint64_t upto = leaders & (l == 63 ? ~0ull : ((2ull << l) - 1ull));
+            if (l < TS) lead[l] = 63 - __clzll(upto);
+        }
+        wave_sync();
         for (int q = l; q < TS * NIN; q += 64) {
             const int i = q / NIN, f = q % NIN;
             const int64_t m = first + i;
-            L.X[i][f] = bf(m < F.n ? feature(F, m, f) : 0.0f);
+            if (lead[i] == i) L.X[i][f] = bf(m < F.n ? feature(F, m, f) : 0.0f);
+        }
+        wave_sync();
+        for (int q = l; q < TS * NIN; q += 64) {
+            const int i = q / NIN, f = q % NIN;
+            if (lead[i] != i) L.X[i][f] = L.X[lead[i]][f];
         }
         wave_sync();
         tile_forward(L, P, o);
