@@ -4,8 +4,8 @@
 // Reference: calculateEGG_GKNN EGT_CUDA.cu:172-189, isEllipicGabrielNeighbor_GKNN :136-170, frame helpers
 // :55-134, batch driver :505-530.
 //
-// One thread per point, threads walking the points in Morton order so the neighbour coordinates a wave
-// gathers sit in a few cache lines. The point's neighbour coordinates are staged in LDS as [slot][lane]
+// Lanes walk the points in Morton order so the neighbour coordinates a wave gathers sit in a few cache lines;
+// 4, 8 or 16 lanes share a point. The point's neighbour coordinates are staged in LDS as [slot][point]
 // (bank-conflict free). Slot j (ranked by distance) is dropped when a valid slot m < j lies strictly inside
 // the ellipsoid with semi-axis |pq_j|/2 along pq_j and alpha*|pq_j|/2 across it. Tests for slot j only read
 // the original entries of slots m < j, so the result equals the reference's in-place downward sweep. A
@@ -112,73 +112,13 @@ __device__ inline bool egg_frame(float x, float y, float z, float xi, float yi, 
     return true;
 }
 
-// mpts: current points in Morton order (w = sorted index). knn: sorted-space rows, updated in place.
-// mrow: per Morton position, the k slots mapped to Morton positions (-1 for empty). keepm: kept-slot mask.
-// One workgroup per 64-point chunk of one cloud: chunks[b] is the chunk's first Morton position (null: 64 b).
-template <int K>
-__global__ void __launch_bounds__(EBLK) k_egg(const float4* __restrict__ mpts, const int32_t* __restrict__ minv,
-                                               int32_t* __restrict__ knn, const CloudDev* __restrict__ clouds,
-                                               const int32_t* __restrict__ mcloud, const int64_t* __restrict__ chunks,
-                                               float alpha, int32_t* __restrict__ mrow, uint64_t* __restrict__ keepm,
-                                               uint32_t* __restrict__ kept) {
-    __shared__ float sx[K][EBLK], sy[K][EBLK], sz[K][EBLK];
-    const int tid = threadIdx.x;
-    const int64_t start = chunks ? chunks[blockIdx.x] : (int64_t)blockIdx.x * EBLK;
-    const CloudDev cd = clouds[mcloud ? mcloud[start] : 0];
-    const int k = cd.k;
-    const int64_t m = start + tid;
-    if (m >= cd.base + cd.n) return;
-    const float4 p = mpts[m];
-    const int32_t s = __float_as_int(p.w);
-    int32_t* row = knn + cd.row_off + (int64_t)(s - cd.base) * k;
-    int32_t* mr = mrow + cd.row_off + (m - cd.base) * k;
-    const int32_t base = (int32_t)cd.base;
-    // staging: the row, then every Morton position, then every coordinate are issued back to back, so the
-    // three dependent loads per slot cost three latencies per point rather than three per slot
-    uint64_t valid = 0;
-    int32_t nbv[K];
-#pragma unroll
-    for (int j = 0; j < K; ++j) nbv[j] = j < k ? row[j] : -1;
-#pragma unroll
-    for (int j = 0; j < K; ++j) nbv[j] = nbv[j] >= 0 ? minv[nbv[j] + base] : -1;
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-        if (nbv[j] >= 0) {
-            valid |= 1ull << j;
-            const float4 c = mpts[nbv[j]];
-            sx[j][tid] = c.x;
-            sy[j][tid] = c.y;
-            sz[j][tid] = c.z;
-        }
-        if (j < k) mr[j] = nbv[j];
-    }
-    const float x = p.x, y = p.y, z = p.z;
-    const float aa = alpha * alpha;
-    const float inv_aa = 1.0f / aa;
-    uint64_t keep = valid;
-    for (int j = k - 1; j >= 0; --j) {
-        if (!((valid >> j) & 1ull)) continue;
-        EggFrame F;
-        if (!egg_frame(x, y, z, sx[j][tid], sy[j][tid], sz[j][tid], alpha, F)) continue;  // kept
-        if (egg_inside(&sx[0][0], &sy[0][0], &sz[0][0], EBLK, tid, j, K, valid, F, aa, inv_aa))
-            keep &= ~(1ull << j);
-    }
-    uint64_t dropped = valid & ~keep;
-    while (dropped) {
-        const int j = __builtin_ctzll(dropped);
-        dropped &= dropped - 1ull;
-        row[j] = -1;
-    }
-    keepm[m] = keep;
-    kept[m] = (uint32_t)__popcll(keep);
-}
-
-// Same filter with LPP lanes per point (LPP consecutive lanes of a wave share one point): the point's
-// neighbour coordinates are staged once in LDS ([slot][point], 12 B per slot per point instead of per
-// lane), and lane r tests the slots j = r, r + LPP, ... against all valid m < j. LDS per lane drops LPP-fold,
-// so several times more waves fit on a CU to hide the gather latency; every test is the one of k_egg, so
-// the kept sets are identical. A workgroup of 64 lanes covers 64 / LPP points; 64-point chunks take LPP
-// workgroups.
+// The filter with LPP lanes per point (LPP consecutive lanes of a wave share one point). mpts: current points
+// in Morton order (w = sorted index); knn: sorted-space rows, updated in place; mrow: per Morton position, the
+// k slots mapped to Morton positions (-1 for empty); keepm: kept-slot mask. The point's neighbour coordinates
+// are staged once in LDS ([slot][point], 12 B per slot per point), and lane r tests the slots
+// j = r, r + LPP, ... against all valid m < j. LDS per lane drops LPP-fold, so several times more waves fit on
+// a CU to hide the gather latency. A workgroup of 64 lanes covers 64 / LPP points; the 64-point chunks of a
+// neighbour-count class (chunks[b] = first Morton position, null: 64 b) take LPP workgroups each.
 template <int K, int LPP>
 __global__ void __launch_bounds__(EBLK) k_egg_shared(const float4* __restrict__ mpts, const int32_t* __restrict__ minv,
                                                       int32_t* __restrict__ knn, const CloudDev* __restrict__ clouds,
@@ -352,12 +292,16 @@ void run_egg(egt_ctx* h, float alpha) {
     const int32_t* mcl = h->nclouds > 1 ? h->mcloud.p : nullptr;
     for (const WorkClass& wc : classes) {
         const int64_t* chunks = h->nclouds > 1 ? h->work_lists.p + wc.first : nullptr;
-        const dim3 grid((unsigned)wc.count), grid4((unsigned)(wc.count * 4)), blk(EBLK);
+        const dim3 grid4((unsigned)(wc.count * 4)), grid8((unsigned)(wc.count * 8)),
+            grid16((unsigned)(wc.count * 16)), blk(EBLK);
 #define EGG_ARGS h->mpts.p, h->minv.p, h->knn.p, h->dclouds.p, mcl, chunks, alpha, mrow, keepm, kept
-        if (wc.K <= 16) hipLaunchKernelGGL((k_egg<16>), grid, blk, 0, s, EGG_ARGS);
-        else if (wc.K <= 32) hipLaunchKernelGGL((k_egg_shared<32, 4>), grid4, blk, 0, s, EGG_ARGS);
-        else if (wc.K <= 40) hipLaunchKernelGGL((k_egg_shared<40, 4>), grid4, blk, 0, s, EGG_ARGS);
-        else hipLaunchKernelGGL((k_egg_shared<64, 4>), grid4, blk, 0, s, EGG_ARGS);
+        // lanes per point by list length (measured on a 4M-point torus and a 16M-point mixture: k=16 with 4
+        // lanes 1.11 ms against 1.43 for one lane; k=40 with 8 lanes 3.95 ms against 4.27 with 4 and 4.84 with
+        // 16; k=64 with 16 lanes 6.89 ms against 7.15 with 8 and 10.4 with 4)
+        if (wc.K <= 16) hipLaunchKernelGGL((k_egg_shared<16, 4>), grid4, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 32) hipLaunchKernelGGL((k_egg_shared<32, 8>), grid8, blk, 0, s, EGG_ARGS);
+        else if (wc.K <= 40) hipLaunchKernelGGL((k_egg_shared<40, 8>), grid8, blk, 0, s, EGG_ARGS);
+        else hipLaunchKernelGGL((k_egg_shared<64, 16>), grid16, blk, 0, s, EGG_ARGS);
 #undef EGG_ARGS
         LAUNCH_CHECK();
     }
