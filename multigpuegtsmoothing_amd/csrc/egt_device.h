@@ -173,14 +173,16 @@ struct LevelDev {
     int32_t max_count;
 };
 
+// Level tables: one 16-byte entry per slot, key (low, high word) then (start, count), so a probe reads key and
+// range in one load and an insert writes one cache line.
 // Looks up a level cell; returns (start, count) or count 0 when empty.
-__device__ inline uint2 level_lookup(const uint64_t* __restrict__ hkeys, const uint2* __restrict__ hvals,
-                                     const LevelDev& L, uint64_t key) {
+__device__ inline uint2 level_lookup(const uint4* __restrict__ tab, const LevelDev& L, uint64_t key) {
     const uint32_t mask = (1u << L.log2cap) - 1u;
     uint32_t s = hash_slot(key, L.log2cap);
     for (uint32_t probe = 0; probe <= mask; ++probe) {  // tables are at most half full
-        const uint64_t kk = hkeys[L.offset + s];
-        if (kk == key) return hvals[L.offset + s];
+        const uint4 e = tab[L.offset + s];
+        const uint64_t kk = (uint64_t)e.x | ((uint64_t)e.y << 32);
+        if (kk == key) return make_uint2(e.z, e.w);
         if (kk == HASH_EMPTY) break;
         s = (s + 1u) & mask;
     }

@@ -274,8 +274,7 @@ __global__ void __launch_bounds__(BLK) k_level_starts(const uint32_t* __restrict
 __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict__ starts, int64_t cells,
                                                        int64_t n, const uint32_t* __restrict__ code,
                                                        const int32_t* __restrict__ mcloud, int shift,
-                                                       unsigned long long* __restrict__ hkeys,
-                                                       uint2* __restrict__ hvals, uint64_t offset, uint32_t log2cap,
+                                                       uint4* __restrict__ tab, uint64_t offset, uint32_t log2cap,
                                                        int* __restrict__ maxc) {
     const int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
     int cnt = 0;
@@ -287,9 +286,11 @@ __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict
         uint32_t s = hash_slot(cc, log2cap);
         // a table has at least twice as many slots as cells, so an empty slot is always found
         for (;;) {
-            const unsigned long long prev = atomicCAS(&hkeys[offset + s], (unsigned long long)HASH_EMPTY, cc);
+            uint4* e = &tab[offset + s];
+            const unsigned long long prev =
+                atomicCAS(reinterpret_cast<unsigned long long*>(e), (unsigned long long)HASH_EMPTY, cc);
             if (prev == HASH_EMPTY) {
-                hvals[offset + s] = make_uint2(st, en - st);
+                reinterpret_cast<uint2*>(e)[1] = make_uint2(st, en - st);
                 break;
             }
             s = (s + 1u) & mask;
@@ -613,9 +614,8 @@ void build_search_structure(egt_ctx* h) {
         h->lev[l].cells = cells[l];
         total += 1ull << log2cap;
     }
-    h->hkeys.alloc(total);
-    h->hvals.alloc(total);
-    k_fill_u64<<<1024, BLK, 0, s>>>(h->hkeys.p, total, HASH_EMPTY);
+    h->htab.alloc(total);
+    k_fill_u64<<<1024, BLK, 0, s>>>(reinterpret_cast<uint64_t*>(h->htab.p), 2 * total, HASH_EMPTY);
     LAUNCH_CHECK();
     DBuf<uint32_t>& dmax = sc.bufs[14];
     dmax.alloc(h->nlev);
@@ -627,8 +627,8 @@ void build_search_structure(egt_ctx* h) {
         k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts.p);
         LAUNCH_CHECK();
         k_level_insert<<<std::max(1, ceil_div(cells[l], BLK)), BLK, 0, s>>>(
-            starts.p, cells[l], n, h->mcode.p, mcl, 3 * l, reinterpret_cast<unsigned long long*>(h->hkeys.p),
-            h->hvals.p, h->lev[l].offset, h->lev[l].log2cap, reinterpret_cast<int*>(dmax.p) + l);
+            starts.p, cells[l], n, h->mcode.p, mcl, 3 * l, h->htab.p, h->lev[l].offset, h->lev[l].log2cap,
+            reinterpret_cast<int*>(dmax.p) + l);
         LAUNCH_CHECK();
     }
     std::vector<int32_t> mx(h->nlev);

@@ -130,8 +130,7 @@ struct egt_ctx {
     egt::DBuf<int32_t> morder;   // Morton position -> sorted index
     egt::DBuf<int32_t> minv;     // sorted index -> Morton position
     bool search_stale = false;   // points moved since the search structure was built
-    egt::DBuf<uint64_t> hkeys;   // all level tables (keys: cell_key(cloud, level code))
-    egt::DBuf<uint2> hvals;
+    egt::DBuf<uint4> htab;       // all level tables: (key = cell_key(cloud, level code), start, count) per slot
     egt::LevelTable lev[EGT_MAX_LEVELS];
     int nlev = 0;
     egt::DBuf<egt::LevelDev> dlev;  // device copy of the level descriptors

@@ -50,8 +50,7 @@ struct KnnParams {
     const float4* __restrict__ mpts;
     const uint32_t* __restrict__ mcell;    // fine cell cx | cy << 10 | cz << 20 of each Morton position
     const unsigned long long* __restrict__ mvcell;  // virtual cell + sub code of each Morton position
-    const uint64_t* __restrict__ hkeys;
-    const uint2* __restrict__ hvals;
+    const uint4* __restrict__ htab;
     const int32_t* __restrict__ key;       // sorted-space fine key (member cell coordinates)
     const int8_t* __restrict__ qlevel;     // per-query start level (Morton order) or null
     const CloudDev* __restrict__ clouds;   // per-cloud k, radius, row offset
@@ -135,7 +134,7 @@ __device__ inline bool ref_continue(const KnnParams& P, int kq, float thres_n, i
 __device__ inline uint32_t cell_count(const KnnParams& P, uint32_t cloud, int lvl, int cx, int cy, int cz) {
     const int Bl = P.B >> lvl;
     if (cx < 0 || cy < 0 || cz < 0 || cx >= Bl || cy >= Bl || cz >= Bl) return 0;
-    return level_lookup(P.hkeys, P.hvals, P.lev[lvl], cell_key(cloud, morton_code(cx, cy, cz))).y;
+    return level_lookup(P.htab, P.lev[lvl], cell_key(cloud, morton_code(cx, cy, cz))).y;
 }
 
 // Range of the cell c (coordinates at internal level il < SUB) inside the range [lo, lo + cnt) of an enclosing
@@ -168,9 +167,9 @@ __device__ inline uint2 sub_range(const KnnParams& P, uint32_t lo, uint32_t cnt,
 // table) narrowed by sub_range.
 __device__ inline uint2 cell_range(const KnnParams& P, uint32_t cloud, int il, const int (&c)[3], uint32_t& probes) {
     if (il >= SUB)
-        return level_lookup(P.hkeys, P.hvals, P.lev[il - SUB], cell_key(cloud, morton_code(c[0], c[1], c[2])));
+        return level_lookup(P.htab, P.lev[il - SUB], cell_key(cloud, morton_code(c[0], c[1], c[2])));
     const int sh = SUB - il;
-    const uint2 r = level_lookup(P.hkeys, P.hvals, P.lev[0], cell_key(cloud, morton_code(c[0] >> sh, c[1] >> sh, c[2] >> sh)));
+    const uint2 r = level_lookup(P.htab, P.lev[0], cell_key(cloud, morton_code(c[0] >> sh, c[1] >> sh, c[2] >> sh)));
     if (r.y == 0) return r;
     return sub_range(P, r.x, r.y, il, c, probes);
 }
@@ -389,7 +388,7 @@ __global__ void __launch_bounds__(KBLK)
                         ++cells;
                         const uint2 rc =
                             level - 1 >= SUB
-                                ? level_lookup(P.hkeys, P.hvals, P.lev[level - 1 - SUB],
+                                ? level_lookup(P.htab, P.lev[level - 1 - SUB],
                                                cell_key(cloud, morton_code(cc3[0], cc3[1], cc3[2])))
                                 : sub_range(P, r.x, r.y, level - 1, cc3, cells);
                         start = rc.x;
@@ -599,8 +598,7 @@ KnnParams knn_params(egt_ctx* h) {
     P.mpts = h->mpts.p;
     P.mcell = h->mcell.p;
     P.mvcell = reinterpret_cast<const unsigned long long*>(h->mvcell.p);
-    P.hkeys = h->hkeys.p;
-    P.hvals = h->hvals.p;
+    P.htab = h->htab.p;
     P.key = h->key.p;
     P.qlevel = (h->learned && h->qlevel.p) ? h->qlevel.p : nullptr;
     P.clouds = h->dclouds.p;

@@ -345,8 +345,7 @@ __global__ void __launch_bounds__(MBLK) k_mlp_adam(float* __restrict__ P, float*
 struct FeatParams {
     const float4* mpts;
     const unsigned long long* mvcell;  // virtual cell of each Morton position (fine cell = coordinate >> SUB)
-    const uint64_t* hkeys;
-    const uint2* hvals;
+    const uint4* htab;
     const LevelDev* lev;     // level descriptors (device)
     const CloudDev* clouds;  // per cloud: reference level, neighbour count
     const int32_t* mcloud;   // cloud of each Morton position, null for one cloud
@@ -368,7 +367,7 @@ __device__ inline float feature(const FeatParams& F, int64_t m, int t) {
     const int x = cx + (t % 3) - 1, y = cy + ((t / 3) % 3) - 1, z = cz + (t / 9) - 1;
     uint32_t c = 0;
     if (x >= 0 && y >= 0 && z >= 0 && x < Bl && y < Bl && z < Bl)
-        c = level_lookup(F.hkeys, F.hvals, F.lev[ref], cell_key(cloud, morton_code(x, y, z))).y;
+        c = level_lookup(F.htab, F.lev[ref], cell_key(cloud, morton_code(x, y, z))).y;
     return log2f(1.0f + (float)c) * (1.0f / 16.0f);
 }
 
@@ -478,8 +477,7 @@ FeatParams feat_params(const egt_ctx* h) {
     F.mpts = h->mpts.p;
     F.mvcell = reinterpret_cast<const unsigned long long*>(h->mvcell.p);
     F.B = h->B;
-    F.hkeys = h->hkeys.p;
-    F.hvals = h->hvals.p;
+    F.htab = h->htab.p;
     F.lev = h->dlev.p;
     F.clouds = h->dclouds.p;
     F.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
