@@ -34,9 +34,9 @@ __device__ inline int64_t xcd_tile() {
 // 1-hop halo (neighbours outside the tile, deduplicated and sorted, at most halo_max<TILE>() entries) is
 // listed once per smoothing call, and every CSR column is rewritten as a 16-bit local index
 // (< TILE: inside the tile; TILE + h: halo entry h). The columns are then laid out as sliced ELLPACK: the
-// tile's rows sorted by length, 64 rows (one wave) per slice, slot-major within a slice. A pass stages tile
-// + halo coordinates in LDS and reads all neighbours from LDS; the lanes of a wave walk rows of nearly equal
-// length and each slot load is one coalesced 128-byte line. The pass is limited by VALU issue (the weight's
+// tile's rows sorted by length, 64 rows (one wave) per slice, slot-pair-major within a slice. A pass stages
+// tile + halo coordinates in LDS and reads all neighbours from LDS; the lanes of a wave walk rows of nearly
+// equal length and each load of a slot pair is one coalesced 256-byte block. The pass is limited by VALU issue (the weight's
 // exp per neighbour) and by the latency of its dependent loads; equal-length lanes cut the issued VALU. Only
 // the first halo_cap<TILE>() halo entries fit in LDS; a tile with more reads the rest from global memory
 // through its halo list. Tiles with more than halo_max<TILE>() halo entries use CSR rows with global gathers.
@@ -47,16 +47,16 @@ constexpr int PBLK = 1024;
 // per CU instead of two: 64.2 vs 67.7 ms for 300 iterations on a 4M-point torus.
 template <int TILE>
 constexpr int tile_threads() { return TILE == 1024 ? 512 : 1024; }
-// Largest halo a tile keeps in LDS: (TILE + cap) float4 plus the slice offsets stay 1 KiB under a half (2048)
-// or a quarter (1024) of the CU's 160 KiB, so two 1024-thread or four 512-thread blocks still fit per CU. In
+// Largest halo a tile keeps in LDS: (TILE + cap + 1) float4 stay 1 KiB under a half (2048) or a quarter (1024)
+// of the CU's 160 KiB, so two 1024-thread or four 512-thread blocks still fit per CU. In
 // volumetric regions a 2048-point Morton range has a halo of 2000-3000 points at 16 kept neighbours a point.
 template <int TILE>
 constexpr int halo_cap() { return TILE == 1024 ? 1456 : 2992; }
 // Longest halo list of a tile (the list's stride per tile); beyond it the tile uses CSR rows.
 template <int TILE>
 constexpr int halo_max() { return TILE == 1024 ? 4096 : 8192; }
-static_assert((2048 + halo_cap<2048>()) * 16 + 4 * 33 <= 160 * 1024 / 2 - 1024, "two tile blocks per CU");
-static_assert((1024 + halo_cap<1024>()) * 16 + 4 * 17 <= 160 * 1024 / 4 - 1024, "four tile blocks per CU");
+static_assert((2048 + halo_cap<2048>() + 1) * 16 <= 160 * 1024 / 2 - 1024, "two tile blocks per CU");
+static_assert((1024 + halo_cap<1024>() + 1) * 16 <= 160 * 1024 / 4 - 1024, "four tile blocks per CU");
 __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
     for (int k = 2; k <= n2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -187,16 +187,17 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     for (int s = threadIdx.x; s < TILE / 64; s += PBLK) width[t * (TILE / 64) + s] = swid[s];
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int s = 0; s < TILE / 64; ++s) tot += 64u * swid[s];
+        for (int s = 0; s < TILE / 64; ++s) tot += 128u * ((swid[s] + 1u) >> 1);
         padded[t] = tot;
         hcount[t] = (int32_t)H;
     }
 }
 
 // Sliced-ELLPACK columns of one tile: slice s (sorted positions 64 s .. 64 s + 63) stores its width W_s
-// slots slot-major, slot u of lane l at 64 u + l, so a wave's load of one slot is 128 contiguous bytes.
-// Slots past a row's end hold the row's own local index (a row never lists itself), lanes past the tile's
-// last row hold 0. Slot order within a row is the CSR (neighbour) order.
+// slots as ceil(W_s / 2) slot pairs, pair-major: slots 2p and 2p + 1 of lane l share the 32-bit word
+// 64 p + l (low and high half), so a wave's load of one pair is 256 contiguous bytes. Slots past a row's
+// end (and the high half of an odd width's last pair) hold the row's own local index (a row never lists
+// itself), lanes past the tile's last row hold 0. Slot order within a row is the CSR (neighbour) order.
 template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ rowptr,
                                                      const uint16_t* __restrict__ col16, int64_t n,
@@ -211,10 +212,10 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
     uint16_t* dst = scol + toff[t];
     uint32_t sb = 0;
     for (int s = 0; s < TILE / 64; ++s) {
-        const uint32_t W = width[t * (TILE / 64) + s];
-        for (uint32_t x = threadIdx.x; x < 64u * W; x += PBLK) {
-            const uint32_t u = x >> 6;
-            const int j = s * 64 + (int)(x & 63u);
+        const uint32_t P = (width[t * (TILE / 64) + s] + 1u) >> 1;
+        for (uint32_t x = threadIdx.x; x < 128u * P; x += PBLK) {
+            const uint32_t u = ((x >> 7) << 1) | (x & 1u);
+            const int j = s * 64 + (int)((x >> 1) & 63u);
             uint16_t v = 0;
             if (j < np) {
                 const int li = perm[t * TILE + j];
@@ -223,19 +224,15 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
             }
             dst[sb + x] = v;
         }
-        sb += 64u * W;
+        sb += 128u * P;
     }
 }
 
-// Neighbour loop of one row over LDS coordinates, its local columns at sc[64 u] (sliced ELLPACK, W slots,
-// W uniform over the wave). The columns of CH slots, then their LDS coordinates, are fetched before use, so
-// a chunk costs one global and one LDS latency (CH = 8: 16 slots in flight cost more in VGPRs, and so in
-// occupancy, than they save); accumulation stays in slot order. A padding slot holds the row's own index
-// (self): its distance is 0, its weight is selected to 0 and it adds fma(-0, 0, acc) = acc and sum + 0 = sum,
-// so every lane runs every slot without a branch on the padding. The weighted difference uses
-// w * (q - p) = (-w) * (p - q): q - p is exactly -(p - q), and an accumulator that starts at +0 never
-// becomes -0, so the sign of a zero difference cannot change a sum; the reuse saves three subtractions
-// per neighbour.
+__device__ inline float4 taubin_result(const float4 p, float cgx, float cgy, float cgz, float sum, float scale) {
+    if (sum == 0.0f) sum = 1e-8f;
+    return make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+}
+
 // Neighbour weight: exp(-d^2), or with isRegularized (EGT_CUDA.cu:263, dormant in the reference) 1 in the
 // lambda pass and 1 / (d^2 + 1e-8) in the mu pass.
 template <bool REG>
@@ -244,19 +241,62 @@ __device__ inline float taubin_weight(float dist, float scale) {
     return egt_expf_taubin(-dist);
 }
 
+// The neighbour loop of a wave's two rows over LDS coordinates (sliced ELLPACK, W0 and W1 slots, both uniform
+// over the wave; this lane's slot pair q of row r is the word sb_r[64 q + lane]). The two rows are walked as
+// one sequence of chunks of CH slots: while a chunk's coordinates are read from LDS and its weights
+// computed, the next chunk's columns (CH / 2 packed words, possibly the second row's first ones) are already
+// in flight, so the column loads' global latency overlaps the arithmetic. Two column buffers alternate (the
+// loop is unrolled by two, so no register copy waits on a load in flight) and every prefetch issues all its
+// loads, a word past the row's end reading the valid word dummy[lane] instead, so the wait before a chunk
+// counts exactly the next chunk's loads. Packing two columns per word keeps both buffers in the registers
+// one unpacked chunk took (CH = 8; 16 slots in flight cost more in VGPRs, and so in occupancy, than they
+// save). Accumulation stays in slot order. A padding slot holds the row's own index (self): its distance is
+// 0, its weight is selected to 0 and it adds fma(-0, 0, acc) = acc and sum + 0 = sum, so every lane runs
+// every slot without a branch on the padding. The weighted difference uses w * (q - p) = (-w) * (p - q):
+// q - p is exactly -(p - q), and an accumulator that starts at +0 never becomes -0, so the sign of a zero
+// difference cannot change a sum; the reuse saves three subtractions per neighbour.
 // With OVF, local columns at or past lim (TILE + the LDS part of the halo) are halo entries kept in global
 // memory: their coordinates are gathered from gin through the tile's halo list hal (indexed by column - TILE).
+constexpr int TCH = 8;  // slots per chunk
+// Columns of chunk g of a wave's two rows (row 0's chunk g, or row 1's chunk g - C0; C0 = row 0's chunks)
+// into w; a word past its row's end, and every word of a chunk g >= the chunk count, reads dummy[lane].
+__device__ inline void taubin_fetch(int g, int C0, int W0, int W1, const uint32_t* sb0, const uint32_t* sb1,
+                                    const uint32_t* dummy, int lane, uint32_t (&w)[TCH / 2]) {
+    const bool second = g >= C0;
+    const uint32_t* sb = second ? sb1 : sb0;
+    const int q0 = (second ? g - C0 : g) * (TCH / 2), P = ((second ? W1 : W0) + 1) >> 1;
+#pragma unroll
+    for (int q = 0; q < TCH / 2; ++q) {
+        const uint32_t* b = q0 + q < P ? sb + (q0 + q) * 64 : dummy;  // wave-uniform choice
+        w[q] = b[lane];
+    }
+}
+
+// ba holds chunk 0's columns (taubin_fetch(0, ...)), fetched by the caller.
 template <bool REG, bool OVF>
-__device__ inline void taubin_sell_row(const float4 p, const float4* __restrict__ src, const uint16_t* __restrict__ sc,
-                                       int W, float scale, float& cgx, float& cgy, float& cgz, float& sum,
+__device__ inline void taubin_two_rows(const float4* __restrict__ src, const uint32_t* sb0, const uint32_t* sb1,
+                                       const uint32_t* dummy, int lane, int W0, int W1, int r0, int r1, float scale,
                                        const float4* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
-                                       uint32_t tile, uint32_t self) {
-    constexpr int CH = 8;
-    for (int u0 = 0; u0 < W; u0 += CH) {
+                                       uint32_t tile, uint32_t (&ba)[TCH / 2], float4& res0, float4& res1) {
+    constexpr int CH = TCH, CP = CH / 2;
+    const int C0 = (W0 + CH - 1) / CH, C = C0 + (W1 + CH - 1) / CH;  // chunks of each row (wave-uniform)
+    // own index as the padding slots hold it; lanes past the last row (r = -1) match no column
+    const uint32_t self0 = (uint32_t)r0, self1 = (uint32_t)r1;
+    float4 p = src[r0 < 0 ? 0 : r0];
+    if (C0 == 0) {
+        res0 = taubin_result(p, 0.0f, 0.0f, 0.0f, 0.0f, scale);
+        p = src[r1 < 0 ? 0 : r1];
+    }
+    float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
+    auto fetch = [&](int g, uint32_t(&w)[CP]) { taubin_fetch(g, C0, W0, W1, sb0, sb1, dummy, lane, w); };
+    auto chunk = [&](int g, const uint32_t(&cw)[CP]) {
+        const bool second = g >= C0;
+        const int u0 = (second ? g - C0 : g) * CH, W = second ? W1 : W0;
+        const uint32_t self = second ? self1 : self0;
         uint32_t c[CH];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) c[u] = u0 + u < W ? (uint32_t)sc[(u0 + u) * 64] : 0u;
-        // every read of the chunk is issued before the first use
+        for (int u = 0; u < CH; ++u) c[u] = (u & 1) ? cw[u >> 1] >> 16 : cw[u >> 1] & 0xFFFFu;
+        // every LDS read of the chunk is issued before the first use
         float4 qs[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
@@ -279,12 +319,21 @@ __device__ inline void taubin_sell_row(const float4 p, const float4* __restrict_
                 sum += w;
             }
         }
+        if (g == C0 - 1) {  // row 0 done: the rest of the sequence is row 1
+            res0 = taubin_result(p, cgx, cgy, cgz, sum, scale);
+            cgx = cgy = cgz = sum = 0.0f;
+            p = src[r1 < 0 ? 0 : r1];
+        }
+    };
+    uint32_t bb[CP];
+    for (int g = 0; g < C; g += 2) {
+        fetch(g + 1, bb);
+        chunk(g, ba);
+        if (g + 1 >= C) break;
+        fetch(g + 2, ba);
+        chunk(g + 1, bb);
     }
-}
-
-__device__ inline float4 taubin_result(const float4 p, float cgx, float cgy, float cgz, float sum, float scale) {
-    if (sum == 0.0f) sum = 1e-8f;
-    return make_float4(p.x + scale * cgx / sum, p.y + scale * cgy / sum, p.z + scale * cgz / sum, 0.0f);
+    res1 = taubin_result(p, cgx, cgy, cgz, sum, scale);
 }
 
 // One lambda or mu pass over one tile. Wave w of the block takes slices w and S - 1 - w (S = TILE / 64
@@ -304,9 +353,8 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     constexpr int S = TILE / 64;
     constexpr int TBLK_TILE = tile_threads<TILE>();
     constexpr int NW = TBLK_TILE / 64;
-    static_assert(S <= 64 && S % NW == 0 && (S / NW) <= 2, "slices per wave");
+    static_assert(S <= 64 && S / NW == 2 && S % NW == 0, "two slices per wave");
     extern __shared__ float4 lds[];
-    __shared__ uint32_t sbase[S + 1];
     const int64_t t = xcd_tile();
     if (t >= ntiles) return;
     const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
@@ -331,46 +379,69 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
         return;
     }
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (wv == 0) {  // slice offsets within the tile's column block: exclusive scan of 64 * width
-        uint32_t v = lane < S ? 64u * width[t * S + lane] : 0u;
+    // Every wave scans the tile's slice widths itself (offsets of the slices' column blocks), so its first
+    // chunk of columns can be in flight together with the staging loads, before the barrier.
+    const uint32_t wd = lane < S ? width[t * S + lane] : 0u;
+    uint32_t v = 128u * ((wd + 1u) >> 1);
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up(v, d, 64);
-            if (lane >= d) v += o;
-        }
-        if (lane < S) sbase[lane + 1] = v;
-        if (lane == 0) sbase[0] = 0;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
     }
-    for (int i = threadIdx.x; i < np; i += TBLK_TILE) lds[i] = in[base + i];
-    const int32_t* hal = halo + t * halo_max<TILE>();
-    const int HL = min(H, halo_cap<TILE>());  // halo entries staged in LDS; the rest are read from global
-    for (int h = threadIdx.x; h < HL; h += TBLK_TILE) lds[TILE + h] = in[hal[h]];
-    __syncthreads();
+    v -= 128u * ((wd + 1u) >> 1);  // exclusive
     const uint16_t* tc = scol + toff[t];
-    float4 res[S / NW];
-    int row[S / NW];
+    int row[2];
+    const uint32_t* sb[2];
+    int W[2];
 #pragma unroll
-    for (int r = 0; r < S / NW; ++r) {
+    for (int r = 0; r < 2; ++r) {
         const int s = r == 0 ? wv : S - 1 - wv;
         const int j = s * 64 + lane;
         row[r] = j < np ? (int)perm[t * TILE + j] : -1;
-        const float4 p = lds[row[r] < 0 ? 0 : row[r]];
-        float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
         // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
-        const uint32_t sb0 = __builtin_amdgcn_readfirstlane(sbase[s]);
-        const int W = (int)((__builtin_amdgcn_readfirstlane(sbase[s + 1]) - sb0) >> 6);
-        const uint32_t self = row[r] < 0 ? 0xFFFFFFFFu : (uint32_t)row[r];  // lanes past the last row hold 0
-        if (H > halo_cap<TILE>())
-            taubin_sell_row<REG, true>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE,
-                                       self);
-        else
-            taubin_sell_row<REG, false>(p, lds, tc + sb0 + lane, W, scale, cgx, cgy, cgz, sum, in, hal, TILE + HL, TILE,
-                                        self);
-        res[r] = taubin_result(p, cgx, cgy, cgz, sum, scale);
+        sb[r] = reinterpret_cast<const uint32_t*>(tc + __builtin_amdgcn_readfirstlane(__shfl(v, s, 64)));
+        W[r] = (int)__builtin_amdgcn_readfirstlane(__shfl(wd, s, 64));
+    }
+    // Staging of tile + halo coordinates: every load is issued before the first wait (the halo indices,
+    // then the coordinates), so the stage costs two global round trips, and the wave's first chunk of
+    // columns rides along. Indices are clamped so that every load reads inside its array, and every store
+    // is made (no load is moved into a branch): rows past the tile's end go to their unused LDS rows, halo
+    // slots past the tile's halo to entry TILE + HL, which no column of this tile reads from LDS (the
+    // allocation has one entry more than the largest staged halo).
+    const int32_t* hal = halo + t * halo_max<TILE>();
+    const int HL = min(H, halo_cap<TILE>());  // halo entries staged in LDS; the rest are read from global
+    constexpr int TP = TILE / TBLK_TILE, HP = (halo_cap<TILE>() + TBLK_TILE - 1) / TBLK_TILE;
+    int32_t hidx[HP];
+#pragma unroll
+    for (int k = 0; k < HP; ++k) hidx[k] = hal[min((int)threadIdx.x + k * TBLK_TILE, max(HL - 1, 0))];
+    float4 tv[TP];
+#pragma unroll
+    for (int k = 0; k < TP; ++k) tv[k] = in[base + min((int)threadIdx.x + k * TBLK_TILE, np - 1)];
+    // the first 64 words of the column array (it is allocated with at least that many)
+    const uint32_t* dummy = reinterpret_cast<const uint32_t*>(scol);
+    uint32_t ba[TCH / 2];
+    taubin_fetch(0, (W[0] + TCH - 1) / TCH, W[0], W[1], sb[0], sb[1], dummy, lane, ba);
+    float4 hv[HP];
+#pragma unroll
+    for (int k = 0; k < HP; ++k) hv[k] = in[(int)threadIdx.x + k * TBLK_TILE < HL ? (int64_t)hidx[k] : base];
+#pragma unroll
+    for (int k = 0; k < TP; ++k) lds[threadIdx.x + k * TBLK_TILE] = tv[k];
+#pragma unroll
+    for (int k = 0; k < HP; ++k) {
+        const int h = (int)threadIdx.x + k * TBLK_TILE;
+        lds[TILE + (h < HL ? h : HL)] = hv[k];
     }
     __syncthreads();
+    float4 res[2];
+    if (H > halo_cap<TILE>())
+        taubin_two_rows<REG, true>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], scale, in, hal,
+                                   TILE + HL, TILE, ba, res[0], res[1]);
+    else
+        taubin_two_rows<REG, false>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], scale, in, hal,
+                                    TILE + HL, TILE, ba, res[0], res[1]);
+    __syncthreads();
 #pragma unroll
-    for (int r = 0; r < S / NW; ++r)
+    for (int r = 0; r < 2; ++r)
         if (row[r] >= 0) lds[row[r]] = res[r];
     __syncthreads();
     for (int i = threadIdx.x; i < np; i += TBLK_TILE) out[base + i] = lds[i];
@@ -440,7 +511,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
         tot += pad[t];
     }
     tp.scol_elems = (int64_t)tot;
-    tp.scol = h->scratch.get<uint16_t>(27, std::max<uint64_t>(tot, 64));
+    tp.scol = h->scratch.get<uint16_t>(27, std::max<uint64_t>(tot, 128));  // the pass reads its first 64 words
     HIP_CHECK(hipMemcpyAsync(tp.toff, off.data(), tp.ntiles * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     k_tile_sell<TILE><<<tp.ntiles, PBLK, 0, s>>>(h->rowptr.p, tp.col16, n, tp.perm, tp.width, tp.toff, tp.hcount,
                                                  tp.scol);
@@ -451,7 +522,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
 
 template <int TILE, bool REG>
 void launch_tile(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
-    const size_t lds_bytes = (size_t)(tp.tile + tp.maxH) * sizeof(float4);
+    const size_t lds_bytes = (size_t)(tp.tile + tp.maxH + 1) * sizeof(float4);  // + the halo's spill entry
     const dim3 tgrid(ceil_div(tp.ntiles, NXCD * XCHUNK) * NXCD * XCHUNK), tblk(tile_threads<TILE>());
     k_taubin_tile<TILE, REG><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
                                                                     tp.width, tp.perm, tp.halo, tp.hcount, h->n, scale,
