@@ -266,13 +266,16 @@ __global__ void __launch_bounds__(NBLK) k_bv_compact(const uint32_t* __restrict_
     out[pos[t]] = alive ? alive[t] : t;
 }
 
+// Hooks every root with a lightest edge to the root at the edge's other end. No hook count is kept: a round
+// hooks at least one root exactly when edges between components are alive at its start, which the alive-edge
+// count already tells (a shared counter took one atomic per wave on one address, about 0.7 ms a round on a
+// 4M-point cloud).
 __global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __restrict__ eu,
                                                    const int32_t* __restrict__ ev, const uint8_t* __restrict__ erel,
                                                    const int32_t* __restrict__ comp, const uint8_t* __restrict__ par,
                                                    const uint64_t* __restrict__ best, uint32_t* __restrict__ H,
-                                                   uint8_t* __restrict__ msf, uint32_t* __restrict__ hooks) {
+                                                   uint8_t* __restrict__ msf) {
     const int64_t c = (int64_t)blockIdx.x * NBLK + threadIdx.x;
-    bool hooked = false;
     if (c < n && comp[c] == c) {
         const uint64_t b = best[c];
         if (b != KEY_EMPTY) {
@@ -283,12 +286,9 @@ __global__ void __launch_bounds__(NBLK) k_bv_hook(int64_t n, const int32_t* __re
             if (!(best[other] == b && (int32_t)c < other)) {  // mutual pair: the smaller id stays root
                 H[c] = ((uint32_t)other << 1) | (uint32_t)(par[u] ^ par[v] ^ erel[e]);
                 msf[e] = 1;
-                hooked = true;
             }
         }
     }
-    const uint64_t hm = __ballot(hooked);  // one counter update per wave
-    if (hm && __lane_id() == __ffsll((unsigned long long)hm) - 1) atomicAdd(hooks, (uint32_t)__popcll(hm));
 }
 
 // Follows hook pointers to the root. The hook graph is a forest (edge keys are a strict total order and
@@ -467,8 +467,9 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     HIP_CHECK(hipMemsetAsync(msf, 0, mm, s));
 
     // Boruvka with parity. The alive-edge count lives in device memory, so rounds follow one another without
-    // a host round trip; the host reads the count (to shrink the grids) and the convergence flags every
-    // second round. A round after convergence finds no edge and changes nothing.
+    // a host round trip; every second round the host reads the count (to shrink the grids, and to stop once
+    // no edge joins two components) and the hook walk's overflow flag. A round after convergence finds no
+    // edge and changes nothing.
     k_bv_init<<<gn, blk, 0, s>>>(n, comp, par);
     LAUNCH_CHECK();
     HIP_CHECK(hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), s));
@@ -479,10 +480,9 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     for (int round = 0; round < 64 && bound > 0; ++round) {
         const dim3 ga(std::max(1, ceil_div(bound, NBLK)));
         k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
-        HIP_CHECK(hipMemsetAsync(flags, 0, sizeof(uint32_t), s));  // hooks of this round; flags[1] accumulates
         k_bv_best<<<ga, blk, 0, s>>>(eu, ev, ekey, alive, dcount + cur, comp, best);
         if (round == 0) HIP_CHECK(hipMemcpyAsync(best1, best, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
-        k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf, flags);
+        k_bv_hook<<<gn, blk, 0, s>>>(n, eu, ev, erel, comp, par, best, H, msf);
         k_bv_jump<<<gn, blk, 0, s>>>(n, comp, H, flags + 1);
         k_bv_relabel<<<gn, blk, 0, s>>>(n, comp, par, H);
         k_bv_alive<<<ga, blk, 0, s>>>(eu, ev, alive, dcount + cur, bound, comp, aflag);
@@ -494,13 +494,13 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
         alive = out;
         cur ^= 1;
         if (round & 1) {
-            uint32_t hf[3] = {0, 0, 0};
-            HIP_CHECK(hipMemcpyAsync(hf, flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipMemcpyAsync(hf + 2, dcount + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            uint32_t hf[2] = {0, 0};
+            HIP_CHECK(hipMemcpyAsync(hf, flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(hf + 1, dcount + cur, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
-            if (hf[1]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
-            if (hf[0] == 0 || hf[2] == 0) break;
-            bound = hf[2];
+            if (hf[0]) throw ArgError{EGT_ERR_STATE, "spanning forest: hook walk exceeded n steps"};
+            if (hf[1] == 0) break;  // no edge joins two components: converged
+            bound = hf[1];
         }
     }
     {
