@@ -67,7 +67,9 @@ __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, c
 // pass 0: count per owner u and record which CSR entries emit an edge; pass 1: write v into u's bucket.
 // Walks the Morton-order CSR (the same rows as the sorted-space neighbour lists, holes removed). Entry (i, nb)
 // emits the edge {i, nb} when i < nb, or when i > nb and row nb does not list i (pass 0 scans nb's compact
-// CSR row for that and stores the answer in own[], which pass 1 reads).
+// CSR row for that and stores the answer in own[], which pass 1 reads). Edges with u = i go to the row's own
+// bucket: one atomic per row reserves them (order within a bucket is free: k_edge_sort sorts it); only the
+// edges owned by another row (u = nb) take an atomic each.
 template <int PASS>
 __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                  const int32_t* __restrict__ morder, int64_t n,
@@ -77,25 +79,32 @@ __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowp
     if (m >= n) return;
     const int32_t i = morder[m];
     const int32_t e0 = rowptr[m], e1 = rowptr[m + 1];
-    for (int32_t e = e0; e < e1; ++e) {
-        const int32_t m2 = col[e];
-        const int32_t nb = morder[m2];
-        bool emit;
-        if (PASS == 0) {
-            emit = nb != i;
+    uint32_t local = 0;  // edges of this row with u = i
+    if (PASS == 0) {
+        for (int32_t e = e0; e < e1; ++e) {
+            const int32_t m2 = col[e];
+            const int32_t nb = morder[m2];
+            bool emit = nb != i;
             if (emit && i > nb) {
                 bool listed = false;  // does row nb list i? then row nb owns the edge
                 for (int32_t f = rowptr[m2], f1 = rowptr[m2 + 1]; f < f1; ++f) listed = listed || col[f] == (int32_t)m;
                 emit = !listed;
+                if (emit) atomicAdd(&cnt[nb], 1u);
+            } else if (emit) {
+                ++local;
             }
             own[e] = emit ? 1 : 0;
-        } else {
-            emit = own[e] != 0;
         }
-        if (!emit) continue;
-        const int32_t u = i < nb ? i : nb, v = i < nb ? nb : i;
-        const uint32_t p = atomicAdd(&cnt[u], 1u);
-        if (PASS == 1) ev[p] = v;
+        if (local) atomicAdd(&cnt[i], local);
+    } else {
+        for (int32_t e = e0; e < e1; ++e) local += own[e] != 0 && morder[col[e]] > i;
+        uint32_t p = local ? atomicAdd(&cnt[i], local) : 0u;
+        for (int32_t e = e0; e < e1; ++e) {
+            if (!own[e]) continue;
+            const int32_t nb = morder[col[e]];
+            if (nb > i) ev[p++] = nb;
+            else ev[atomicAdd(&cnt[nb], 1u)] = i;
+        }
     }
 }
 
@@ -220,26 +229,30 @@ __device__ inline void block_atomic_min(uint64_t* base, int32_t idx, uint64_t va
     }
 }
 
-// The alive-edge count is read from device memory (*mcount); the grid covers an upper bound of it.
+// The alive-edge count is read from device memory (*mcount). The edge kernels of a round loop over it with a
+// grid capped at BV_GRID blocks, so a round with few edges left costs a few microseconds, not a launch over
+// the host's stale bound.
+constexpr int BV_GRID = 4096;
 __global__ void __launch_bounds__(NBLK) k_bv_best(const int32_t* __restrict__ eu, const int32_t* __restrict__ ev,
                                                    const uint64_t* __restrict__ ekey, const uint32_t* __restrict__ alive,
                                                    const uint32_t* __restrict__ mcount, const int32_t* __restrict__ comp,
                                                    uint64_t* __restrict__ best) {
-    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
     const uint32_t m = *mcount;
-    if (blockIdx.x * NBLK >= m) return;  // whole block past the count (uniform)
-    int32_t cu = 0, cv = 0;
-    uint64_t key = 0;
-    bool act = t < m;
-    if (act) {
-        const uint32_t e = alive ? alive[t] : t;
-        cu = comp[eu[e]];
-        cv = comp[ev[e]];
-        act = cu != cv;
-        key = ekey[e];
+    for (uint32_t t0 = blockIdx.x * NBLK; t0 < m; t0 += gridDim.x * NBLK) {  // block-uniform bound
+        const uint32_t t = t0 + threadIdx.x;
+        int32_t cu = 0, cv = 0;
+        uint64_t key = 0;
+        bool act = t < m;
+        if (act) {
+            const uint32_t e = alive ? alive[t] : t;
+            cu = comp[eu[e]];
+            cv = comp[ev[e]];
+            act = cu != cv;
+            key = ekey[e];
+        }
+        wave_atomic_min(best, cu, key, act);
+        wave_atomic_min(best, cv, key, act);
     }
-    wave_atomic_min(best, cu, key, act);
-    wave_atomic_min(best, cv, key, act);
 }
 
 // Edges still joining two components after this round's merge (the rest can never be chosen again). Flags
@@ -248,23 +261,24 @@ __global__ void __launch_bounds__(NBLK) k_bv_alive(const int32_t* __restrict__ e
                                                     const uint32_t* __restrict__ alive,
                                                     const uint32_t* __restrict__ mcount, uint32_t bound,
                                                     const int32_t* __restrict__ comp, uint32_t* __restrict__ flag) {
-    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
-    if (t >= bound) return;
-    uint32_t f = 0;
-    if (t < *mcount) {
-        const uint32_t e = alive ? alive[t] : t;
-        f = comp[eu[e]] != comp[ev[e]] ? 1u : 0u;
+    const uint32_t m = *mcount;
+    for (uint32_t t = blockIdx.x * NBLK + threadIdx.x; t < bound; t += gridDim.x * NBLK) {
+        uint32_t f = 0;
+        if (t < m) {
+            const uint32_t e = alive ? alive[t] : t;
+            f = comp[eu[e]] != comp[ev[e]] ? 1u : 0u;
+        }
+        flag[t] = f;
     }
-    flag[t] = f;
 }
 
 __global__ void __launch_bounds__(NBLK) k_bv_compact(const uint32_t* __restrict__ alive,
                                                       const uint32_t* __restrict__ mcount,
                                                       const uint32_t* __restrict__ flag,
                                                       const uint32_t* __restrict__ pos, uint32_t* __restrict__ out) {
-    const uint32_t t = blockIdx.x * NBLK + threadIdx.x;
-    if (t >= *mcount || !flag[t]) return;
-    out[pos[t]] = alive ? alive[t] : t;
+    const uint32_t m = *mcount;
+    for (uint32_t t = blockIdx.x * NBLK + threadIdx.x; t < m; t += gridDim.x * NBLK)
+        if (flag[t]) out[pos[t]] = alive ? alive[t] : t;
 }
 
 // Hooks every root with a lightest edge to the root at the edge's other end. No hook count is kept: a round
@@ -479,7 +493,7 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     uint32_t bound = m;         // host-side upper bound of the alive count
     int cur = 0;
     for (int round = 0; round < 64 && bound > 0; ++round) {
-        const dim3 ga(std::max(1, ceil_div(bound, NBLK)));
+        const dim3 ga(std::min(BV_GRID, std::max(1, ceil_div(bound, NBLK))));
         k_bv_reset<<<gn, blk, 0, s>>>(n, best, H);
         k_bv_best<<<ga, blk, 0, s>>>(eu, ev, ekey, alive, dcount + cur, comp, best);
         if (round == 0) HIP_CHECK(hipMemcpyAsync(best1, best, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
