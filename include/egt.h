@@ -83,7 +83,7 @@ typedef enum {
     EGT_OPT_MAX_PTS_IN_CUBE = 3, /* kNN validity rule: cell population that invalidates a query (1e5) */
     EGT_OPT_KNN_PERSISTENT = 4,  /* kNN: 1 = one persistent work-queue launch per neighbour-count class,
                                     0 = one workgroup per 256 queries (default) */
-    EGT_OPT_TAUBIN_TILE = 5,     /* Taubin: points per LDS tile, 1024 or 2048 (default); 0 = autotune on the
+    EGT_OPT_TAUBIN_TILE = 5,     /* Taubin: points per LDS tile, 1024 (default) or 2048; 0 = autotune on the
                                     next smoothing call (times both on copies, keeps the faster; the
                                     per-pass times land in egt_stats.taubin_tile_ms) */
     EGT_OPT_TAUBIN_REGULARIZED = 6 /* Taubin weights: 0 = exp(-d^2) (default, as the reference runs); 1 = the

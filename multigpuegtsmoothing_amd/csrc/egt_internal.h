@@ -174,7 +174,7 @@ struct egt_ctx {
     int learned = 0;
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;
-    int taubin_tile = 2048;  // Taubin tile width: 1024 or 2048 points; 0 = tune on the next smoothing call
+    int taubin_tile = 1024;  // Taubin tile width: 1024 or 2048 points; 0 = tune on the next smoothing call
     int taubin_regularized = 0;  // isRegularized weights (1 / inverse-square), off as in the reference
     int maxc = 100000;
 

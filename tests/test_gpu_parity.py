@@ -122,6 +122,7 @@ def test_taubin_volumetric_cloud_bit_exact(egt_mod, oracle_mod):
     pts = (p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy())
     k, iters, alpha = 64, 4, 1e-8
     g = _cloud(E, pts, k)
+    g.set_option(E.OPT_TAUBIN_TILE, 2048)
     E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, alpha)
     out = E.getOutputPoints(g)
     st = g.stats()
