@@ -583,10 +583,11 @@ void build_search_structure(egt_ctx* h) {
     h->nlev = log2B + 1;
     DBuf<uint32_t>& flag = sc.bufs[9];
     DBuf<uint32_t>& seg = sc.bufs[10];
-    DBuf<uint32_t>& starts = sc.bufs[11];
     flag.alloc(n);
     seg.alloc(n);
-    starts.alloc(n);
+    // first Morton position of every cell, per level: kept from the counting pass for the inserts, so the
+    // flags and their scan run once per level
+    std::vector<DBuf<uint32_t>> starts(h->nlev);
     std::vector<uint32_t> cells(h->nlev);
     uint64_t total = 0;
     const int nc = h->nclouds;
@@ -597,6 +598,9 @@ void build_search_structure(egt_ctx* h) {
         k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
         LAUNCH_CHECK();
         exclusive_scan_u32(flag.p, seg.p, n, s, sc, &cells[l]);
+        starts[l].alloc(cells[l]);
+        k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts[l].p);
+        LAUNCH_CHECK();
         if (batch) {
             // cells of each cloud at this level (the running count at the cloud boundaries)
             k_cells_at_bases<<<ceil_div(nc + 1, BLK), BLK, 0, s>>>(seg.p, h->dclouds.p, nc, cells[l], dbase_cells);
@@ -621,13 +625,8 @@ void build_search_structure(egt_ctx* h) {
     dmax.alloc(h->nlev);
     HIP_CHECK(hipMemsetAsync(dmax.p, 0, h->nlev * sizeof(uint32_t), s));
     for (int l = 0; l < h->nlev; ++l) {
-        k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
-        LAUNCH_CHECK();
-        exclusive_scan_u32(flag.p, seg.p, n, s, sc, nullptr);
-        k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts.p);
-        LAUNCH_CHECK();
         k_level_insert<<<std::max(1, ceil_div(cells[l], BLK)), BLK, 0, s>>>(
-            starts.p, cells[l], n, h->mcode.p, mcl, 3 * l, h->htab.p, h->lev[l].offset, h->lev[l].log2cap,
+            starts[l].p, cells[l], n, h->mcode.p, mcl, 3 * l, h->htab.p, h->lev[l].offset, h->lev[l].log2cap,
             reinterpret_cast<int*>(dmax.p) + l);
         LAUNCH_CHECK();
     }
