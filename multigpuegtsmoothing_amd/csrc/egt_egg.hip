@@ -196,23 +196,44 @@ __global__ void __launch_bounds__(EBLK) k_egg_shared(const float4* __restrict__ 
     }
 }
 
+// One wave per row, CSR_ROWS consecutive rows per wave: lane j reads slot j of the row (one contiguous load
+// of the k slots) and writes it, when kept, at the row's offset plus the number of kept slots below j, so the
+// row's kept columns go out as one contiguous store. (A thread per row read and wrote its own row's slots, a
+// cache line per lane and per slot.) The rows' offsets, keep masks and row starts are loaded once, lane r
+// holding row m0 + r, and read back per row with readlane, so the rows' slot loads do not wait on each other.
+constexpr int CSR_ROWS = 16;
 __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mrow, const uint64_t* __restrict__ keepm,
                                                   int64_t n, const CloudDev* __restrict__ clouds,
                                                   const int32_t* __restrict__ mcloud, const uint32_t* __restrict__ off,
                                                   int32_t* __restrict__ rowptr, int32_t* __restrict__ col,
                                                   uint32_t total) {
-    const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (m >= n) return;
-    uint32_t o = off[m];
-    rowptr[m] = (int32_t)o;
-    if (m == n - 1) rowptr[n] = (int32_t)total;
-    uint64_t keep = keepm[m];
-    const CloudDev& cd = clouds[mcloud ? mcloud[m] : 0];
-    const int32_t* mr = mrow + cd.row_off + (m - cd.base) * cd.k;
-    while (keep) {
-        const int j = __builtin_ctzll(keep);
-        keep &= keep - 1ull;
-        col[o++] = mr[j];
+    const int lane = __lane_id();
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t m0 = w * CSR_ROWS;
+    if (m0 >= n) return;  // wave-uniform
+    const int nr = (int)min<int64_t>(CSR_ROWS, n - m0);
+    const int64_t mr = m0 + min(lane & (CSR_ROWS - 1), nr - 1);
+    const uint32_t o_l = off[mr];
+    const uint64_t keep_l = keepm[mr];
+    const CloudDev& cd = clouds[mcloud ? mcloud[mr] : 0];
+    const int64_t rb_l = cd.row_off + (mr - cd.base) * cd.k;
+    const int k_l = cd.k;
+    if (lane < nr) rowptr[m0 + lane] = (int32_t)o_l;
+    if (m0 + nr == n && lane == 0) rowptr[n] = (int32_t)total;
+    const uint64_t lt = (1ull << lane) - 1ull;  // lanes below this one
+#pragma unroll
+    for (int i = 0; i < CSR_ROWS; ++i) {
+        // rows past the end repeat the last row: the same values to the same places, and no branch in the loop
+        const int r = min(i, nr - 1);
+        const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)o_l, r);
+        const uint64_t keep = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)keep_l, r)) |
+                              ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(keep_l >> 32), r) << 32);
+        const int64_t rb = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb_l, r)) |
+                                     ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rb_l >> 32), r)
+                                      << 32));
+        const int k = __builtin_amdgcn_readlane(k_l, r);
+        const int32_t v = mrow[rb + min(lane, k - 1)];
+        if ((keep >> lane) & 1ull) col[o + (uint32_t)__popcll(keep & lt)] = v;
     }
 }
 
@@ -246,7 +267,7 @@ void finish_csr(egt_ctx* h, const int32_t* mrow, const uint64_t* keepm, const ui
     exclusive_scan_u32(kept, off, n, h->stream, h->scratch, &total);
     h->rowptr.alloc(n + 1);
     h->col.alloc(std::max<uint32_t>(total, 1));
-    k_csr_fill<<<ceil_div(n, 256), 256, 0, h->stream>>>(mrow, keepm, n, h->dclouds.p,
+    k_csr_fill<<<ceil_div(n, 4 * CSR_ROWS), 256, 0, h->stream>>>(mrow, keepm, n, h->dclouds.p,
                                                          h->nclouds > 1 ? h->mcloud.p : nullptr, off, h->rowptr.p,
                                                          h->col.p, total);
     LAUNCH_CHECK();
