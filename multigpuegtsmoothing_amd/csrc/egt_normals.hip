@@ -26,7 +26,8 @@ constexpr int NBLK = 256;
 constexpr uint64_t KEY_EMPTY = ~0ull;
 
 // One thread per Morton position; neighbours from the Morton-order CSR (slot order of the rows), points
-// from the Morton float4 copy. Writes the normal of sorted point s = mpts[m].w.
+// from the Morton float4 copy. Writes the normal of sorted point s = mpts[m].w. The neighbour loops fetch four
+// neighbours at a time (indices clamped to the row, so every load is valid) and add them in slot order.
 __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, const int32_t* __restrict__ rowptr,
                                                const int32_t* __restrict__ col, int64_t n, float* __restrict__ nx,
                                                float* __restrict__ ny, float* __restrict__ nz) {
@@ -37,24 +38,38 @@ __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, c
     const int32_t e0 = rowptr[m], e1 = rowptr[m + 1];
     float sx = p.x, sy = p.y, sz = p.z;
     int counter = 1;
-    for (int32_t e = e0; e < e1; ++e) {
-        const float4 q = mpts[col[e]];
-        sx += q.x;
-        sy += q.y;
-        sz += q.z;
-        ++counter;
+    for (int32_t e = e0; e < e1; e += 4) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = mpts[col[min(e + u, e1 - 1)]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (e + u < e1) {
+                sx += q[u].x;
+                sy += q[u].y;
+                sz += q[u].z;
+                ++counter;
+            }
+        }
     }
     const float cx = sx / (float)counter, cy = sy / (float)counter, cz = sz / (float)counter;
     float a00 = 0, a01 = 0, a02 = 0, a11 = 0, a12 = 0, a22 = 0;
-    for (int32_t e = e0; e < e1; ++e) {
-        const float4 q = mpts[col[e]];
-        const float dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
-        a00 = __builtin_fmaf(dx, dx, a00);
-        a01 = __builtin_fmaf(dx, dy, a01);
-        a02 = __builtin_fmaf(dx, dz, a02);
-        a11 = __builtin_fmaf(dy, dy, a11);
-        a12 = __builtin_fmaf(dy, dz, a12);
-        a22 = __builtin_fmaf(dz, dz, a22);
+    for (int32_t e = e0; e < e1; e += 4) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = mpts[col[min(e + u, e1 - 1)]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (e + u < e1) {
+                const float dx = q[u].x - cx, dy = q[u].y - cy, dz = q[u].z - cz;
+                a00 = __builtin_fmaf(dx, dx, a00);
+                a01 = __builtin_fmaf(dx, dy, a01);
+                a02 = __builtin_fmaf(dx, dz, a02);
+                a11 = __builtin_fmaf(dy, dy, a11);
+                a12 = __builtin_fmaf(dy, dz, a12);
+                a22 = __builtin_fmaf(dz, dz, a22);
+            }
+        }
     }
     float vx, vy, vz;
     eig3_smallest(a00, a01, a02, a11, a12, a22, vx, vy, vz);
@@ -68,13 +83,13 @@ __global__ void __launch_bounds__(NBLK) k_pca(const float4* __restrict__ mpts, c
 // Walks the Morton-order CSR (the same rows as the sorted-space neighbour lists, holes removed). Entry (i, nb)
 // emits the edge {i, nb} when i < nb, or when i > nb and row nb does not list i (pass 0 scans nb's compact
 // CSR row for that and stores the answer in own[], which pass 1 reads). Edges with u = i go to the row's own
-// bucket: one atomic per row reserves them (order within a bucket is free: k_edge_sort sorts it); only the
-// edges owned by another row (u = nb) take an atomic each.
+// bucket: one atomic per row reserves them (order within a bucket is free: k_edge_sort sorts it; pass 0 keeps
+// their count per row in lcount[]); only the edges owned by another row (u = nb) take an atomic each.
 template <int PASS>
 __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                  const int32_t* __restrict__ morder, int64_t n,
                                                  uint32_t* __restrict__ cnt, uint8_t* __restrict__ own,
-                                                 int32_t* __restrict__ ev) {
+                                                 uint8_t* __restrict__ lcount, int32_t* __restrict__ ev) {
     const int64_t m = (int64_t)blockIdx.x * NBLK + threadIdx.x;
     if (m >= n) return;
     const int32_t i = morder[m];
@@ -95,9 +110,10 @@ __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowp
             }
             own[e] = emit ? 1 : 0;
         }
+        lcount[m] = (uint8_t)local;  // a row has at most 64 entries
         if (local) atomicAdd(&cnt[i], local);
     } else {
-        for (int32_t e = e0; e < e1; ++e) local += own[e] != 0 && morder[col[e]] > i;
+        local = lcount[m];
         uint32_t p = local ? atomicAdd(&cnt[i], local) : 0u;
         for (int32_t e = e0; e < e1; ++e) {
             if (!own[e]) continue;
@@ -437,9 +453,10 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     // canonical undirected edge list
     uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
     uint32_t* off = h->scratch.get<uint32_t>(10, n);
-    uint8_t* own = h->scratch.get<uint8_t>(11, std::max<int64_t>(h->nnz, 1));
+    uint8_t* own = h->scratch.get<uint8_t>(11, std::max<int64_t>(h->nnz, 1) + n);
+    uint8_t* lcount = own + std::max<int64_t>(h->nnz, 1);  // per Morton position: edges the row owns
     HIP_CHECK(hipMemsetAsync(cnt, 0, n * sizeof(uint32_t), s));
-    k_edges<0><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, nullptr);
+    k_edges<0><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, lcount, nullptr);
     LAUNCH_CHECK();
     uint32_t m = 0;
     exclusive_scan_u32(cnt, off, n, s, h->scratch, &m);
@@ -470,7 +487,7 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     uint32_t* part_c = dalloc<uint32_t>(pool, po, 1024);
     if (po > pool.n) throw ArgError{EGT_ERR_STATE, "normals pool overflow"};
 
-    k_edges<1><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, ev);
+    k_edges<1><<<gn, blk, 0, s>>>(h->rowptr.p, h->col.p, h->morder.p, n, cnt, own, lcount, ev);
     LAUNCH_CHECK();
     k_edge_sort<<<gn, blk, 0, s>>>(off, n, m, ev, eu);
     LAUNCH_CHECK();
