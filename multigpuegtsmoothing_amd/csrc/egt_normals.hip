@@ -102,7 +102,14 @@ __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowp
             bool emit = nb != i;
             if (emit && i > nb) {
                 bool listed = false;  // does row nb list i? then row nb owns the edge
-                for (int32_t f = rowptr[m2], f1 = rowptr[m2 + 1]; f < f1; ++f) listed = listed || col[f] == (int32_t)m;
+                const int32_t f1 = rowptr[m2 + 1];
+                for (int32_t f = rowptr[m2]; f < f1; f += 4) {  // four entries per step, clamped to the row
+                    int32_t c4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) c4[u] = col[min(f + u, f1 - 1)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) listed = listed || c4[u] == (int32_t)m;
+                }
                 emit = !listed;
                 if (emit) atomicAdd(&cnt[nb], 1u);
             } else if (emit) {
