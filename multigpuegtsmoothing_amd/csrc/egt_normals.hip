@@ -136,6 +136,34 @@ __global__ void __launch_bounds__(NBLK) k_edge_sort(const uint32_t* __restrict__
     const int64_t u = (int64_t)blockIdx.x * NBLK + threadIdx.x;
     if (u >= n) return;
     const uint32_t b = off[u], e = u + 1 < n ? off[u + 1] : m;
+    if (e - b <= 16) {  // the common case: sorted in registers by a 16-wide bitonic network, padded with INT_MAX
+        int32_t r[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) r[i] = b + i < e ? ev[b + i] : INT32_MAX;
+#pragma unroll
+        for (int k = 2; k <= 16; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const int32_t x = r[i], y = r[l];
+                        const bool up = (i & k) == 0;
+                        r[i] = up ? min(x, y) : max(x, y);
+                        r[l] = up ? max(x, y) : min(x, y);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (b + i < e) {
+                ev[b + i] = r[i];
+                eu[b + i] = (int32_t)u;
+            }
+        return;
+    }
     for (uint32_t a = b + 1; a < e; ++a) {  // insertion sort of the bucket by v
         const int32_t t = ev[a];
         uint32_t c = a;
