@@ -95,38 +95,56 @@ __global__ void __launch_bounds__(NBLK) k_edges(const int32_t* __restrict__ rowp
     const int32_t i = morder[m];
     const int32_t e0 = rowptr[m], e1 = rowptr[m + 1];
     uint32_t local = 0;  // edges of this row with u = i
+    // the row is walked four entries per step, their loads issued together (indices clamped to the row)
     if (PASS == 0) {
-        for (int32_t e = e0; e < e1; ++e) {
-            const int32_t m2 = col[e];
-            const int32_t nb = morder[m2];
-            bool emit = nb != i;
-            if (emit && i > nb) {
-                bool listed = false;  // does row nb list i? then row nb owns the edge
-                const int32_t f1 = rowptr[m2 + 1];
-                for (int32_t f = rowptr[m2]; f < f1; f += 4) {  // four entries per step, clamped to the row
-                    int32_t c4[4];
+        for (int32_t e = e0; e < e1; e += 4) {
+            int32_t m2v[4], nbv[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) c4[u] = col[min(f + u, f1 - 1)];
+            for (int u = 0; u < 4; ++u) m2v[u] = col[min(e + u, e1 - 1)];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) listed = listed || c4[u] == (int32_t)m;
+            for (int u = 0; u < 4; ++u) nbv[u] = morder[m2v[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (e + u >= e1) continue;
+                const int32_t m2 = m2v[u], nb = nbv[u];
+                bool emit = nb != i;
+                if (emit && i > nb) {
+                    bool listed = false;  // does row nb list i? then row nb owns the edge
+                    const int32_t f1 = rowptr[m2 + 1];
+                    for (int32_t f = rowptr[m2]; f < f1; f += 4) {
+                        int32_t c4[4];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) c4[v] = col[min(f + v, f1 - 1)];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) listed = listed || c4[v] == (int32_t)m;
+                    }
+                    emit = !listed;
+                    if (emit) atomicAdd(&cnt[nb], 1u);
+                } else if (emit) {
+                    ++local;
                 }
-                emit = !listed;
-                if (emit) atomicAdd(&cnt[nb], 1u);
-            } else if (emit) {
-                ++local;
+                own[e + u] = emit ? 1 : 0;
             }
-            own[e] = emit ? 1 : 0;
         }
         lcount[m] = (uint8_t)local;  // a row has at most 64 entries
         if (local) atomicAdd(&cnt[i], local);
     } else {
         local = lcount[m];
         uint32_t p = local ? atomicAdd(&cnt[i], local) : 0u;
-        for (int32_t e = e0; e < e1; ++e) {
-            if (!own[e]) continue;
-            const int32_t nb = morder[col[e]];
-            if (nb > i) ev[p++] = nb;
-            else ev[atomicAdd(&cnt[nb], 1u)] = i;
+        for (int32_t e = e0; e < e1; e += 4) {
+            uint8_t ow[4];
+            int32_t nbv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ow[u] = own[min(e + u, e1 - 1)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) nbv[u] = morder[col[min(e + u, e1 - 1)]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (e + u >= e1 || !ow[u]) continue;
+                const int32_t nb = nbv[u];
+                if (nb > i) ev[p++] = nb;
+                else ev[atomicAdd(&cnt[nb], 1u)] = i;
+            }
         }
     }
 }
