@@ -223,30 +223,47 @@ __global__ void __launch_bounds__(NBLK) k_bv_reset(int64_t n, uint64_t* __restri
     H[v] = (uint32_t)v << 1;
 }
 
-// atomicMin(base[idx], val) for the active lanes of a wave, one atomic per run of consecutive lanes with the
-// same target. Edges come in edge-id order, so lanes that share an endpoint's component form runs; a wave
-// with no such neighbours (the common case for the second endpoint in the first rounds) issues its atomics
-// directly. Otherwise a segmented inclusive min-scan over the lanes (log2(64) steps, a step merging the lane
-// d below when it has the same target) leaves each run's minimum in its last lane, which alone issues the
-// atomic. Lanes that share a target without being adjacent may merge too, which is harmless: their values
-// go to the same address. Call from all lanes.
+// atomicMin(base[idx], val) for the active lanes of a wave. Edges come in edge-id order, so lanes that share
+// an endpoint's component often form runs: when a wave has any, a segmented inclusive min-scan over the lanes
+// (log2(64) steps, a step merging the lane d below when it has the same target) leaves each run's minimum in
+// its last lane, the only one of the run still to issue. Lanes that share a target without being adjacent may
+// merge too, which is harmless: their values go to the same address. Late in Boruvka most edges join a few
+// large components, so the remaining lanes often share a few targets without being adjacent: the target of
+// the first remaining lane is then reduced over the wave and issued once, for up to 4 targets, as long as
+// each is shared by at least two lanes; the rest issue their own atomics. Call from all lanes.
 __device__ inline void wave_atomic_min(uint64_t* base, int32_t idx, uint64_t val, bool active) {
     const int lane = __lane_id();
     const int32_t k = active ? idx : -1 - lane;  // inactive lanes: distinct keys that match no target
-    const int32_t kp = __shfl_up(k, 1, 64);
-    if (__ballot(active && lane > 0 && kp == k) == 0) {
-        if (active) atomicMin((unsigned long long*)&base[idx], (unsigned long long)val);
-        return;
-    }
     unsigned long long v = active ? (unsigned long long)val : ~0ull;
+    bool issue = active;
+    const int32_t kp = __shfl_up(k, 1, 64);
+    if (__ballot(active && lane > 0 && kp == k) != 0) {
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int32_t ok = __shfl_up(k, d, 64);
-        const unsigned long long ov = __shfl_up(v, d, 64);
-        if (lane >= d && ok == k && ov < v) v = ov;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t ok = __shfl_up(k, d, 64);
+            const unsigned long long ov = __shfl_up(v, d, 64);
+            if (lane >= d && ok == k && ov < v) v = ov;
+        }
+        const int32_t kn = __shfl_down(k, 1, 64);
+        issue = active && (lane == 63 || kn != k);
     }
-    const int32_t kn = __shfl_down(k, 1, 64);
-    if (active && (lane == 63 || kn != k)) atomicMin((unsigned long long*)&base[idx], v);
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t amask = __ballot(issue);
+        if (!amask) return;
+        const int leader = __ffsll((unsigned long long)amask) - 1;
+        const int32_t lk = __shfl(k, leader, 64);
+        const bool mine = issue && k == lk;
+        if (__popcll(__ballot(mine)) < 2) break;
+        unsigned long long t = mine ? v : ~0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long o = __shfl_xor(t, d, 64);
+            t = o < t ? o : t;
+        }
+        if (lane == leader) atomicMin((unsigned long long*)&base[lk], t);
+        issue = issue && !mine;
+    }
+    if (issue) atomicMin((unsigned long long*)&base[idx], v);
 }
 
 // Block-level variant: when all active lanes of the block target one address, one atomic per block.
