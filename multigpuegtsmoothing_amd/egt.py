@@ -413,15 +413,38 @@ class Normal_CUDA:
 
     def GetNormals(self):
         n = self.provider.pointsRefCount()
-        nx, ny, nz = (np.empty(n, np.float32) for _ in range(3))
+        nx, ny, nz = (_host_out(n) for _ in range(3))
         _check(_lib.egt_normals(self.provider.handle, nx, ny, nz))
         return nx, ny, nz
+
+
+_PINNED = None
+
+
+def _host_out(n):
+    """float32 output array for a device-to-host copy of n values. With a GPU present it is page-locked memory
+    from torch's caching pinned allocator (plumbing only), so the library's copy runs as one DMA instead of
+    through the runtime's staging of pageable memory (48 MB of normals: about 1 ms instead of 3); otherwise a
+    plain numpy array. The array keeps its memory alive."""
+    global _PINNED
+    if _PINNED is None:
+        try:
+            import torch
+
+            _PINNED = torch.cuda.is_available()
+        except Exception:  # no torch: pageable output
+            _PINNED = False
+    if _PINNED:
+        import torch
+
+        return torch.empty(max(n, 1), dtype=torch.float32, pin_memory=True).numpy()[:n]
+    return np.empty(n, np.float32)
 
 
 def getOutputPoints(provider):
     """De-normalized points in original order (getOutputPoints.cu:34-63)."""
     n = provider.pointsRefCount()
-    x, y, z = (np.empty(n, np.float32) for _ in range(3))
+    x, y, z = (_host_out(n) for _ in range(3))
     _check(_lib.egt_get_output_points(provider.handle, x, y, z))
     return x, y, z
 
