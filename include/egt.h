@@ -13,6 +13,15 @@
  *    Device buffers handed out by the accessors are in that space; host outputs are in original order.
  *  - Every call returns an egt_status; egt_last_error_string() describes the last failure on this thread.
  *  - num_cards must be 1: the whole point set lives on one GPU.
+ *  - Limits (each refused with EGT_ERR_UNSUPPORTED and a message, never wrapped silently):
+ *      k <= 64 at every search entry point. The reference takes any k (its lists live in global memory,
+ *        GridKNNCuda.cu:311-339; GridMeshCuda1.cuh:17); here a query's list is held in registers.
+ *      Kept neighbour slots (the EGG adjacency, or every valid slot when normals follow the search directly)
+ *        <= 2^31 - 1 per handle, all clouds together: the adjacency is a CSR with int32 offsets. egt_smooth /
+ *        egt_normals check the exact count after the filter; when it does not fit they return the error with
+ *        the rows already EGG-filtered (the filter's in-place side effect, EGT_CUDA.cu:185) and no smoothing
+ *        done.
+ *      Mesh triangles <= (2^31 - 1) / 6 (int32 corner-list offsets); points < 2^31.
  */
 #ifndef EGT_H
 #define EGT_H

@@ -184,6 +184,7 @@ egt_status egt_mesh_create(const float* x, const float* y, const float* z, int64
     egt_status st = check_common(n, num_cards, bucket_count, out);
     if (st != EGT_OK) return st;
     if (ntri < 0 || (ntri > 0 && !tri)) return fail(EGT_ERR_INVALID_ARG, "bad triangle array");
+    if (const char* err = egt::mesh_triangles_error(ntri)) return fail(EGT_ERR_UNSUPPORTED, err);
     st = make_handle(x, y, z, n, eps, bucket_count, false, 1, out);
     if (st != EGT_OK) return st;
     egt_handle h = *out;

@@ -434,9 +434,7 @@ void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     Scratch& sc = h->scratch;
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
 
     // bounding cube
@@ -475,8 +473,6 @@ void build_grid(egt_ctx* h, const float* dx, const float* dy, const float* dz) {
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&h->stats.grid_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float* dz, const int64_t* offsets,
@@ -484,9 +480,7 @@ void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float*
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     Scratch& sc = h->scratch;
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     h->nclouds = nclouds;
     h->in_off.alloc(nclouds + 1);
@@ -530,8 +524,6 @@ void build_grid_batch(egt_ctx* h, const float* dx, const float* dy, const float*
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&h->stats.grid_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 void build_search_structure(egt_ctx* h) {

@@ -6,6 +6,7 @@
 #include <vector>
 #include "../../include/egt.h"
 #include "egt_device.h"
+#include "egt_limits.h"
 
 #define EGT_MAX_LEVELS 11
 static_assert(EGT_AUTOTUNE_SLOTS == EGT_MAX_LEVELS + egt::SUB, "autotuner slots: sub-cell levels + pyramid levels");
@@ -29,6 +30,57 @@ struct ArgError {
     } while (0)
 
 #define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+// A HIP event owned for one scope (stage timing): destroyed on every exit path, exceptions included.
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { HIP_CHECK(hipEventCreate(&e)); }
+    ~Event() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    Event(const Event&) = delete;
+    Event& operator=(const Event&) = delete;
+    operator hipEvent_t() const { return e; }
+};
+
+// An instantiated graph owned for one scope.
+struct GraphExec {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    GraphExec() = default;
+    GraphExec(const GraphExec&) = delete;
+    GraphExec& operator=(const GraphExec&) = delete;
+    ~GraphExec() {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+// Stream capture for one scope: if anything throws between begin and end, the destructor ends the capture
+// (and drops the partial graph), so the stream leaves capture mode and the handle stays usable.
+struct Capture {
+    hipStream_t s;
+    bool active = false;
+    explicit Capture(hipStream_t st) : s(st) {
+        HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        active = true;
+    }
+    Capture(const Capture&) = delete;
+    Capture& operator=(const Capture&) = delete;
+    // Ends the capture into out.g and instantiates out.ge.
+    void end(GraphExec& out) {
+        active = false;
+        HIP_CHECK(hipStreamEndCapture(s, &out.g));
+        HIP_CHECK(hipGraphInstantiate(&out.ge, out.g, nullptr, nullptr, 0));
+    }
+    ~Capture() {
+        if (!active) return;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+    }
+};
 
 // Device memory cache (egt_memory.hip): device_alloc returns a block of at least `bytes` (its capacity in
 // *capacity), reusing a block an earlier handle released; device_release hands a block back (the caller
@@ -93,9 +145,9 @@ struct WorkClass {
 
 // Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code (15 = k-th
 // distance of the last search), 16 host-input staging, 17-23 EGG / Taubin, 24-26 learned proposal,
-// 27-31 Taubin sliced-ELLPACK tiles, 32-35 query-vs-reference search.
+// 27-31 Taubin sliced-ELLPACK tiles, 32-35 query-vs-reference search, 36 64-bit totals.
 struct Scratch {
-    DBuf<uint32_t> bufs[36];
+    DBuf<uint32_t> bufs[37];
     template <class T>
     T* get(int i, size_t count) {
         bufs[i].alloc((count * sizeof(T) + sizeof(uint32_t) - 1) / sizeof(uint32_t));
@@ -204,6 +256,8 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_
 void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
                             uint32_t* total_dev);
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, Scratch& sc);
+// Sum of n uint32 values in 64 bits (synchronizes the stream).
+uint64_t sum_u32(const uint32_t* in, int64_t n, hipStream_t s, Scratch& sc);
 uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
 
 // grid (egt_grid.hip)

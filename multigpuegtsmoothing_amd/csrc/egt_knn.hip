@@ -687,9 +687,7 @@ void run_knn_clouds(egt_ctx* h) {
     h->work_classes = classes;
     HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
     unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     for (const WorkClass& cr : classes) {
         P.chunks = h->nclouds > 1 ? h->work_lists.p + cr.first : nullptr;
@@ -705,8 +703,6 @@ void run_knn_clouds(egt_ctx* h) {
     HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->stats.knn_candidates = (int64_t)hc[0];
     h->stats.invalid_rows = (int64_t)hc[1];
     h->stats.learned_queries = (int64_t)hc[2];
@@ -735,9 +731,7 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     h->qvcell.alloc((size_t)std::max<int64_t>(nq, 1));
     KnnParams P = knn_params(h);
     HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     CloudDev qcd{};
     qcd.base = 0;
@@ -777,8 +771,6 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));  // also keeps qcd alive until its copy has run
     HIP_CHECK(hipEventElapsedTime(&h->stats.query_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->stats.query_count = nq;
     h->stats.query_k = k;
     h->stats.query_candidates = (int64_t)hc[0];
@@ -821,9 +813,7 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     const int K = kclass(k);
     const int top = std::min(h->nlev - 1, std::max(3, auto_start_level(h, k) + 1));
     for (int i = 0; i < EGT_AUTOTUNE_SLOTS; ++i) trial_ms[i] = trial_cost[i] = -1.0f;
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     int best = 0;
     double best_cost = 0.0;
     for (int l = -SUB; l <= top; ++l) {
@@ -846,8 +836,6 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
             best_cost = cost;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->start_level = best;
     h->have_knn = false;  // the trial rows cover only the sample
     return best;

@@ -93,9 +93,7 @@ void run_mesh_adjacency(egt_ctx* h, int k) {
     int32_t* inv = h->scratch.get<int32_t>(8, n);
     uint32_t* cnt = h->scratch.get<uint32_t>(9, n);
     uint32_t* off = h->scratch.get<uint32_t>(10, n);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     k_invert<<<ceil_div(n, MBLK), MBLK, 0, s>>>(h->perm.p, n, inv);
     LAUNCH_CHECK();
@@ -126,8 +124,6 @@ void run_mesh_adjacency(egt_ctx* h, int k) {
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->k = k;
     h->have_knn = true;
     h->stats.invalid_rows = 0;

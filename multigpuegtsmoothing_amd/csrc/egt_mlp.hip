@@ -733,9 +733,7 @@ void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks) {
     set_cloud_features(h, ks);
     const float a = -h->eps, ba = (1.0f + h->eps) - a;
     HIP_CHECK(hipStreamSynchronize(m->stream));  // parameters written on the model stream are final
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     const int64_t tiles = (n + TS - 1) / TS;
     k_mlp_propose<<<(int)std::min<int64_t>(8192, (tiles + WPB - 1) / WPB), MBLK, 0, s>>>(
@@ -744,8 +742,6 @@ void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks) {
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));
     HIP_CHECK(hipEventElapsedTime(&h->stats.propose_ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->learned = 1;
 }
 

@@ -508,9 +508,7 @@ T* dalloc(DBuf<uint8_t>& b, size_t& off, size_t count) {
 void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
-    hipEvent_t ev0, ev1;
-    HIP_CHECK(hipEventCreate(&ev0));
-    HIP_CHECK(hipEventCreate(&ev1));
+    Event ev0, ev1;
     HIP_CHECK(hipEventRecord(ev0, s));
     const dim3 gn(ceil_div(n, NBLK)), blk(NBLK);
 
@@ -645,8 +643,6 @@ void run_normals(egt_ctx* h, float* nxh, float* nyh, float* nzh) {
     HIP_CHECK(hipMemcpyAsync(nzh, on + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.normals_ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
     double W = 0.0;
     int64_t C = 0;
     for (int b = 0; b < sb; ++b) {

@@ -1,5 +1,6 @@
 // Device primitives for gfx950: exclusive prefix sum over uint32 and a stable LSD radix sort of
 // (uint32 key, uint32 value) pairs. Wave64 throughout; blocks of 256 threads (4 waves).
+#include <algorithm>
 #include "egt_internal.h"
 
 namespace egt {
@@ -190,9 +191,32 @@ __global__ void __launch_bounds__(SORT_BLOCK) k_sort_scatter(const uint32_t* __r
     }
 }
 
+__global__ void __launch_bounds__(SCAN_BLOCK) k_sum_u64(const uint32_t* __restrict__ in, int64_t n,
+                                                         unsigned long long* __restrict__ total) {
+    uint64_t s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SCAN_BLOCK)
+        s += in[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, WAVE);
+    if (__lane_id() == 0 && s) atomicAdd(total, (unsigned long long)s);
+}
+
 }  // namespace
 
 static DBuf<uint32_t>& part_buf(Scratch& sc, int i);
+
+uint64_t sum_u32(const uint32_t* in, int64_t n, hipStream_t s, Scratch& sc) {
+    unsigned long long* t = reinterpret_cast<unsigned long long*>(sc.get<uint64_t>(36, 1));
+    HIP_CHECK(hipMemsetAsync(t, 0, sizeof(uint64_t), s));
+    if (n > 0) {
+        k_sum_u64<<<(int)std::min<int64_t>(2048, ceil_div(n, SCAN_BLOCK)), SCAN_BLOCK, 0, s>>>(in, n, t);
+        LAUNCH_CHECK();
+    }
+    uint64_t h = 0;
+    HIP_CHECK(hipMemcpyAsync(&h, t, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return h;
+}
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
                         uint32_t* total_host) {

@@ -9,8 +9,10 @@
  * Reference citations are file:line into the reference repository.
  *
  * Rounding contract (the HIP kernels implement the same sequence independently): every float operation
- * is rounded as written in the reference source, left to right, except these fused forms, which the
- * reference's nvcc build also contracts by default:
+ * is rounded as written in the reference source, left to right, except these fused forms. The set is this
+ * contract's own choice: the reference's nvcc build contracts by default and may fuse more (for example the
+ * EGG cross product, EGT_CUDA.cu:97-104, and the ellipsoid value's leading term, :165, which stay unfused
+ * here), so bit parity with the reference binary is unpinned:
  *   sq(p, q)         = fmaf(dz, dz, fmaf(dy, dy, dx * dx)), d* = p* - q*   (squared distances, norms)
  *   row(a, v) + t    = fmaf(a3, v3, fmaf(a2, v2, a1 * v1)) + t               (EGG frame transform)
  *   acc += w * v     -> acc = fmaf(w, v, acc)                                (Taubin / PCA sums)
@@ -34,7 +36,8 @@
  *                          the inverse-square weights, FP32 vs FP64 bound
  *   egt_expf            -- pinned: <=2 ulp against double exp over [-90, 0]
  *   orc_eig3            -- pinned: numpy.linalg.eigh on random symmetric matrices (angle/eigenvalue bounds)
- *   orc_pca_normals     -- pinned: plane -> plane normal, sphere -> radial
+ *   orc_pca_normals     -- pinned: plane -> plane normal, sphere -> radial, a hand-built 5-point case whose
+ *                          normal flips axis if the point itself enters the covariance
  *   orc_orient_normals  -- pinned: MSF weight equals scipy minimum_spanning_tree; per-component sign
  *                          consistency on sphere / two-sphere clouds
  *   orc_mesh_adjacency  -- pinned: icosphere one-ring valence 5/6 and brute-force corner union

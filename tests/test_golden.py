@@ -58,3 +58,17 @@ def test_fp32_taubin_fixture_within_bound_of_fp64():
         diag = np.linalg.norm(p.max(0) - p.min(0))
         err = np.abs(f["taubin10_f32"].astype(np.float64) - f["taubin10_f64"]).max()
         assert err <= 1e-5 * diag, (name, err / diag)
+
+
+def test_oracle_sanitizer_build_runs_clean():
+    """The oracle built with AddressSanitizer + UndefinedBehaviorSanitizer (make -C oracle sanitize) runs every
+    entry point on small seeded inputs without a report."""
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["make", "-s", "-C", os.path.join(root, "oracle"), "sanitize"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", OMP_NUM_THREADS="4")
+    r = subprocess.run([os.path.join(root, "oracle", "_build", "oracle_sanitize")], capture_output=True, text=True,
+                       env=env, timeout=600)
+    assert r.returncode == 0 and "sanitize ok" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]

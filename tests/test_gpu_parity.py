@@ -208,6 +208,7 @@ def test_taubin_vs_fp64(egt_mod, oracle_mod):
     fx, fy, fz = O.taubin_f32(og.xs, og.ys, og.zs, egg, iters, lam, mu)
     dx, dy, dz = O.taubin_f64(og.xs, og.ys, og.zs, egg, iters, lam, mu)
     np.testing.assert_array_equal(gx, fx)
+    np.testing.assert_array_equal(gy, fy)
     np.testing.assert_array_equal(gz, fz)
     err = np.sqrt((gx - dx) ** 2 + (gy - dy) ** 2 + (gz - dz) ** 2)
     diag = np.sqrt(3.0)  # normalized cube [0,1]^3
@@ -248,9 +249,49 @@ def test_mesh_rows_and_smoothing(egt_mod, oracle_mod):
     assert set(np.unique(deg)) <= {5, 6}
     E.EGT_CUDA(m).PerformSmoothing(iters, 0.5, -0.53, 1e-8)
     out = E.getOutputPoints(m)
-    ref = O.pipeline(x, y, z, k, iters, 0.5, -0.53, 1e-8, normals=False, mesh_tri=tri)
+    nx, ny, nz = E.Normal_CUDA(m).GetNormals()
+    ref = O.pipeline(x, y, z, k, iters, 0.5, -0.53, 1e-8, normals=True, mesh_tri=tri)
     for a, b in zip(out, ref["output"]):
         np.testing.assert_array_equal(a, b)
+    # mesh-mode normals (main.cpp:112-134: the mesh provider feeds Normal_CUDA like a cloud)
+    for a, b in zip((nx, ny, nz), ref["normals"]):
+        np.testing.assert_array_equal(a, b)
+    st = m.stats()
+    assert st["msf_edges"] == ref["orient_stats"][1] and st["msf_trees"] == ref["orient_stats"][2]
+
+
+@pytest.mark.parametrize("regularized", [False, True])
+def test_mesh_degenerate_triangles_keep_self(egt_mod, oracle_mod, regularized):
+    """Triangles with a repeated corner make a vertex list itself at distance 0 (GridMeshCuda1.cu:115-125); the
+    EGG filter keeps that entry (d < 1e-6, EGT_CUDA.cu:147) and Taubin weights it like any listed slot
+    (exp(0) = 1, or 1 / (0 + 1e-8) in the regularized mu pass, EGT_CUDA.cu:242-270). LDS tiles must weight it
+    as the oracle does."""
+    E, O = egt_mod, oracle_mod
+    x, y, z, tri = D.icosphere(4, seed=62)
+    r = np.random.default_rng(7)
+    pick = r.choice(tri.shape[0], 300, replace=False)
+    deg = tri[pick].copy()
+    deg[:, 1] = deg[:, 0]  # (a, a, c)
+    tri = np.ascontiguousarray(np.concatenate([tri, deg]), np.uint32)
+    k, iters, alpha = 16, 12, 1e-8
+    m = E.GridMeshCuda((x, y, z), tri)
+    m.set_option(E.OPT_TAUBIN_REGULARIZED, int(regularized))
+    m.GetNeighborsFromTriangles(k)
+    og = O.Grid(x, y, z)
+    want = og.mesh_adjacency(tri, k)
+    rows = m.GetKNNIndexes()
+    np.testing.assert_array_equal(rows, og.to_original(want))
+    self_rows = int((rows == np.arange(x.size)[:, None]).any(1).sum())
+    _log("rows listing themselves", self_rows)
+    assert self_rows > 100
+    E.EGT_CUDA(m).PerformSmoothing(iters, 0.5, -0.53, alpha)
+    out = E.getOutputPoints(m)
+    egg = O.egg(og.xs, og.ys, og.zs, want, alpha)
+    assert (egg == np.arange(x.size)[:, None]).any(1).sum() == self_rows  # the filter keeps self (sorted space)
+    sx, sy, sz = O.taubin_f32(og.xs, og.ys, og.zs, egg, iters, 0.5, -0.53, regularized=regularized)
+    for a, b in zip(out, og.output(sx, sy, sz)):
+        np.testing.assert_array_equal(a, b)
+    m.close()
 
 
 def test_output_round_trip(egt_mod, oracle_mod):

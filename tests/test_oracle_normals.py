@@ -38,6 +38,27 @@ def test_plane_normals(oracle_mod):
     assert np.mean(np.abs(nz) > 0.99) > 0.99
 
 
+def test_pca_excludes_self_from_covariance(oracle_mod):
+    """calculateNormal (Normal_CUDA.cu:87-144) puts the point itself into the centroid but not into the
+    covariance. Hand-built case where that choice decides the normal: a point p at the origin with four
+    neighbours on the plane z = 1 at (+-0.3, 0, 1), (0, +-0.35, 1). Centroid c = (0, 0, 0.8). Neighbours only:
+    C = diag(0.18, 0.245, 0.16), smallest along z -> n = +-z. With p added: C_zz = 0.16 + 0.64 = 0.8, smallest
+    along x -> n = +-x. The oracle must give +-z."""
+    P = np.float64([[0, 0, 0], [0.3, 0, 1], [-0.3, 0, 1], [0, 0.35, 1], [0, -0.35, 1]])
+    c = P.mean(0)
+    np.testing.assert_allclose(c, [0, 0, 0.8], atol=1e-12)
+    C_excl = sum(np.outer(q - c, q - c) for q in P[1:])
+    C_incl = C_excl + np.outer(P[0] - c, P[0] - c)
+    np.testing.assert_allclose(np.diag(C_excl), [0.18, 0.245, 0.16], atol=1e-12)
+    assert abs(np.linalg.eigh(C_excl)[1][:, 0] @ [0, 0, 1]) > 0.999  # the case discriminates
+    assert abs(np.linalg.eigh(C_incl)[1][:, 0] @ [1, 0, 0]) > 0.999
+    xs, ys, zs = (np.ascontiguousarray(P[:, i], np.float32) for i in range(3))
+    knn = np.full((5, 4), -1, np.int32)
+    knn[0] = [1, 2, 3, 4]
+    nx, ny, nz = oracle_mod.pca_normals(xs, ys, zs, knn)
+    assert abs(nz[0]) > 0.999 and abs(nx[0]) < 1e-3 and abs(ny[0]) < 1e-3
+
+
 def test_sphere_normals_radial_and_oriented(oracle_mod):
     res = oracle_mod.pipeline(*D.sphere(20_000, seed=43), 24, 5, 0.5, -0.53, 0.65, normals=True)
     g = res["grid"]
