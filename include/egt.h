@@ -63,8 +63,8 @@ typedef struct {
     int64_t knn_cells;         /* cells looked up by the kNN search */
     float taubin_tile_ms[2];   /* Taubin tile autotuner: ms per pass at tile width 1024 and 2048 (0 if untried) */
     int32_t taubin_tile;       /* tile width of the last smoothing call */
-    int32_t taubin_fallback_tiles; /* tiles of that call whose halo list exceeded its cap (CSR rows, global
-                                      gathers) */
+    int32_t taubin_fallback_tiles; /* tiles of that call run as CSR rows with global gathers: halo list over
+                                      its cap, or a row that lists itself (mesh with degenerate triangles) */
     int64_t query_count;       /* queries of the last query-vs-reference search */
     int64_t query_candidates;  /* candidate distances it evaluated */
     int64_t query_invalid_rows; /* its rows blanked by the reference's validity rule */
@@ -74,6 +74,9 @@ typedef struct {
     float taubin_setup_ms;     /* its tile set-up and graph build (taubin_ms = setup + all passes) */
     int32_t taubin_overflow_tiles; /* tiles of that call whose halo only partly fit in LDS (the rest is read
                                       from global memory) */
+    int64_t learned_bound_retries; /* last kNN: queries with fewer than k points within their proposed radius,
+                                      searched again without the bound */
+    int64_t knn_insertions;    /* last kNN: keys inserted into the neighbour lists */
 } egt_stats;
 
 /* EGT_OPT_START_LEVEL value: choose the level from the mean cell occupancy. */
@@ -95,9 +98,13 @@ typedef enum {
     EGT_OPT_TAUBIN_TILE = 5,     /* Taubin: points per LDS tile, 1024 (default) or 2048; 0 = autotune on the
                                     next smoothing call (times both on copies, keeps the faster; the
                                     per-pass times land in egt_stats.taubin_tile_ms) */
-    EGT_OPT_TAUBIN_REGULARIZED = 6 /* Taubin weights: 0 = exp(-d^2) (default, as the reference runs); 1 = the
+    EGT_OPT_TAUBIN_REGULARIZED = 6, /* Taubin weights: 0 = exp(-d^2) (default, as the reference runs); 1 = the
                                       reference's dormant isRegularized variant, EGT_CUDA.cu:263, 544: weight
                                       1 in the lambda pass, 1 / (d^2 + 1e-8) in the mu pass */
+    EGT_OPT_LEARNED_MODE = 7     /* kNN with the learned proposal: bit 0 = start at the proposed level, bit 1 =
+                                    bound the search by the proposed radius (queries with fewer than k points
+                                    inside it are searched again without the bound); default 1 (the level only:
+                                    the bound measured slower). Rows do not depend on it */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -274,8 +281,12 @@ egt_status egt_model_train_batch(egt_model m, egt_handle h, const int32_t* k, in
                                  int64_t max_samples, float lr, float quantile, float* loss);
 egt_status egt_model_get_safety(egt_model m, float* safety);
 egt_status egt_model_set_safety(egt_model m, float safety);
-/* Proposes a start level for every query of h for neighbour count k and switches h to learned starts
- * (EGT_OPT_LEARNED = 1); the next egt_grid_knn_self uses them. */
+/* Margin (log2 units) of the radius bound: the bound is the predicted radius times 2^margin. Training sets it
+ * to the 0.99 quantile of the residuals (at least the level margin). */
+egt_status egt_model_get_bound_safety(egt_model m, float* safety);
+egt_status egt_model_set_bound_safety(egt_model m, float safety);
+/* Proposes a start level and a radius bound for every query of h for neighbour count k and switches h to
+ * learned starts (EGT_OPT_LEARNED = 1); the next egt_grid_knn_self uses them. */
 egt_status egt_model_propose(egt_model m, egt_handle h, int k);
 /* Batch handle: proposals for every cloud with k[c] neighbours; the next egt_batch_knn uses them. */
 egt_status egt_model_propose_batch(egt_model m, egt_handle h, const int32_t* k);

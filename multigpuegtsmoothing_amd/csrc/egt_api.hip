@@ -329,6 +329,10 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
         case EGT_OPT_TAUBIN_REGULARIZED:
             h->taubin_regularized = value ? 1 : 0;
             return EGT_OK;
+        case EGT_OPT_LEARNED_MODE:
+            if (value < 1 || value > 3) return fail(EGT_ERR_INVALID_ARG, "learned mode must be 1, 2 or 3");
+            h->learned_mode = (int)value;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }
@@ -585,6 +589,19 @@ egt_status egt_model_set_safety(egt_model m, float safety) {
     if (!m) return fail(EGT_ERR_INVALID_ARG, "null model");
     if (!std::isfinite(safety)) return fail(EGT_ERR_INVALID_ARG, "safety must be finite");
     m->safety = safety;
+    return EGT_OK;
+}
+
+egt_status egt_model_get_bound_safety(egt_model m, float* safety) {
+    if (!m || !safety) return fail(EGT_ERR_INVALID_ARG, "null model or output");
+    *safety = m->bound_safety;
+    return EGT_OK;
+}
+
+egt_status egt_model_set_bound_safety(egt_model m, float safety) {
+    if (!m) return fail(EGT_ERR_INVALID_ARG, "null model");
+    if (!std::isfinite(safety)) return fail(EGT_ERR_INVALID_ARG, "safety must be finite");
+    m->bound_safety = safety;
     return EGT_OK;
 }
 

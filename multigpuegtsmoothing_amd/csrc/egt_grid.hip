@@ -59,14 +59,15 @@ __global__ void __launch_bounds__(BLK) k_minmax_partial(const float* __restrict_
 }
 
 // normalize_points GridKNNCuda.cu:840-849 fused with get_buckets_no_binary :74-88. With icloud (batch) each
-// point uses its own cloud's cube; key_in (batch) keeps an unsorted copy of the keys.
+// point uses its own cloud's cube. The normalized point and its key go out as one 16-byte record in input
+// order (the gather into sorted order then reads one record per point), the key and index also as the sort's
+// key and value arrays.
 __global__ void __launch_bounds__(BLK) k_normalize_key(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ z, int64_t n, float mn, float d,
                                                         const int32_t* __restrict__ icloud,
                                                         const CloudDev* __restrict__ clouds, float a, float ba, int B,
-                                                        int log2B, float* __restrict__ nx, float* __restrict__ ny,
-                                                        float* __restrict__ nz, uint32_t* __restrict__ key,
-                                                        uint32_t* __restrict__ key_in, uint32_t* __restrict__ idx) {
+                                                        int log2B, float4* __restrict__ tp, uint32_t* __restrict__ key,
+                                                        uint32_t* __restrict__ idx) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     if (icloud) {
@@ -75,13 +76,11 @@ __global__ void __launch_bounds__(BLK) k_normalize_key(const float* __restrict__
         d = c.extent;
     }
     const float xn = (x[i] - mn) / d, yn = (y[i] - mn) / d, zn = (z[i] - mn) / d;
-    nx[i] = xn;
-    ny[i] = yn;
-    nz[i] = zn;
     const uint32_t cx = (uint32_t)cell_of(xn, a, ba, B), cy = (uint32_t)cell_of(yn, a, ba, B),
                    cz = (uint32_t)cell_of(zn, a, ba, B);
-    key[i] = cx | (cy << log2B) | (cz << (2 * log2B));
-    if (key_in) key_in[i] = key[i];
+    const uint32_t k = cx | (cy << log2B) | (cz << (2 * log2B));
+    tp[i] = make_float4(xn, yn, zn, __uint_as_float(k));
+    key[i] = k;
     idx[i] = (uint32_t)i;
 }
 
@@ -186,11 +185,8 @@ __global__ void __launch_bounds__(BLK) k_gather_cloud(const int32_t* __restrict_
     if (i < n) out[i] = (uint32_t)cloud_of[idx[i]];
 }
 
-// thrust::gather into sorted order, GridKNNCuda.cu:1114-1128.
-__global__ void __launch_bounds__(BLK) k_gather_sorted(const float* __restrict__ nx, const float* __restrict__ ny,
-                                                        const float* __restrict__ nz, const uint32_t* __restrict__ skey,
-                                                        const uint32_t* __restrict__ key_in,
-                                                        const int32_t* __restrict__ icloud,
+// thrust::gather into sorted order, GridKNNCuda.cu:1114-1128: one 16-byte record per point.
+__global__ void __launch_bounds__(BLK) k_gather_sorted(const float4* __restrict__ tp, const int32_t* __restrict__ icloud,
                                                         const uint32_t* __restrict__ sidx, int64_t n,
                                                         float* __restrict__ x, float* __restrict__ y,
                                                         float* __restrict__ z, int32_t* __restrict__ perm,
@@ -198,29 +194,33 @@ __global__ void __launch_bounds__(BLK) k_gather_sorted(const float* __restrict__
     const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (s >= n) return;
     const uint32_t o = sidx[s];
+    const float4 v = tp[o];
     perm[s] = (int32_t)o;
-    key[s] = (int32_t)(key_in ? key_in[o] : skey[s]);
+    key[s] = __float_as_int(v.w);
     if (scloud) scloud[s] = icloud[o];
-    x[s] = nx[o];
-    y[s] = ny[o];
-    z[s] = nz[o];
+    x[s] = v.x;
+    y[s] = v.y;
+    z[s] = v.z;
 }
 
 // Sort keys of the Morton order: the fine cell's Morton code and the Morton code of the point's position
-// inside the cell (sorted first, stably, so that points of a cell end up in sub-code order).
+// inside the cell (sorted first, stably, so that points of a cell end up in sub-code order); also the point as
+// one 16-byte record (x, y, z, key) in sorted order for the gather into Morton order.
 __global__ void __launch_bounds__(BLK) k_morton_keys(const float* __restrict__ x, const float* __restrict__ y,
                                                       const float* __restrict__ z, const int32_t* __restrict__ key,
                                                       int64_t n, float a, float ba, int B, int log2B,
                                                       uint32_t* __restrict__ code, uint32_t* __restrict__ sub,
-                                                      uint32_t* __restrict__ idx) {
+                                                      uint32_t* __restrict__ idx, float4* __restrict__ spts) {
     const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (s >= n) return;
     const uint32_t k = (uint32_t)key[s], m = (1u << log2B) - 1u;
     const uint32_t cx = k & m, cy = (k >> log2B) & m, cz = k >> (2 * log2B);
+    const float px = x[s], py = y[s], pz = z[s];
     code[s] = morton_code(cx, cy, cz);
-    sub[s] = vcell_sub(vcell_pack(vcoord(x[s], a, ba, B, (int)cx), vcoord(y[s], a, ba, B, (int)cy),
-                                  vcoord(z[s], a, ba, B, (int)cz)));
+    sub[s] = vcell_sub(vcell_pack(vcoord(px, a, ba, B, (int)cx), vcoord(py, a, ba, B, (int)cy),
+                                  vcoord(pz, a, ba, B, (int)cz)));
     idx[s] = (uint32_t)s;
+    spts[s] = make_float4(px, py, pz, __int_as_float((int32_t)k));
 }
 
 __global__ void __launch_bounds__(BLK) k_gather_u32(const uint32_t* __restrict__ src,
@@ -230,8 +230,7 @@ __global__ void __launch_bounds__(BLK) k_gather_u32(const uint32_t* __restrict__
     if (i < n) out[i] = src[idx[i]];
 }
 
-__global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__ x, const float* __restrict__ y,
-                                                        const float* __restrict__ z, const int32_t* __restrict__ key,
+__global__ void __launch_bounds__(BLK) k_morton_points(const float4* __restrict__ spts,
                                                         const int32_t* __restrict__ scloud,
                                                         const uint32_t* __restrict__ order, int64_t n, float a,
                                                         float ba, int B, int log2B, float4* __restrict__ mpts,
@@ -243,9 +242,10 @@ __global__ void __launch_bounds__(BLK) k_morton_points(const float* __restrict__
     const uint32_t s = order[m];
     morder[m] = (int32_t)s;
     minv[s] = (int32_t)m;
-    const float px = x[s], py = y[s], pz = z[s];
+    const float4 v = spts[s];
+    const float px = v.x, py = v.y, pz = v.z;
     mpts[m] = make_float4(px, py, pz, __int_as_float((int)s));
-    const uint32_t k = (uint32_t)key[s], msk = (1u << log2B) - 1u;
+    const uint32_t k = (uint32_t)__float_as_int(v.w), msk = (1u << log2B) - 1u;
     const uint32_t cx = k & msk, cy = (k >> log2B) & msk, cz = k >> (2 * log2B);
     mcell[m] = cx | (cy << 10) | (cz << 20);
     mvcell[m] = vcell_pack(vcoord(px, a, ba, B, (int)cx), vcoord(py, a, ba, B, (int)cy), vcoord(pz, a, ba, B, (int)cz));
@@ -305,10 +305,11 @@ __global__ void __launch_bounds__(BLK) k_level_insert(const uint32_t* __restrict
 
 // seg value at each cloud's first Morton position (running cell count), plus the total at the end.
 __global__ void __launch_bounds__(BLK) k_cells_at_bases(const uint32_t* __restrict__ seg, const CloudDev* __restrict__ clouds,
-                                                         int nclouds, uint32_t total, uint32_t* __restrict__ out) {
+                                                         int nclouds, const uint32_t* __restrict__ total,
+                                                         uint32_t* __restrict__ out) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c < nclouds) out[c] = seg[clouds[c].base];
-    if (c == nclouds) out[c] = total;
+    if (c == nclouds) out[c] = *total;
 }
 
 __global__ void __launch_bounds__(BLK) k_fill_u64(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
@@ -398,15 +399,11 @@ void sort_into_sorted_space(egt_ctx* h, const float* dx, const float* dy, const 
     Scratch& sc = h->scratch;
     const int log2B = ilog2(h->B);
     const float a = -h->eps, b = 1.0f + h->eps, ba = b - a;
-    float* tx = sc.get<float>(9, n);
-    float* ty = sc.get<float>(10, n);
-    float* tz = sc.get<float>(11, n);
+    float4* tp = sc.get<float4>(9, n);
     uint32_t* skey = sc.get<uint32_t>(12, n);
     uint32_t* sidx = sc.get<uint32_t>(13, n);
-    uint32_t* key_in = icloud ? sc.get<uint32_t>(23, n) : nullptr;
     k_normalize_key<<<ceil_div(n, BLK), BLK, 0, s>>>(dx, dy, dz, n, h->min_extent, h->max_extent - h->min_extent,
-                                                      icloud, h->dclouds.p, a, ba, h->B, log2B, tx, ty, tz, skey,
-                                                      key_in, sidx);
+                                                      icloud, h->dclouds.p, a, ba, h->B, log2B, tp, skey, sidx);
     LAUNCH_CHECK();
     radix_sort_pairs(skey, sidx, n, 3 * log2B, s, sc);
     if (icloud) {
@@ -422,9 +419,8 @@ void sort_into_sorted_space(egt_ctx* h, const float* dx, const float* dy, const 
     h->z.alloc(n);
     h->perm.alloc(n);
     h->key.alloc(n);
-    k_gather_sorted<<<ceil_div(n, BLK), BLK, 0, s>>>(tx, ty, tz, skey, key_in, icloud, sidx, n, h->x.p, h->y.p,
-                                                      h->z.p, h->perm.p, h->key.p,
-                                                      icloud ? h->scloud.p : nullptr);
+    k_gather_sorted<<<ceil_div(n, BLK), BLK, 0, s>>>(tp, icloud, sidx, n, h->x.p, h->y.p, h->z.p, h->perm.p,
+                                                      h->key.p, icloud ? h->scloud.p : nullptr);
     LAUNCH_CHECK();
 }
 
@@ -537,13 +533,14 @@ void build_search_structure(egt_ctx* h) {
     code.alloc(n);
     order.alloc(n);
     const float a = -h->eps, ba = (1.0f + h->eps) - a;
+    float4* spts = sc.get<float4>(11, n);  // sorted-space records (x, y, z, key)
     {
         // Morton order of the fine cells, points of a cell in sub-code order: stable sort by sub code, then
         // stable sort by cell code
         uint32_t* sub = sc.get<uint32_t>(22, n);
         uint32_t* cell = sc.get<uint32_t>(8, n);
         k_morton_keys<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, n, a, ba, h->B, log2B,
-                                                       cell, sub, order.p);
+                                                       cell, sub, order.p, spts);
         LAUNCH_CHECK();
         radix_sort_pairs(sub, order.p, n, 3 * SUB, s, sc);
         k_gather_u32<<<ceil_div(n, BLK), BLK, 0, s>>>(cell, order.p, n, code.p);
@@ -564,42 +561,55 @@ void build_search_structure(egt_ctx* h) {
     h->mcode.alloc(n);
     h->morder.alloc(n);
     h->minv.alloc(n);
-    k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(h->x.p, h->y.p, h->z.p, h->key.p, batch ? h->scloud.p : nullptr,
+    k_morton_points<<<ceil_div(n, BLK), BLK, 0, s>>>(spts, batch ? h->scloud.p : nullptr,
                                                       order.p, n, a, ba, h->B, log2B, h->mpts.p, h->mcell.p,
                                                       reinterpret_cast<unsigned long long*>(h->mvcell.p), h->mcode.p,
                                                       batch ? h->mcloud.p : nullptr, h->morder.p, h->minv.p);
     LAUNCH_CHECK();
     const int32_t* mcl = batch ? h->mcloud.p : nullptr;
 
-    // level tables: level l groups 2^l fine cells per axis (Morton code >> 3l), per cloud
+    // level tables: level l groups 2^l fine cells per axis (Morton code >> 3l), per cloud. Counting pass, level
+    // by level without a host wait: a flag per Morton position (first of its cell), a scan whose total stays on
+    // the device, the cells' first positions (kept for the inserts) and, for a batch, the running count at
+    // each cloud's first position. One read-back of every level's counts then sizes the tables.
     h->nlev = log2B + 1;
+    const int nc = h->nclouds, nl = h->nlev;
     DBuf<uint32_t>& flag = sc.bufs[9];
     DBuf<uint32_t>& seg = sc.bufs[10];
     flag.alloc(n);
     seg.alloc(n);
-    // first Morton position of every cell, per level: kept from the counting pass for the inserts, so the
-    // flags and their scan run once per level
-    std::vector<DBuf<uint32_t>> starts(h->nlev);
-    std::vector<uint32_t> cells(h->nlev);
-    uint64_t total = 0;
-    const int nc = h->nclouds;
-    h->cloud_cells.assign((size_t)nc * h->nlev, 0);
-    uint32_t* dbase_cells = batch ? sc.get<uint32_t>(22, nc + 1) : nullptr;
-    std::vector<uint32_t> hb(nc + 1);
-    for (int l = 0; l < h->nlev; ++l) {
+    // room for the starts of every level: at most min(n, cells of the level's grid over all clouds) each
+    std::vector<size_t> soff(nl + 1, 0);
+    for (int l = 0; l < nl; ++l) {
+        const double grid_cells = (double)nc * std::pow((double)(h->B >> l), 3.0);
+        soff[l + 1] = soff[l] + (size_t)std::min<double>((double)n, grid_cells);
+    }
+    uint32_t* starts = sc.get<uint32_t>(38, soff[nl]);
+    uint32_t* dtot = sc.get<uint32_t>(23, (size_t)nl * (nc + 2));  // per level: total, then the cloud bases
+    for (int l = 0; l < nl; ++l) {
         k_level_flags<<<ceil_div(n, BLK), BLK, 0, s>>>(h->mcode.p, mcl, n, 3 * l, flag.p);
         LAUNCH_CHECK();
-        exclusive_scan_u32(flag.p, seg.p, n, s, sc, &cells[l]);
-        starts[l].alloc(cells[l]);
-        k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts[l].p);
+        uint32_t* tl = dtot + (size_t)l * (nc + 2);
+        exclusive_scan_u32_dev(flag.p, seg.p, n, s, sc, tl);
+        k_level_starts<<<ceil_div(n, BLK), BLK, 0, s>>>(flag.p, seg.p, n, starts + soff[l]);
         LAUNCH_CHECK();
         if (batch) {
-            // cells of each cloud at this level (the running count at the cloud boundaries)
-            k_cells_at_bases<<<ceil_div(nc + 1, BLK), BLK, 0, s>>>(seg.p, h->dclouds.p, nc, cells[l], dbase_cells);
+            k_cells_at_bases<<<ceil_div(nc + 1, BLK), BLK, 0, s>>>(seg.p, h->dclouds.p, nc, tl, tl + 1);
             LAUNCH_CHECK();
-            HIP_CHECK(hipMemcpyAsync(hb.data(), dbase_cells, (nc + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            for (int c = 0; c < nc; ++c) h->cloud_cells[(size_t)c * h->nlev + l] = hb[c + 1] - hb[c];
+        }
+    }
+    std::vector<uint32_t> ht((size_t)nl * (nc + 2));
+    HIP_CHECK(hipMemcpyAsync(ht.data(), dtot, ht.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    h->cloud_cells.assign((size_t)nc * nl, 0);
+    std::vector<uint32_t> cells(nl);
+    uint64_t total = 0;
+    LevelDev lv[EGT_MAX_LEVELS];
+    for (int l = 0; l < nl; ++l) {
+        const uint32_t* tl = &ht[(size_t)l * (nc + 2)];
+        cells[l] = tl[0];
+        if (batch) {
+            for (int c = 0; c < nc; ++c) h->cloud_cells[(size_t)c * nl + l] = tl[1 + c + 1] - tl[1 + c];
         } else {
             h->cloud_cells[l] = cells[l];
         }
@@ -608,28 +618,30 @@ void build_search_structure(egt_ctx* h) {
         h->lev[l].log2cap = log2cap;
         h->lev[l].offset = total;
         h->lev[l].cells = cells[l];
+        lv[l] = LevelDev{total, log2cap, 0};
         total += 1ull << log2cap;
     }
     h->htab.alloc(total);
     k_fill_u64<<<1024, BLK, 0, s>>>(reinterpret_cast<uint64_t*>(h->htab.p), 2 * total, HASH_EMPTY);
     LAUNCH_CHECK();
     DBuf<uint32_t>& dmax = sc.bufs[14];
-    dmax.alloc(h->nlev);
-    HIP_CHECK(hipMemsetAsync(dmax.p, 0, h->nlev * sizeof(uint32_t), s));
-    for (int l = 0; l < h->nlev; ++l) {
+    dmax.alloc(nl);
+    HIP_CHECK(hipMemsetAsync(dmax.p, 0, nl * sizeof(uint32_t), s));
+    for (int l = 0; l < nl; ++l) {
         k_level_insert<<<std::max(1, ceil_div(cells[l], BLK)), BLK, 0, s>>>(
-            starts[l].p, cells[l], n, h->mcode.p, mcl, 3 * l, h->htab.p, h->lev[l].offset, h->lev[l].log2cap,
+            starts + soff[l], cells[l], n, h->mcode.p, mcl, 3 * l, h->htab.p, h->lev[l].offset, h->lev[l].log2cap,
             reinterpret_cast<int*>(dmax.p) + l);
         LAUNCH_CHECK();
     }
-    std::vector<int32_t> mx(h->nlev);
-    HIP_CHECK(hipMemcpyAsync(mx.data(), dmax.p, h->nlev * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    std::vector<int32_t> mx(nl);
+    HIP_CHECK(hipMemcpyAsync(mx.data(), dmax.p, nl * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    for (int l = 0; l < h->nlev; ++l) h->lev[l].max_count = mx[l];
-    LevelDev lv[EGT_MAX_LEVELS];
-    for (int l = 0; l < h->nlev; ++l) lv[l] = LevelDev{h->lev[l].offset, h->lev[l].log2cap, h->lev[l].max_count};
+    for (int l = 0; l < nl; ++l) {
+        h->lev[l].max_count = mx[l];
+        lv[l].max_count = mx[l];
+    }
     h->dlev.alloc(EGT_MAX_LEVELS);
-    HIP_CHECK(hipMemcpyAsync(h->dlev.p, lv, h->nlev * sizeof(LevelDev), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(h->dlev.p, lv, nl * sizeof(LevelDev), hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
 }
 

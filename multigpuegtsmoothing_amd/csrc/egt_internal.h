@@ -145,9 +145,10 @@ struct WorkClass {
 
 // Reusable temporaries: 0-2 scan partials, 3 scan total, 4-7 radix sort, 8-15 stage code (15 = k-th
 // distance of the last search), 16 host-input staging, 17-23 EGG / Taubin, 24-26 learned proposal,
-// 27-31 Taubin sliced-ELLPACK tiles, 32-35 query-vs-reference search, 36 64-bit totals.
+// 27-31 Taubin sliced-ELLPACK tiles, 32-35 query-vs-reference search, 36 64-bit totals, 37 queries handed
+// back by a bounded kNN search, 38 the level tables' cell starts.
 struct Scratch {
-    DBuf<uint32_t> bufs[37];
+    DBuf<uint32_t> bufs[39];
     template <class T>
     T* get(int i, size_t count) {
         bufs[i].alloc((count * sizeof(T) + sizeof(uint32_t) - 1) / sizeof(uint32_t));
@@ -209,7 +210,8 @@ struct egt_ctx {
     int64_t ntri = 0;
 
     // learned proposal
-    egt::DBuf<int8_t> qlevel;
+    egt::DBuf<int8_t> qlevel;   // per Morton position: proposed start level
+    egt::DBuf<float> qbound;    // per Morton position: proposed radius bound (squared, normalized units)
 
     // query-vs-reference search (its rows live beside the self rows, which smoothing keeps using)
     egt::DBuf<float4> qpts;       // queries in Morton order of their cell: x', y', z', bits(original index)
@@ -224,6 +226,8 @@ struct egt_ctx {
     int start_level = 0;  // reference schedule: every query starts at its own level-0 cell; -SUB..-1 sub-cell
                           // boxes; EGT_START_LEVEL_OCCUPANCY = from the mean cell occupancy
     int learned = 0;
+    int learned_mode = 1;  // with learned: 1 = start at the proposed level, 2 = bound the search by the proposed
+                           // radius, 3 = both
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int taubin_graph = 1;
     int taubin_tile = 1024;  // Taubin tile width: 1024 or 2048 points; 0 = tune on the next smoothing call
@@ -241,7 +245,8 @@ struct egt_model_s {
     egt::DBuf<uint32_t> sel;     // training sample order
     int step = 0;
     int last_parts = 0;
-    float safety = 0.0f;
+    float safety = 0.0f;        // level choice: margin (log2 units) added to the predicted radius
+    float bound_safety = 0.0f;  // radius bound: margin added to the predicted radius
     int64_t train_samples = 0;
     uint64_t rng = 0x243F6A8885A308D3ull;
     hipStream_t stream = nullptr;

@@ -53,15 +53,21 @@ struct KnnParams {
     const uint4* __restrict__ htab;
     const int32_t* __restrict__ key;       // sorted-space fine key (member cell coordinates)
     const int8_t* __restrict__ qlevel;     // per-query start level (Morton order) or null
+    const float* __restrict__ qbound;      // per-query radius bound, squared normalized units (Morton order), or
+                                           // null: the list starts full of placeholder keys at that distance
     const CloudDev* __restrict__ clouds;   // per-cloud k, radius, row offset
     const int32_t* __restrict__ mcloud;    // cloud of each Morton position, null for one cloud
     const int64_t* __restrict__ chunks;    // work list: first Morton position of each 64-query chunk, or
                                            // null for positions 0, 64, 128, ...
+    const uint32_t* __restrict__ qlist;    // list launch: Morton positions of the queries, 64 per wave
+    int64_t nlist;                         // its length
+    uint32_t* __restrict__ retry;          // bounded search: queries to search again without the bound
     unsigned long long* __restrict__ queue;  // work-queue counter (persistent launch) or null
     int32_t* __restrict__ knn;             // rows, cloud after cloud, local sorted indices
     float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
     unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks,
-                                                // [4] cells looked up
+                                                // [4] cells looked up, [5] work queue, [6] bound retries,
+                                                // [7] keys inserted into the lists
     const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
     const float4* __restrict__ qpts;       // query search: queries in Morton order (x', y', z', bits(index))
     const unsigned long long* __restrict__ qvcell;  // query search: virtual cell, clamped to the grid
@@ -199,22 +205,25 @@ __device__ inline int64_t next_chunk(const KnnParams& P, int64_t cur) {
 // Occupancy: the list (2 K registers) dominates the register budget; small lists are held to 128 VGPRs
 // (four waves per SIMD) and K <= 40 to 168 (three), which the measured runs favour over fewer waves without
 // spills.
-template <int K, bool PERSIST, bool QUERY = false>
+// LIST: each wave takes 64 queries of a list (the queries a bounded search handed back), each lane its own
+// query and cloud.
+template <int K, bool PERSIST, bool QUERY = false, bool LIST = false>
 __global__ void __launch_bounds__(KBLK)
     __attribute__((amdgpu_waves_per_eu(K <= 8 || (K <= 16 && !PERSIST) ? 4 : (K <= 40 && !PERSIST ? 3 : 1))))
     k_knn(KnnParams P) {
     uint64_t cand = 0;
-    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0;
+    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0, inserted = 0;
     // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
     // persistent launch, chunk after chunk from a work queue until the list is exhausted.
     for (int64_t v = next_chunk<PERSIST>(P, -1); v < P.nchunks; v = next_chunk<PERSIST>(P, v)) {
-    const int64_t start = P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64;
-    const int64_t m = start + __lane_id();
-    const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[start] : 0u;
+    const int64_t start = LIST ? 0 : (P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64);
+    const int64_t li = v * 64 + __lane_id();
+    const int64_t m = LIST ? (li < P.nlist ? (int64_t)P.qlist[li] : -1) : start + __lane_id();
+    const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[LIST ? (m < 0 ? 0 : m) : start] : 0u;
     const CloudDev cd = P.clouds[cloud];
     const int kq = cd.k;
     const float thres_n = cd.thres_n;
-    const bool active = m < cd.base + cd.n;
+    const bool active = LIST ? m >= 0 : m < cd.base + cd.n;
     // Learned start: every lane of a wave starts at the smallest level proposed in the wave, so the wave's
     // 64 neighbouring queries scan the same cells in step; lanes whose proposal is higher expand from there.
     int own_level = P.start_level;
@@ -237,9 +246,17 @@ __global__ void __launch_bounds__(KBLK)
                       uz = cell_of(q.z, P.a, P.ba, P.B);
             outside = ux < 0 || uy < 0 || uz < 0 || ux >= P.B || uy >= P.B || uz >= P.B;
         }
+        // Bounded search (learned radius r): the list starts full of placeholder keys (r^2 bits, index
+        // 0xFFFFFFFF), above every real key at distance <= r and below every key beyond it. Candidates beyond
+        // r are then never buffered and cells beyond r are pruned from the start; the exactness stop below
+        // holds as written (once every point within the bound is scanned, the list is exact). A list that
+        // still holds a placeholder at the end has fewer than k points within r: the query is handed back
+        // and searched again without the bound.
+        const uint64_t init_key =
+            P.qbound ? (((uint64_t)__float_as_uint(fminf(P.qbound[m], thres_n)) << 32) | 0xFFFFFFFFull) : KEY_EMPTY;
         double L[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) L[i] = key_d((i < K - kq) ? 0ull : KEY_EMPTY);
+        for (int i = 0; i < K; ++i) L[i] = key_d((i < K - kq) ? 0ull : init_key);
         // Keys that beat the list's bound wait in a small per-lane buffer and are inserted in batches: the
         // wave executes an insertion step only as often as its busiest lane needs one, instead of at every
         // candidate where any lane inserts. The list (and so the bound used for pruning) lags by at most
@@ -255,6 +272,7 @@ __global__ void __launch_bounds__(KBLK)
                 if (j < npend) topk_insert<K>(L, pbuf[j]);
                 pbuf[j] = KEY_EMPTY;
             }
+            inserted += (uint32_t)npend;
             npend = 0;
         };
 
@@ -423,6 +441,9 @@ __global__ void __launch_bounds__(KBLK)
             ++level;
         }
 
+        const bool handed_back = P.qbound && (uint32_t)key_u(L[K - 1]) == 0xFFFFFFFFu;
+        if (handed_back) P.retry[atomicAdd(&P.counters[6], 1ull)] = (uint32_t)m;
+        if (!handed_back) {
         // replay of the reference schedule on the exact list
         int c = 0;
         int lo[3] = {P.B, P.B, P.B}, hi[3] = {-1, -1, -1};
@@ -483,15 +504,17 @@ __global__ void __launch_bounds__(KBLK)
         if (!QUERY) P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
         invalid_row += valid ? 0u : 1u;
         fallback += fallback_q;
+        }
     }
     }
     // one atomic per wave
     unsigned long long c0 = cand;
-    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells;
+    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells, c7 = inserted;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         c0 += __shfl_xor(c0, d, 64);
         c4 += __shfl_xor(c4, d, 64);
+        c7 += __shfl_xor(c7, d, 64);
         c1 += __shfl_xor(c1, d, 64);
         c2 += __shfl_xor(c2, d, 64);
         c3 += __shfl_xor(c3, d, 64);
@@ -502,6 +525,7 @@ __global__ void __launch_bounds__(KBLK)
         if (c2) atomicAdd(&P.counters[2], (unsigned long long)c2);
         if (c3) atomicAdd(&P.counters[3], (unsigned long long)c3);
         if (c4) atomicAdd(&P.counters[4], (unsigned long long)c4);
+        if (c7) atomicAdd(&P.counters[7], (unsigned long long)c7);
     }
 }
 
@@ -563,11 +587,13 @@ int device_cus() {
 }
 
 template <int K>
-void launch_knn(const KnnParams& P, hipStream_t s, bool query) {
+void launch_knn(const KnnParams& P, hipStream_t s, bool query, bool list) {
     int grid = ceil_div(P.nchunks, KBLK / 64);
     if (grid < 1) return;
     if (query) {
         hipLaunchKernelGGL((k_knn<K, false, true>), dim3(grid), dim3(KBLK), 0, s, P);
+    } else if (list) {
+        hipLaunchKernelGGL((k_knn<K, false, false, true>), dim3(grid), dim3(KBLK), 0, s, P);
     } else if (P.queue) {
         // persistent: as many workgroups as fit on the GPU at once, each wave looping over the queue
         int per_cu = 0;
@@ -580,32 +606,38 @@ void launch_knn(const KnnParams& P, hipStream_t s, bool query) {
     LAUNCH_CHECK();
 }
 
-void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false) {
+void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false, bool list = false) {
     switch (K) {
-        case 8: launch_knn<8>(P, s, query); break;
-        case 16: launch_knn<16>(P, s, query); break;
-        case 24: launch_knn<24>(P, s, query); break;
-        case 32: launch_knn<32>(P, s, query); break;
-        case 40: launch_knn<40>(P, s, query); break;
-        case 48: launch_knn<48>(P, s, query); break;
-        case 56: launch_knn<56>(P, s, query); break;
-        default: launch_knn<64>(P, s, query); break;
+        case 8: launch_knn<8>(P, s, query, list); break;
+        case 16: launch_knn<16>(P, s, query, list); break;
+        case 24: launch_knn<24>(P, s, query, list); break;
+        case 32: launch_knn<32>(P, s, query, list); break;
+        case 40: launch_knn<40>(P, s, query, list); break;
+        case 48: launch_knn<48>(P, s, query, list); break;
+        case 56: launch_knn<56>(P, s, query, list); break;
+        default: launch_knn<64>(P, s, query, list); break;
     }
 }
 
-KnnParams knn_params(egt_ctx* h) {
+// Search parameters of the handle; with scratch, the per-point k-th distance, the counters and the bounded
+// search's hand-back list come from the handle's scratch (else they are left null for the caller).
+KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     KnnParams P{};
     P.mpts = h->mpts.p;
     P.mcell = h->mcell.p;
     P.mvcell = reinterpret_cast<const unsigned long long*>(h->mvcell.p);
     P.htab = h->htab.p;
     P.key = h->key.p;
-    P.qlevel = (h->learned && h->qlevel.p) ? h->qlevel.p : nullptr;
+    P.qlevel = (h->learned && (h->learned_mode & 1) && h->qlevel.p) ? h->qlevel.p : nullptr;
+    P.qbound = (h->learned && (h->learned_mode & 2) && h->qbound.p) ? h->qbound.p : nullptr;
+    P.qlist = nullptr;
+    P.nlist = 0;
+    P.retry = (scratch && P.qbound) ? h->scratch.get<uint32_t>(37, h->n) : nullptr;
     P.clouds = h->dclouds.p;
     P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     P.knn = h->knn.p;
-    P.kth = h->scratch.get<float>(15, h->n);
-    P.counters = reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 6));
+    P.kth = scratch ? h->scratch.get<float>(15, h->n) : nullptr;
+    P.counters = scratch ? reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 8)) : nullptr;
     P.queue = nullptr;
     P.bigmask = 0;
     for (int l = 0; l < h->nlev; ++l)
@@ -685,10 +717,11 @@ void run_knn_clouds(egt_ctx* h) {
     KnnParams P = knn_params(h);
     const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
     h->work_classes = classes;
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
     unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
+    int64_t handed_back = 0;
     for (const WorkClass& cr : classes) {
         P.chunks = h->nclouds > 1 ? h->work_lists.p + cr.first : nullptr;
         P.nchunks = cr.count;
@@ -697,12 +730,35 @@ void run_knn_clouds(egt_ctx* h) {
             P.queue = queues;
         }
         launch_any(P, cr.K, s);
+        if (P.qbound) {
+            // queries with fewer than k points within their proposed radius: searched again without the
+            // bound (list launch, each lane its own query), from the handle's start level
+            uint64_t nb = 0;
+            HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            if (nb > 0) {
+                KnnParams R = P;
+                R.qbound = nullptr;
+                R.qlevel = nullptr;
+                R.chunks = nullptr;
+                R.queue = nullptr;
+                R.qlist = P.retry;
+                R.nlist = (int64_t)nb;
+                R.nchunks = ((int64_t)nb + 63) / 64;
+                R.stride = 1;
+                launch_any(R, cr.K, s, false, true);
+                HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));
+                handed_back += (int64_t)nb;
+            }
+        }
     }
     HIP_CHECK(hipEventRecord(e1, s));
-    uint64_t hc[5];
+    uint64_t hc[8];
     HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
+    h->stats.learned_bound_retries = handed_back;
+    h->stats.knn_insertions = (int64_t)hc[7];
     h->stats.knn_candidates = (int64_t)hc[0];
     h->stats.invalid_rows = (int64_t)hc[1];
     h->stats.learned_queries = (int64_t)hc[2];
@@ -730,7 +786,7 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     h->qpts.alloc((size_t)std::max<int64_t>(nq, 1));
     h->qvcell.alloc((size_t)std::max<int64_t>(nq, 1));
     KnnParams P = knn_params(h);
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 6 * sizeof(uint64_t), s));
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     CloudDev qcd{};
@@ -760,6 +816,8 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
         P.clouds = h->dqcloud.p;
         P.mcloud = nullptr;
         P.qlevel = nullptr;
+        P.qbound = nullptr;
+        P.retry = nullptr;
         P.chunks = nullptr;
         P.queue = nullptr;
         P.stride = 1;
@@ -818,7 +876,7 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     double best_cost = 0.0;
     for (int l = -SUB; l <= top; ++l) {
         P.start_level = l + SUB;
-        HIP_CHECK(hipMemsetAsync(P.counters, 0, 5 * sizeof(uint64_t), s));
+        HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
         HIP_CHECK(hipEventRecord(e0, s));
         launch_any(P, K, s);
         HIP_CHECK(hipEventRecord(e1, s));

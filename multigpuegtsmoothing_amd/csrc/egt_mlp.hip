@@ -397,7 +397,8 @@ __global__ void __launch_bounds__(256) k_targets(const float4* __restrict__ mpts
     target[m] = good ? log2f(sqrtf(d) * inv_href) : 0.0f;
 }
 
-// Proposal: features -> MLP -> per-query start level, fused (no feature matrix in HBM). The level is the
+// Proposal: features -> MLP -> per-query start level and radius bound, fused (no feature matrix in HBM). The
+// bound is h_ref * 2^(y + bound_safety), squared; the search keeps only candidates within it. The level is the
 // smallest (from the finest sub-cell level -SUB up) whose 3x3x3 block around the query reaches beyond
 // r = h_ref * 2^(y + safety) on every side (reach along an axis = h_l * (1 + distance of the query to the
 // nearer face of its cell in cell units)).
@@ -411,7 +412,8 @@ __device__ inline uint64_t feature_cell(const FeatParams& F, int64_t m) {
 }
 
 __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ P, FeatParams F, float a, float ba,
-                                                       int nlev, float safety, int8_t* __restrict__ qlevel) {
+                                                       int nlev, float safety, float bound_safety,
+                                                       int8_t* __restrict__ qlevel, float* __restrict__ qbound) {
     __shared__ WaveLds lds[WPB];
     __shared__ int lead_s[WPB][TS];
     WaveLds& L = lds[threadIdx.x / 64];
@@ -449,6 +451,8 @@ __global__ void __launch_bounds__(MBLK) k_mlp_propose(const float* __restrict__ 
         if (l < TS && m < F.n) {
             const float href = ba / (float)(F.B >> F.clouds[F.mcloud ? F.mcloud[m] : 0].ref_level);
             const float r = href * exp2f(L.y[l] + safety);
+            const float rb = href * exp2f(L.y[l] + bound_safety);
+            qbound[m] = rb * rb;
             const float4 p = F.mpts[m];
             const float v[3] = {p.x, p.y, p.z};
             int lvl = nlev - 1;
@@ -719,6 +723,10 @@ void mlp_train(egt_model_s* m, egt_ctx* h, const int* ks, int epochs, int batch,
     const size_t qi = std::min<size_t>(ns - 1, (size_t)(qn * (float)(ns - 1)));
     std::nth_element(res.begin(), res.begin() + qi, res.end());
     m->safety = res[qi];
+    // radius bound: the 0.99 quantile of the residuals (a query beyond its bound is searched twice)
+    const size_t qb = std::min<size_t>(ns - 1, (size_t)(0.99 * (double)(ns - 1)));
+    std::nth_element(res.begin(), res.begin() + qb, res.end());
+    m->bound_safety = std::max(m->safety, res[qb]);
     m->rng = rs;
     m->train_samples = ns;
     if (loss_out) *loss_out = (float)(mse / (double)ns);
@@ -730,6 +738,7 @@ void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
     h->qlevel.alloc(n);
+    h->qbound.alloc(n);
     set_cloud_features(h, ks);
     const float a = -h->eps, ba = (1.0f + h->eps) - a;
     HIP_CHECK(hipStreamSynchronize(m->stream));  // parameters written on the model stream are final
@@ -737,7 +746,7 @@ void mlp_propose(egt_model_s* m, egt_ctx* h, const int* ks) {
     HIP_CHECK(hipEventRecord(e0, s));
     const int64_t tiles = (n + TS - 1) / TS;
     k_mlp_propose<<<(int)std::min<int64_t>(8192, (tiles + WPB - 1) / WPB), MBLK, 0, s>>>(
-        m->P.p, feat_params(h), a, ba, h->nlev, m->safety, h->qlevel.p);
+        m->P.p, feat_params(h), a, ba, h->nlev, m->safety, m->bound_safety, h->qlevel.p, h->qbound.p);
     LAUNCH_CHECK();
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(hipEventSynchronize(e1));

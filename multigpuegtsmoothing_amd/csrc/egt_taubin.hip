@@ -88,21 +88,30 @@ template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                      int64_t n, uint16_t* __restrict__ col16,
                                                      int32_t* __restrict__ halo, int32_t* __restrict__ hcount,
-                                                     uint32_t* __restrict__ perm, uint32_t* __restrict__ width,
+                                                     uint16_t* __restrict__ perm, uint32_t* __restrict__ width,
                                                      uint32_t* __restrict__ padded) {
     constexpr int HS = hset_size<TILE>(), HMAX = halo_max<TILE>();
     static_assert((HS & (HS - 1)) == 0 && HS >= TILE && HS >= HMAX + PBLK, "hash set size");
     __shared__ uint32_t hs[HS];      // hash set, then the row-sort keys
     __shared__ uint32_t uniq[HMAX];  // the halo, sorted
-    __shared__ uint32_t cnt, wsum[PBLK / 64];
+    __shared__ uint32_t cnt, selfrow, wsum[PBLK / 64];
     __shared__ uint32_t swid[TILE / 64];
     const int64_t t = blockIdx.x;
     const int64_t base = t * TILE, end = min<int64_t>(base + TILE, n);
     const int np = (int)(end - base);
     const int32_t eb = rowptr[base], ee = rowptr[end];
-    if (threadIdx.x == 0) cnt = 0;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        selfrow = 0;
+    }
     for (int i = threadIdx.x; i < HS; i += PBLK) hs[i] = HS_EMPTY;
     __syncthreads();
+    // A row that lists itself (a mesh vertex of a degenerate triangle, GridMeshCuda1.cu:115-125) keeps that
+    // entry's weight exp(0) = 1 in the reference's loop over the listed slots (EGT_CUDA.cu:242-270); the
+    // sliced tile pass tells padding slots by their column equal to the row, so such a tile runs as CSR rows.
+    for (int i = threadIdx.x; i < np; i += PBLK)
+        for (int32_t e = rowptr[base + i]; e < rowptr[base + i + 1]; ++e)
+            if (col[e] == base + i) selfrow = 1;
     for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
         const uint32_t c = (uint32_t)col[e];
         if ((int64_t)c >= base && (int64_t)c < end) continue;
@@ -120,7 +129,7 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     }
     __syncthreads();
     const uint32_t H = cnt;
-    if (H > (uint32_t)HMAX) {
+    if (H > (uint32_t)HMAX || selfrow) {
         if (threadIdx.x == 0) {
             hcount[t] = -1;
             padded[t] = 0;
@@ -169,9 +178,7 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
         col16[e] = loc;
     }
     // Lane order of the pass: rows sorted by length (ties by position), so the 64 rows of a slice have
-    // nearly equal lengths; a slice's width is its longest row. perm holds the local row in its low 16 bits
-    // and the row's length in the high 16 bits: the pass weights a slot only below that length, so a row
-    // that lists itself (a mesh vertex of a degenerate triangle) keeps that entry's weight.
+    // nearly equal lengths; a slice's width is its longest row.
     __syncthreads();
     for (int i = threadIdx.x; i < TILE; i += PBLK) {
         uint32_t key = 0xFFFFFFFFu;
@@ -183,9 +190,8 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     block_bitonic_sort(hs, TILE);
     for (int j = threadIdx.x; j < np; j += PBLK) {
         const uint32_t li = hs[j] & 0xFFFu;
-        const uint32_t len = (uint32_t)(rowptr[base + li + 1] - rowptr[base + li]);
-        perm[t * TILE + j] = li | (len << 16);
-        atomicMax(&swid[j >> 6], len);
+        perm[t * TILE + j] = (uint16_t)li;
+        atomicMax(&swid[j >> 6], (uint32_t)(rowptr[base + li + 1] - rowptr[base + li]));
     }
     __syncthreads();
     for (int s = threadIdx.x; s < TILE / 64; s += PBLK) width[t * (TILE / 64) + s] = swid[s];
@@ -200,13 +206,13 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
 // Sliced-ELLPACK columns of one tile: slice s (sorted positions 64 s .. 64 s + 63) stores its width W_s
 // slots as ceil(W_s / 2) slot pairs, pair-major: slots 2p and 2p + 1 of lane l share the 32-bit word
 // 64 p + l (low and high half), so a wave's load of one pair is 256 contiguous bytes. Slots past a row's
-// end (and the high half of an odd width's last pair) hold the row's own local index (any in-tile index
-// would do: the pass reads it but weights it 0 by slot position), lanes past the tile's last row hold 0. Slot
-// order within a row is the CSR (neighbour) order.
+// end (and the high half of an odd width's last pair) hold the row's own local index (a tile with a row
+// that lists itself runs as CSR rows instead), lanes past the tile's last row hold 0. Slot order within a
+// row is the CSR (neighbour) order.
 template <int TILE>
 __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ rowptr,
                                                      const uint16_t* __restrict__ col16, int64_t n,
-                                                     const uint32_t* __restrict__ perm,
+                                                     const uint16_t* __restrict__ perm,
                                                      const uint32_t* __restrict__ width,
                                                      const uint64_t* __restrict__ toff,
                                                      const int32_t* __restrict__ hcount, uint16_t* __restrict__ scol) {
@@ -223,7 +229,7 @@ __global__ void __launch_bounds__(PBLK) k_tile_sell(const int32_t* __restrict__ 
             const int j = s * 64 + (int)((x >> 1) & 63u);
             uint16_t v = 0;
             if (j < np) {
-                const int li = (int)(perm[t * TILE + j] & 0xFFFFu);
+                const int li = perm[t * TILE + j];
                 const int32_t e0 = rowptr[base + li];
                 v = (int32_t)u < rowptr[base + li + 1] - e0 ? col16[e0 + u] : (uint16_t)li;
             }
@@ -255,12 +261,9 @@ __device__ inline float taubin_weight(float dist, float scale) {
 // loads, a word past the row's end reading the valid word dummy[lane] instead, so the wait before a chunk
 // counts exactly the next chunk's loads. Packing two columns per word keeps both buffers in the registers
 // one unpacked chunk took (CH = 8; 16 slots in flight cost more in VGPRs, and so in occupancy, than they
-// save). Accumulation stays in slot order. A padding slot (slot position at or past the row's length len0 /
-// len1) holds the row's own index (self): its distance is 0, its weight is selected to 0 and it adds
-// fma(-0, 0, acc) = acc and sum + 0 = sum, so every lane runs every slot without a branch on the padding.
-// Padding is told by position, not by column value: a real entry equal to the row itself (a mesh vertex of a
-// degenerate triangle lists itself, GridMeshCuda1.cu:115-125) keeps its weight exp(0) = 1 (or 1 / 1e-8 in the
-// regularized mu pass), as in the reference's loop over the listed slots (EGT_CUDA.cu:242-270). The weighted difference uses w * (q - p) = (-w) * (p - q):
+// save). Accumulation stays in slot order. A padding slot holds the row's own index (self): its distance is
+// 0, its weight is selected to 0 and it adds fma(-0, 0, acc) = acc and sum + 0 = sum, so every lane runs
+// every slot without a branch on the padding. The weighted difference uses w * (q - p) = (-w) * (p - q):
 // q - p is exactly -(p - q), and an accumulator that starts at +0 never becomes -0, so the sign of a zero
 // difference cannot change a sum; the reuse saves three subtractions per neighbour.
 // With OVF, local columns at or past lim (TILE + the LDS part of the halo) are halo entries kept in global
@@ -283,13 +286,13 @@ __device__ inline void taubin_fetch(int g, int C0, int W0, int W1, const uint32_
 // ba holds chunk 0's columns (taubin_fetch(0, ...)), fetched by the caller.
 template <bool REG, bool OVF>
 __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uint32_t* sb0, const uint32_t* sb1,
-                                       const uint32_t* dummy, int lane, int W0, int W1, int r0, int r1, int len0,
-                                       int len1, float scale,
+                                       const uint32_t* dummy, int lane, int W0, int W1, int r0, int r1, float scale,
                                        const float4* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
                                        uint32_t tile, uint32_t (&ba)[TCH / 2], float4& res0, float4& res1) {
     constexpr int CH = TCH, CP = CH / 2;
     const int C0 = (W0 + CH - 1) / CH, C = C0 + (W1 + CH - 1) / CH;  // chunks of each row (wave-uniform)
-    // lanes past the last row (r = -1) have length 0: every slot of theirs is padding
+    // own index as the padding slots hold it; lanes past the last row (r = -1) match no column
+    const uint32_t self0 = (uint32_t)r0, self1 = (uint32_t)r1;
     float4 p = src[r0 < 0 ? 0 : r0];
     if (C0 == 0) {
         res0 = taubin_result(p, 0.0f, 0.0f, 0.0f, 0.0f, scale);
@@ -300,7 +303,7 @@ __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uin
     auto chunk = [&](int g, const uint32_t(&cw)[CP]) {
         const bool second = g >= C0;
         const int u0 = (second ? g - C0 : g) * CH, W = second ? W1 : W0;
-        const int len = second ? len1 : len0;
+        const uint32_t self = second ? self1 : self0;
         uint32_t c[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) c[u] = (u & 1) ? cw[u >> 1] >> 16 : cw[u >> 1] & 0xFFFFu;
@@ -320,7 +323,7 @@ __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uin
                 const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
                 const float dist = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 float w = taubin_weight<REG>(dist, scale);
-                w = u0 + u < len ? w : 0.0f;
+                w = c[u] != self ? w : 0.0f;
                 cgx = __builtin_fmaf(-w, dx, cgx);
                 cgy = __builtin_fmaf(-w, dy, cgy);
                 cgz = __builtin_fmaf(-w, dz, cgz);
@@ -354,7 +357,7 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
                                                        const uint16_t* __restrict__ scol,
                                                        const uint64_t* __restrict__ toff,
                                                        const uint32_t* __restrict__ width,
-                                                       const uint32_t* __restrict__ perm,
+                                                       const uint16_t* __restrict__ perm,
                                                        const int32_t* __restrict__ halo,
                                                        const int32_t* __restrict__ hcount, int64_t n, float scale,
                                                        int ntiles) {
@@ -369,7 +372,7 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     const int np = (int)(end - base);
     const int32_t H = hcount[t];
     if (H < 0) {
-        // halo larger than the tile: CSR rows with global gathers
+        // halo larger than the tile, or a row that lists itself: CSR rows with global gathers
         for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
             const float4 p = in[i];
             float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
@@ -398,16 +401,14 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     }
     v -= 128u * ((wd + 1u) >> 1);  // exclusive
     const uint16_t* tc = scol + toff[t];
-    int row[2], len[2];
+    int row[2];
     const uint32_t* sb[2];
     int W[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const int s = r == 0 ? wv : S - 1 - wv;
         const int j = s * 64 + lane;
-        const uint32_t pv = j < np ? perm[t * TILE + j] : 0u;
-        row[r] = j < np ? (int)(pv & 0xFFFFu) : -1;
-        len[r] = (int)(pv >> 16);
+        row[r] = j < np ? (int)perm[t * TILE + j] : -1;
         // slice offset and width are wave-uniform: keep them scalar so the slot loops branch on SGPRs
         sb[r] = reinterpret_cast<const uint32_t*>(tc + __builtin_amdgcn_readfirstlane(__shfl(v, s, 64)));
         W[r] = (int)__builtin_amdgcn_readfirstlane(__shfl(wd, s, 64));
@@ -444,12 +445,10 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     __syncthreads();
     float4 res[2];
     if (H > halo_cap<TILE>())
-        taubin_two_rows<REG, true>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], len[0], len[1],
-                                   scale, in, hal,
+        taubin_two_rows<REG, true>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], scale, in, hal,
                                    TILE + HL, TILE, ba, res[0], res[1]);
     else
-        taubin_two_rows<REG, false>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], len[0], len[1],
-                                    scale, in, hal,
+        taubin_two_rows<REG, false>(lds, sb[0], sb[1], dummy, lane, W[0], W[1], row[0], row[1], scale, in, hal,
                                     TILE + HL, TILE, ba, res[0], res[1]);
     __syncthreads();
 #pragma unroll
@@ -484,7 +483,7 @@ struct TilePlan {
     uint16_t* col16 = nullptr;   // CSR-order local columns
     int32_t* halo = nullptr;
     int32_t* hcount = nullptr;
-    uint32_t* perm = nullptr;    // per tile: sorted position -> local row | row length << 16
+    uint16_t* perm = nullptr;    // per tile: sorted position -> local row
     uint32_t* width = nullptr;   // per slice: width in slots
     uint64_t* toff = nullptr;    // per tile: start of its column block in scol
     uint16_t* scol = nullptr;    // sliced-ELLPACK local columns
@@ -501,7 +500,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
     tp.col16 = h->scratch.get<uint16_t>(19, std::max<int64_t>(h->nnz, 1));
     tp.halo = h->scratch.get<int32_t>(20, (size_t)tp.ntiles * halo_max<TILE>());
     tp.hcount = h->scratch.get<int32_t>(21, tp.ntiles);
-    tp.perm = h->scratch.get<uint32_t>(28, (size_t)tp.ntiles * TILE);
+    tp.perm = h->scratch.get<uint16_t>(28, (size_t)tp.ntiles * TILE);
     tp.width = h->scratch.get<uint32_t>(29, (size_t)tp.ntiles * (TILE / 64));
     uint32_t* padded = h->scratch.get<uint32_t>(30, tp.ntiles);
     tp.toff = h->scratch.get<uint64_t>(31, tp.ntiles);

@@ -26,6 +26,7 @@ EGT_OK = 0
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPORTED", 5: "IO_ERROR", 6: "STATE"}
 OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT, OPT_TAUBIN_TILE = 0, 1, 2, 3, 4, 5
 OPT_TAUBIN_REGULARIZED = 6
+OPT_LEARNED_MODE = 7         # learned proposal: 1 = start level (default), 2 = radius bound, 3 = both
 START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
 SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
 AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
@@ -49,7 +50,8 @@ class Stats(ctypes.Structure):
                 ("query_count", ctypes.c_int64), ("query_candidates", ctypes.c_int64),
                 ("query_invalid_rows", ctypes.c_int64), ("query_ms", ctypes.c_float), ("query_k", ctypes.c_int32),
                 ("taubin_pass_ms", ctypes.c_float), ("taubin_setup_ms", ctypes.c_float),
-                ("taubin_overflow_tiles", ctypes.c_int32)]
+                ("taubin_overflow_tiles", ctypes.c_int32), ("learned_bound_retries", ctypes.c_int64),
+                ("knn_insertions", ctypes.c_int64)]
 
     def as_dict(self):
         d = {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -133,6 +135,8 @@ def _load():
         "egt_model_train": ([_H, _H, i32, i32, i32, i64, f32, f32, ctypes.POINTER(f32)], st),
         "egt_model_get_safety": ([_H, ctypes.POINTER(f32)], st),
         "egt_model_set_safety": ([_H, f32], st),
+        "egt_model_get_bound_safety": ([_H, ctypes.POINTER(f32)], st),
+        "egt_model_set_bound_safety": ([_H, f32], st),
         "egt_model_propose": ([_H, _H, i32], st),
         "egt_model_train_batch": ([_H, _H, _I32P, i32, i32, i64, f32, f32, ctypes.POINTER(f32)], st),
         "egt_model_propose_batch": ([_H, _H, _I32P], st),
@@ -533,6 +537,17 @@ class RadiusModel:
     @safety.setter
     def safety(self, value):
         _check(_lib.egt_model_set_safety(self.handle, value))
+
+    @property
+    def bound_safety(self):
+        """Margin (log2 units) of the radius bound over the predicted radius."""
+        v = ctypes.c_float()
+        _check(_lib.egt_model_get_bound_safety(self.handle, ctypes.byref(v)))
+        return v.value
+
+    @bound_safety.setter
+    def bound_safety(self, value):
+        _check(_lib.egt_model_set_bound_safety(self.handle, value))
 
     def propose(self, provider, k):
         if isinstance(provider, BatchGrid):

@@ -328,9 +328,22 @@ int orc_grid_build(const float* x, const float* y, const float* z, int64_t n, fl
  * knn: n*k int32 (sorted space, -1 padding); kth: n floats (k-th distance of the exact list or -1);
  * invalid: n flags. stats[0] = invalid rows, stats[1] = rows where the literal reference schedule (list at
  * its own stop stage) differs from this result, stats[2] = total candidate distance evaluations. */
+int orc_knn_self_subset(const float* xs, const float* ys, const float* zs, const int32_t* key, int64_t n, float eps,
+                        int B, int k, float thres_n, int maxc, const int32_t* qsel, int64_t nsel, int32_t* knn,
+                        float* kth_out, int32_t* invalid, int64_t* stats);
+
 int orc_knn_self(const float* xs, const float* ys, const float* zs, const int32_t* key, int64_t n, float eps, int B,
                  int k, float thres_n, int maxc, int32_t* knn, float* kth_out, int32_t* invalid, int64_t* stats) {
-    if (n < 1 || k < 1 || B < 1) return -1;
+    return orc_knn_self_subset(xs, ys, zs, key, n, eps, B, k, thres_n, maxc, nullptr, n, knn, kth_out, invalid,
+                               stats);
+}
+
+/* The same search for the queries qsel[0..nsel) only (sorted indices; null = every point, nsel = n): output
+ * row t belongs to query qsel[t] (timing samples of large clouds). */
+int orc_knn_self_subset(const float* xs, const float* ys, const float* zs, const int32_t* key, int64_t n, float eps,
+                        int B, int k, float thres_n, int maxc, const int32_t* qsel, int64_t nsel, int32_t* knn,
+                        float* kth_out, int32_t* invalid, int64_t* stats) {
+    if (n < 1 || k < 1 || B < 1 || nsel < 0 || (!qsel && nsel != n)) return -1;
     GridGeom g;
     g.a = -eps;
     g.b = 1.0f + eps;
@@ -345,7 +358,8 @@ int orc_knn_self(const float* xs, const float* ys, const float* zs, const int32_
     const double margin = 2e-6;
 
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : n_invalid, n_literal_diff, n_evals)
-    for (int64_t q = 0; q < n; ++q) {
+    for (int64_t t = 0; t < nsel; ++t) {
+        const int64_t q = qsel ? (int64_t)qsel[t] : t;
         const float x = xs[q], y = ys[q], z = zs[q];
         const int fx = key[q] % B, fy = (key[q] / B) % B, fz = key[q] / (B * B);
         TopK L(k);
@@ -454,15 +468,15 @@ int orc_knn_self(const float* xs, const float* ys, const float* zs, const int32_
                 valid = true;
             }
         }
-        int32_t* row = knn + (size_t)q * k;
+        int32_t* row = knn + (size_t)t * k;
         if (valid) {
             for (int i = 0; i < k; ++i) row[i] = i < c ? L.e[i].j : -1;
-            kth_out[q] = c > 0 ? kth : -1.0f;
-            invalid[q] = 0;
+            kth_out[t] = c > 0 ? kth : -1.0f;
+            invalid[t] = 0;
         } else {
             for (int i = 0; i < k; ++i) row[i] = -1;
-            kth_out[q] = -1.0f;
-            invalid[q] = 1;
+            kth_out[t] = -1.0f;
+            invalid[t] = 1;
             ++n_invalid;
         }
         // literal schedule comparison
@@ -788,6 +802,19 @@ bool egg_keep(const float* xs, const float* ys, const float* zs, const int32_t* 
 int orc_egg(const float* xs, const float* ys, const float* zs, int64_t n, int k, float alpha, int32_t* knn) {
     parallel_for(int64_t i = 0; i < n; ++i) {
         int32_t* row = knn + (size_t)i * k;
+        const float x = xs[i], y = ys[i], z = zs[i];
+        for (int j = k - 1; j >= 0; --j)
+            if (row[j] != -1 && !egg_keep(xs, ys, zs, row, j, x, y, z, alpha)) row[j] = -1;
+    }
+    return 0;
+}
+
+/* The EGG filter of rows that belong to the points qsel[0..nsel) (row t is point qsel[t]'s). */
+int orc_egg_subset(const float* xs, const float* ys, const float* zs, const int32_t* qsel, int64_t nsel, int k,
+                   float alpha, int32_t* knn) {
+    parallel_for(int64_t t = 0; t < nsel; ++t) {
+        int32_t* row = knn + (size_t)t * k;
+        const int64_t i = qsel[t];
         const float x = xs[i], y = ys[i], z = zs[i];
         for (int j = k - 1; j >= 0; --j)
             if (row[j] != -1 && !egg_keep(xs, ys, zs, row, j, x, y, z, alpha)) row[j] = -1;

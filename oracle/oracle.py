@@ -28,6 +28,9 @@ def lib():
         i64, i32, f32 = ctypes.c_int64, ctypes.c_int, ctypes.c_float
         L.orc_grid_build.argtypes = [F32P, F32P, F32P, i64, f32, i32, F32P, F32P, F32P, I32P, I32P, F32P]
         L.orc_knn_self.argtypes = [F32P, F32P, F32P, I32P, i64, f32, i32, i32, f32, i32, I32P, F32P, I32P, I64P]
+        L.orc_knn_self_subset.argtypes = [F32P, F32P, F32P, I32P, i64, f32, i32, i32, f32, i32, I32P, i64, I32P,
+                                          F32P, I32P, I64P]
+        L.orc_egg_subset.argtypes = [F32P, F32P, F32P, I32P, i64, i32, f32, I32P]
         L.orc_knn_brute.argtypes = [F32P, F32P, F32P, i64, i32, f32, I32P]
         L.orc_knn_query.argtypes = [F32P, F32P, F32P, I32P, i64, F32P, F32P, F32P, i64, f32, f32, f32, i32, i32,
                                     f32, i32, I32P, F32P, I32P, I64P]
@@ -89,6 +92,20 @@ class Grid:
             raise ValueError("orc_knn_self failed")
         return knn.reshape(n, k), kth, inv, stats
 
+    def knn_subset(self, k, qsel, thres=1e8, maxc=100000):
+        """Rows of the sorted-space queries qsel only (row t is query qsel[t]), as knn() computes them."""
+        qsel = np.ascontiguousarray(qsel, np.int32)
+        m = qsel.size
+        knn = np.empty(max(m, 1) * k, np.int32)
+        kth = np.empty(max(m, 1), np.float32)
+        inv = np.empty(max(m, 1), np.int32)
+        stats = np.zeros(8, np.int64)
+        rc = lib().orc_knn_self_subset(self.xs, self.ys, self.zs, self.key, self.n, np.float32(self.eps), self.B, k,
+                                       self.thres_n(thres), maxc, qsel, m, knn, kth, inv, stats)
+        if rc != 0:
+            raise ValueError("orc_knn_self_subset failed")
+        return knn[:m * k].reshape(m, k), kth[:m], inv[:m], stats
+
     def knn_brute(self, k, thres=1e8):
         knn = np.empty(self.n * k, np.int32)
         lib().orc_knn_brute(self.xs, self.ys, self.zs, self.n, k, self.thres_n(thres), knn)
@@ -147,6 +164,15 @@ def egg(xs, ys, zs, knn, alpha):
     n, k = knn.shape
     out = np.ascontiguousarray(knn, np.int32).copy()
     lib().orc_egg(_f32(xs), _f32(ys), _f32(zs), n, k, np.float32(alpha), out.ravel())
+    return out
+
+
+def egg_subset(xs, ys, zs, qsel, rows, alpha):
+    """EGG filter of the rows of the points qsel (row t is point qsel[t]'s), as egg() filters them."""
+    qsel = np.ascontiguousarray(qsel, np.int32)
+    m, k = rows.shape
+    out = np.ascontiguousarray(rows, np.int32).copy()
+    lib().orc_egg_subset(_f32(xs), _f32(ys), _f32(zs), qsel, m, k, np.float32(alpha), out.ravel())
     return out
 
 

@@ -21,6 +21,11 @@ int orc_knn_query_brute(const float* xs, const float* ys, const float* zs, int64
                         const float* qy, const float* qz, int64_t nq, float min_extent, float max_extent, int k,
                         float thres_n, int32_t* knn);
 int orc_knn_brute(const float* xs, const float* ys, const float* zs, int64_t n, int k, float thres_n, int32_t* knn);
+int orc_knn_self_subset(const float* xs, const float* ys, const float* zs, const int32_t* key, int64_t n, float eps,
+                        int B, int k, float thres_n, int maxc, const int32_t* qsel, int64_t nsel, int32_t* knn,
+                        float* kth_out, int32_t* invalid, int64_t* stats);
+int orc_egg_subset(const float* xs, const float* ys, const float* zs, const int32_t* qsel, int64_t nsel, int k,
+                   float alpha, int32_t* knn);
 int orc_egg(const float* xs, const float* ys, const float* zs, int64_t n, int k, float alpha, int32_t* knn);
 int orc_taubin_f32(float* x, float* y, float* z, const int32_t* knn, int64_t n, int k, int iters, float lambda,
                    float mu, int regularized);
@@ -86,6 +91,18 @@ int main() {
                                ext[1], k, thres_n, qbrute.data());
     std::vector<int32_t> egg = knn;
     bad |= orc_egg(xs.data(), ys.data(), zs.data(), n, k, 0.65f, egg.data());
+    {  // every third query through the subset entry points
+        std::vector<int32_t> sel;
+        for (int64_t i = 0; i < n; i += 3) sel.push_back((int32_t)i);
+        const int64_t m = (int64_t)sel.size();
+        std::vector<int32_t> sk(m * k), sinv(m);
+        std::vector<float> skth(m);
+        bad |= orc_knn_self_subset(xs.data(), ys.data(), zs.data(), key.data(), n, 1e-6f, B, k, thres_n, 100000,
+                                   sel.data(), m, sk.data(), skth.data(), sinv.data(), stats.data());
+        bad |= orc_egg_subset(xs.data(), ys.data(), zs.data(), sel.data(), m, k, 0.65f, sk.data());
+        for (int64_t t = 0; t < m; ++t)
+            for (int j = 0; j < k; ++j) bad |= sk[t * k + j] != egg[(int64_t)sel[t] * k + j];
+    }
     std::vector<float> tx = xs, ty = ys, tz = zs;
     bad |= orc_taubin_f32(tx.data(), ty.data(), tz.data(), egg.data(), n, k, 5, 0.5f, -0.53f, 0);
     bad |= orc_taubin_f32(tx.data(), ty.data(), tz.data(), egg.data(), n, k, 2, 0.5f, -0.53f, 1);

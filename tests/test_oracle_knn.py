@@ -128,3 +128,18 @@ def test_radius_threshold(oracle_mod):
     valid = inv == 0
     np.testing.assert_array_equal(knn[valid], bf[valid])
     assert (knn[valid] == -1).any()
+
+
+def test_knn_and_egg_subsets_equal_full_rows(oracle_mod):
+    """The subset entry points (the CPU baseline's timing samples) give the full search's rows for the chosen
+    queries, and the subset EGG filter the full filter's rows."""
+    O = oracle_mod
+    g = O.Grid(*D.mixture(30_000, seed=5))
+    full, kth, inv, _ = g.knn(40)
+    sel = np.sort(np.random.default_rng(6).choice(g.n, 2_000, replace=False)).astype(np.int32)
+    sub, skth, sinv, _ = g.knn_subset(40, sel)
+    np.testing.assert_array_equal(sub, full[sel])
+    np.testing.assert_array_equal(skth, kth[sel])
+    np.testing.assert_array_equal(sinv, inv[sel])
+    egg_full = O.egg(g.xs, g.ys, g.zs, full, 0.65)
+    np.testing.assert_array_equal(O.egg_subset(g.xs, g.ys, g.zs, sel, sub, 0.65), egg_full[sel])
