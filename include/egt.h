@@ -140,6 +140,18 @@ egt_status egt_grid_knn_query(egt_handle h, const float* qx, const float* qy, co
  * the reference always copies from the host, GridKNNCuda.cu:1693-1703). */
 egt_status egt_grid_knn_query_device(egt_handle h, const float* qx, const float* qy, const float* qz, int64_t nq,
                                      int k, float thres);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): egt_grid_knn_query_device without any allocation.
+ * rows: caller's device buffer of nq*k int32 that receives the rows egt_get_device_query_knn would hand out
+ * (original query order, reference sorted indices, -1 holes); the handle's own query rows and stats are left
+ * as they were. At most 1,048,576 queries per call (larger counts return EGT_ERR_UNSUPPORTED). Temporaries live
+ * in a fixed device array of 56 MiB per GPU, allocated when the library's code loads on that GPU and held until
+ * the process ends; calls that overlap on one GPU (two threads, or two streams in flight) race on it. The work
+ * is queued on the handle's stream (egt_get_stream) without a host wait: the rows are ready when that stream
+ * completes. Needs the handle's device to be current and an up-to-date search structure (EGT_ERR_STATE after
+ * smoothing moved the points, until a query or self search rebuilds it). */
+egt_status egt_grid_knn_query_device_noalloc(egt_handle h, const float* qx, const float* qy, const float* qz,
+                                             int64_t nq, int k, float thres, int32_t* rows);
 /* GridStructure::GetKNNIndexes(int* knn) after a query search, GridKNNCuda.cu:1941-1968: host nq*k int32,
  * original query order, original reference indices, -1 holes. */
 egt_status egt_get_knn_query_original(egt_handle h, int32_t* out);

@@ -164,6 +164,30 @@ egt_status egt_grid_knn_query_device(egt_handle h, const float* qx, const float*
     return knn_query(h, qx, qy, qz, nq, k, thres, true);
 }
 
+egt_status egt_grid_knn_query_device_noalloc(egt_handle h, const float* qx, const float* qy, const float* qz,
+                                             int64_t nq, int k, float thres, int32_t* rows) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "the query search needs a one-cloud point-cloud handle");
+    if (nq < 0) return fail(EGT_ERR_INVALID_ARG, "query count must be >= 0");
+    if (nq > egt::QNA_MAX_QUERIES)
+        return fail(EGT_ERR_UNSUPPORTED, "more than 1048576 queries in one allocation-free call");
+    if (nq > 0 && (!qx || !qy || !qz || !rows)) return fail(EGT_ERR_INVALID_ARG, "null query or row array");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    if (h->search_stale)
+        return fail(EGT_ERR_STATE, "the points moved since the search structure was built; run egt_grid_knn_query "
+                                   "or egt_grid_knn_self once to rebuild it");
+    return guarded([&] {
+        int dev = -1;
+        HIP_CHECK(hipGetDevice(&dev));
+        if (dev != h->device) throw egt::ArgError{EGT_ERR_STATE, "the handle's device is not the current device"};
+        float t = thres / (h->max_extent - h->min_extent);  // GridKNNCuda.cu:1649-1650
+        t *= t;
+        egt::run_knn_query_noalloc(h, qx, qy, qz, nq, k, t, rows);
+    });
+}
+
 egt_status egt_get_knn_query_original(egt_handle h, int32_t* out) {
     if (need(h)) return EGT_ERR_INVALID_ARG;
     if (!h->have_query) return fail(EGT_ERR_STATE, "no query search yet");

@@ -261,6 +261,9 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, hipStream_
 void exclusive_scan_u32_dev(const uint32_t* in, uint32_t* out, int64_t n, hipStream_t s, Scratch& sc,
                             uint32_t* total_dev);
 void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, Scratch& sc);
+// The same sort over a caller's workspace of radix_sort_workspace_words(n) 32-bit words (no allocation).
+size_t radix_sort_workspace_words(int64_t n);
+void radix_sort_pairs_ws(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, uint32_t* ws);
 // Sum of n uint32 values in 64 bits (synchronizes the stream).
 uint64_t sum_u32(const uint32_t* in, int64_t n, hipStream_t s, Scratch& sc);
 uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
@@ -314,6 +317,13 @@ void run_knn_original(egt_ctx* h, int32_t* out);
 void run_knn_original_cloud(egt_ctx* h, int cloud, int32_t* out);
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Query search without allocation (egt_grid_knn_query_device_noalloc): at most QNA_MAX_QUERIES queries per
+// call; its temporaries live in a file-scope device array of QNA_WORDS 32-bit words (one copy per GPU).
+constexpr int64_t QNA_MAX_QUERIES = (int64_t)1 << 20;
+constexpr size_t QNA_WORDS = 14 * (size_t)QNA_MAX_QUERIES + 64;
+void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const float* dqz, int64_t nq, int k,
+                           float thres_n, int32_t* rows);
 
 // One-cloud handles: mirror the handle's k into the cloud table (rows start at 0) and upload it.
 inline void sync_single_cloud(egt_ctx* h) {

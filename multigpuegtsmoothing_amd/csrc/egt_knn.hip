@@ -18,6 +18,7 @@
 // decide whether the reference would have blanked the row (cell with >= maxc points, or still searching
 // after the bucket-count-8 ring).
 #include <algorithm>
+#include <mutex>
 #include "egt_device.h"
 #include "egt_internal.h"
 
@@ -845,6 +846,97 @@ void run_knn_query_original(egt_ctx* h, int32_t* out) {
     LAUNCH_CHECK();
     HIP_CHECK(hipMemcpyAsync(out, tmp, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Clears the counters and writes the query set's cloud entry (kernel arguments, so the sequence holds no host
+// copy and no memset).
+__global__ void k_qna_init(unsigned long long* __restrict__ counters, CloudDev* __restrict__ cloud, CloudDev c) {
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) *cloud = c;
+}
+
+}  // namespace egt
+
+// Temporaries of the query search without allocation: one copy per GPU, held until the process ends. Host code
+// never uses this name as an address (on the host it only names the symbol): each device's address comes
+// from hipGetSymbolAddress with that device current.
+static __device__ __align__(256) uint32_t scratch[egt::QNA_WORDS];
+
+namespace egt {
+namespace {
+
+constexpr int QNA_DEVICES = 64;
+
+// Address of this device's copy of the scratch array, looked up once per device (dev must be current).
+uint32_t* qna_scratch(int dev) {
+    static std::mutex mu;
+    static uint32_t* ptr[QNA_DEVICES] = {};
+    if (dev < 0 || dev >= QNA_DEVICES) throw ArgError{EGT_ERR_UNSUPPORTED, "device index beyond 63"};
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    return ptr[dev];
+}
+
+}  // namespace
+
+// The query-vs-reference search of run_knn_query with every temporary in the device scratch array and the rows
+// written to the caller's buffer (original query order, reference sorted indices): no allocation, no memset,
+// no host copy and no synchronization; the work is queued on the handle's stream. Layout in 32-bit words,
+// NQ = QNA_MAX_QUERIES: normalized queries (4 NQ), their virtual cells (2 NQ), sort keys (NQ) and values
+// (NQ), the queries in Morton order (4 NQ; the radix sort's workspace before they are written), their cells
+// (2 NQ), then the counters and the query set's cloud entry.
+void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const float* dqz, int64_t nq, int k,
+                           float thres_n, int32_t* rows) {
+    if (nq <= 0) return;
+    if (nq > QNA_MAX_QUERIES) throw ArgError{EGT_ERR_UNSUPPORTED, "more queries than the scratch holds"};
+    constexpr size_t NQ = (size_t)QNA_MAX_QUERIES;
+    static_assert(QNA_WORDS >= 14 * NQ + 16 + (sizeof(CloudDev) + 3) / 4, "scratch layout");
+    if (radix_sort_workspace_words(nq) > 4 * NQ) throw ArgError{EGT_ERR_UNSUPPORTED, "sort workspace"};
+    uint32_t* w = qna_scratch(h->device);
+    float4* qn = reinterpret_cast<float4*>(w);
+    uint64_t* cellp = reinterpret_cast<uint64_t*>(w + 4 * NQ);
+    uint32_t* code = w + 6 * NQ;
+    uint32_t* idx = w + 7 * NQ;
+    float4* qpts = reinterpret_cast<float4*>(w + 8 * NQ);
+    uint64_t* qcell = reinterpret_cast<uint64_t*>(w + 12 * NQ);
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(w + 14 * NQ);
+    CloudDev* qcloud = reinterpret_cast<CloudDev*>(w + 14 * NQ + 16);
+    hipStream_t s = h->stream;
+    KnnParams P = knn_params(h, false);
+    CloudDev qcd{};
+    qcd.base = 0;
+    qcd.n = nq;
+    qcd.row_off = 0;
+    qcd.k = k;
+    qcd.thres_n = thres_n;
+    k_qna_init<<<1, 64, 0, s>>>(counters, qcloud, qcd);
+    LAUNCH_CHECK();
+    const int g = ceil_div(nq, KBLK);
+    k_query_prep<<<g, KBLK, 0, s>>>(dqx, dqy, dqz, nq, h->min_extent, h->max_extent, P.a, P.ba, h->B, qn, cellp,
+                                    code, idx);
+    LAUNCH_CHECK();
+    radix_sort_pairs_ws(code, idx, nq, 30, s, w + 8 * NQ);
+    k_query_gather<<<g, KBLK, 0, s>>>(qn, cellp, idx, nq, qpts, qcell);
+    LAUNCH_CHECK();
+    P.qpts = qpts;
+    P.qvcell = reinterpret_cast<const unsigned long long*>(qcell);
+    P.knn = rows;
+    P.kth = nullptr;
+    P.counters = counters;
+    P.clouds = qcloud;
+    P.mcloud = nullptr;
+    P.qlevel = nullptr;
+    P.qbound = nullptr;
+    P.retry = nullptr;
+    P.chunks = nullptr;
+    P.queue = nullptr;
+    P.stride = 1;
+    P.nchunks = (nq + 63) / 64;
+    launch_any(P, kclass(k), s, true);
 }
 
 // Autotuner for the search's start level (the cell size of the first search box). For each candidate level

@@ -305,6 +305,29 @@ void radix_sort_pairs(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, h
     radix_passes(keys, vals, n, key_bits, s, ak.p, av.p, cnt.p, off.p, scratch_parts(sc));
 }
 
+size_t radix_sort_workspace_words(int64_t n) {
+    const int64_t nb = (n + SORT_TILE - 1) / SORT_TILE;
+    const int64_t c = ((int64_t)1 << RADIX_BITS) * nb;  // count and offset tables
+    const int64_t p0 = (c + SCAN_TILE - 1) / SCAN_TILE, p1 = (p0 + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t p2 = (p1 + SCAN_TILE - 1) / SCAN_TILE;
+    return (size_t)(2 * n + 2 * c + 2 * (p0 + p1 + p2));
+}
+
+void radix_sort_pairs_ws(uint32_t* keys, uint32_t* vals, int64_t n, int key_bits, hipStream_t s, uint32_t* ws) {
+    if (n <= 1) return;
+    const int64_t nb = (n + SORT_TILE - 1) / SORT_TILE, c = ((int64_t)1 << RADIX_BITS) * nb;
+    const int64_t p0 = (c + SCAN_TILE - 1) / SCAN_TILE, p1 = (p0 + SCAN_TILE - 1) / SCAN_TILE;
+    uint32_t* ak = ws;
+    uint32_t* av = ak + n;
+    uint32_t* cnt = av + n;
+    uint32_t* off = cnt + c;
+    uint32_t* parts = off + c;  // depth 0: 2 p0 words, depth 1: 2 p1, depth 2: the rest
+    auto part = [parts, p0, p1](int depth, size_t) {
+        return depth == 0 ? parts : (depth == 1 ? parts + 2 * p0 : parts + 2 * (p0 + p1));
+    };
+    radix_passes(keys, vals, n, key_bits, s, ak, av, cnt, off, part);
+}
+
 static DBuf<uint32_t>& part_buf(Scratch& sc, int i) { return sc.bufs[i]; }
 
 }  // namespace egt

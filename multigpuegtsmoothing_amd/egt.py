@@ -95,6 +95,8 @@ def _load():
         "egt_grid_knn_self": ([_H, i32, f32], st),
         "egt_grid_knn_query": ([_H, _F32P, _F32P, _F32P, i64, i32, f32], st),
         "egt_grid_knn_query_device": ([_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, i64, i32, f32], st),
+        "egt_grid_knn_query_device_noalloc": ([_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, i64, i32, f32,
+                                               ctypes.c_void_p], st),
         "egt_get_knn_query_original": ([_H, _I32P], st),
         "egt_get_device_query_knn": ([_H, _PF, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int32)], st),
         "egt_mesh_create": ([_F32P, _F32P, _F32P, i64, _U32P, i64, i32, f32, i32, _PH], st),
