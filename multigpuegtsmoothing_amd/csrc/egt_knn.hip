@@ -587,6 +587,13 @@ int device_cus() {
     return cus;
 }
 
+// dst[i] = src[i] (the values of a key-only sort).
+__global__ void __launch_bounds__(KBLK) k_copy_u32(const uint32_t* __restrict__ src, int64_t n,
+                                                     uint32_t* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * KBLK + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 template <int K>
 void launch_knn(const KnnParams& P, hipStream_t s, bool query, bool list) {
     int grid = ceil_div(P.nchunks, KBLK / 64);
@@ -633,7 +640,8 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     P.qbound = (h->learned && (h->learned_mode & 2) && h->qbound.p) ? h->qbound.p : nullptr;
     P.qlist = nullptr;
     P.nlist = 0;
-    P.retry = (scratch && P.qbound) ? h->scratch.get<uint32_t>(37, h->n) : nullptr;
+    // the hand-back list and, after it, room for the values of its sort
+    P.retry = (scratch && P.qbound) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
     P.clouds = h->dclouds.p;
     P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     P.knn = h->knn.p;
@@ -738,6 +746,13 @@ void run_knn_clouds(egt_ctx* h) {
             HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
             if (nb > 0) {
+                // in Morton order again, so a wave's 64 queries are spatial neighbours (the list fills in the
+                // order the waves reach their atomics)
+                int bits = 1;
+                while (bits < 31 && ((int64_t)1 << bits) < h->n) ++bits;
+                k_copy_u32<<<ceil_div((int64_t)nb, KBLK), KBLK, 0, s>>>(P.retry, (int64_t)nb, P.retry + h->n);
+                LAUNCH_CHECK();
+                radix_sort_pairs(P.retry, P.retry + h->n, (int64_t)nb, bits, s, h->scratch);
                 KnnParams R = P;
                 R.qbound = nullptr;
                 R.qlevel = nullptr;
