@@ -77,6 +77,7 @@ typedef struct {
     int64_t learned_bound_retries; /* last kNN: queries with fewer than k points within their proposed radius,
                                       searched again without the bound */
     int64_t knn_insertions;    /* last kNN: keys inserted into the neighbour lists */
+    int64_t knn_max_candidates; /* last kNN: candidate distances of its costliest query */
 } egt_stats;
 
 /* EGT_OPT_START_LEVEL value: choose the level from the mean cell occupancy. */

@@ -68,7 +68,8 @@ struct KnnParams {
     float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
     unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks,
                                                 // [4] cells looked up, [5] work queue, [6] bound retries,
-                                                // [7] keys inserted into the lists
+                                                // [7] keys inserted into the lists, [8] most candidates of a
+                                                // query
     const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
     const float4* __restrict__ qpts;       // query search: queries in Morton order (x', y', z', bits(index))
     const unsigned long long* __restrict__ qvcell;  // query search: virtual cell, clamped to the grid
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(KBLK)
     __attribute__((amdgpu_waves_per_eu(K <= 8 || (K <= 16 && !PERSIST) ? 4 : (K <= 40 && !PERSIST ? 3 : 1))))
     k_knn(KnnParams P) {
     uint64_t cand = 0;
-    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0, inserted = 0;
+    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0, inserted = 0, most = 0;
     // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
     // persistent launch, chunk after chunk from a work queue until the list is exhausted.
     for (int64_t v = next_chunk<PERSIST>(P, -1); v < P.nchunks; v = next_chunk<PERSIST>(P, v)) {
@@ -237,6 +238,7 @@ __global__ void __launch_bounds__(KBLK)
         learned += active ? 1u : 0u;
     }
     if (active) {
+        const uint64_t cand0 = cand;
         const float4 q = QUERY ? P.qpts[m] : P.mpts[m];
         const int32_t jq = __float_as_int(q.w);
         const uint64_t qvc = QUERY ? P.qvcell[m] : P.mvcell[m];
@@ -442,6 +444,7 @@ __global__ void __launch_bounds__(KBLK)
             ++level;
         }
 
+        most = max(most, (uint32_t)min<uint64_t>(cand - cand0, 0xFFFFFFFFull));
         const bool handed_back = P.qbound && (uint32_t)key_u(L[K - 1]) == 0xFFFFFFFFu;
         if (handed_back) P.retry[atomicAdd(&P.counters[6], 1ull)] = (uint32_t)m;
         if (!handed_back) {
@@ -510,12 +513,13 @@ __global__ void __launch_bounds__(KBLK)
     }
     // one atomic per wave
     unsigned long long c0 = cand;
-    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells, c7 = inserted;
+    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells, c7 = inserted, c8 = most;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         c0 += __shfl_xor(c0, d, 64);
         c4 += __shfl_xor(c4, d, 64);
         c7 += __shfl_xor(c7, d, 64);
+        c8 = max(c8, (uint32_t)__shfl_xor(c8, d, 64));
         c1 += __shfl_xor(c1, d, 64);
         c2 += __shfl_xor(c2, d, 64);
         c3 += __shfl_xor(c3, d, 64);
@@ -527,6 +531,7 @@ __global__ void __launch_bounds__(KBLK)
         if (c3) atomicAdd(&P.counters[3], (unsigned long long)c3);
         if (c4) atomicAdd(&P.counters[4], (unsigned long long)c4);
         if (c7) atomicAdd(&P.counters[7], (unsigned long long)c7);
+        if (c8) atomicMax(&P.counters[8], (unsigned long long)c8);
     }
 }
 
@@ -646,7 +651,7 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     P.knn = h->knn.p;
     P.kth = scratch ? h->scratch.get<float>(15, h->n) : nullptr;
-    P.counters = scratch ? reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 8)) : nullptr;
+    P.counters = scratch ? reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 9)) : nullptr;
     P.queue = nullptr;
     P.bigmask = 0;
     for (int l = 0; l < h->nlev; ++l)
@@ -726,7 +731,7 @@ void run_knn_clouds(egt_ctx* h) {
     KnnParams P = knn_params(h);
     const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
     h->work_classes = classes;
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
     unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
@@ -769,12 +774,13 @@ void run_knn_clouds(egt_ctx* h) {
         }
     }
     HIP_CHECK(hipEventRecord(e1, s));
-    uint64_t hc[8];
+    uint64_t hc[9];
     HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
     h->stats.learned_bound_retries = handed_back;
     h->stats.knn_insertions = (int64_t)hc[7];
+    h->stats.knn_max_candidates = (int64_t)hc[8];
     h->stats.knn_candidates = (int64_t)hc[0];
     h->stats.invalid_rows = (int64_t)hc[1];
     h->stats.learned_queries = (int64_t)hc[2];
@@ -802,7 +808,7 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     h->qpts.alloc((size_t)std::max<int64_t>(nq, 1));
     h->qvcell.alloc((size_t)std::max<int64_t>(nq, 1));
     KnnParams P = knn_params(h);
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     CloudDev qcd{};
@@ -863,10 +869,14 @@ void run_knn_query_original(egt_ctx* h, int32_t* out) {
     HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// Counters of the search kernel (KnnParams::counters) in the allocation-free query search's scratch.
+constexpr int QNA_COUNTERS = 16;
+static_assert(QNA_COUNTERS >= 9, "every counter k_knn writes");
+
 // Clears the counters and writes the query set's cloud entry (kernel arguments, so the sequence holds no host
 // copy and no memset).
 __global__ void k_qna_init(unsigned long long* __restrict__ counters, CloudDev* __restrict__ cloud, CloudDev c) {
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0ull;
+    if (threadIdx.x < QNA_COUNTERS) counters[threadIdx.x] = 0ull;
     if (threadIdx.x == 0) *cloud = c;
 }
 
@@ -909,7 +919,7 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
     if (nq <= 0) return;
     if (nq > QNA_MAX_QUERIES) throw ArgError{EGT_ERR_UNSUPPORTED, "more queries than the scratch holds"};
     constexpr size_t NQ = (size_t)QNA_MAX_QUERIES;
-    static_assert(QNA_WORDS >= 14 * NQ + 16 + (sizeof(CloudDev) + 3) / 4, "scratch layout");
+    static_assert(QNA_WORDS >= 14 * NQ + 2 * QNA_COUNTERS + (sizeof(CloudDev) + 3) / 4, "scratch layout");
     if (radix_sort_workspace_words(nq) > 4 * NQ) throw ArgError{EGT_ERR_UNSUPPORTED, "sort workspace"};
     uint32_t* w = qna_scratch(h->device);
     float4* qn = reinterpret_cast<float4*>(w);
@@ -919,7 +929,7 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
     float4* qpts = reinterpret_cast<float4*>(w + 8 * NQ);
     uint64_t* qcell = reinterpret_cast<uint64_t*>(w + 12 * NQ);
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(w + 14 * NQ);
-    CloudDev* qcloud = reinterpret_cast<CloudDev*>(w + 14 * NQ + 16);
+    CloudDev* qcloud = reinterpret_cast<CloudDev*>(w + 14 * NQ + 2 * QNA_COUNTERS);
     hipStream_t s = h->stream;
     KnnParams P = knn_params(h, false);
     CloudDev qcd{};
@@ -983,7 +993,7 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     double best_cost = 0.0;
     for (int l = -SUB; l <= top; ++l) {
         P.start_level = l + SUB;
-        HIP_CHECK(hipMemsetAsync(P.counters, 0, 8 * sizeof(uint64_t), s));
+        HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
         HIP_CHECK(hipEventRecord(e0, s));
         launch_any(P, K, s);
         HIP_CHECK(hipEventRecord(e1, s));

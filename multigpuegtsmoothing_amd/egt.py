@@ -51,7 +51,7 @@ class Stats(ctypes.Structure):
                 ("query_invalid_rows", ctypes.c_int64), ("query_ms", ctypes.c_float), ("query_k", ctypes.c_int32),
                 ("taubin_pass_ms", ctypes.c_float), ("taubin_setup_ms", ctypes.c_float),
                 ("taubin_overflow_tiles", ctypes.c_int32), ("learned_bound_retries", ctypes.c_int64),
-                ("knn_insertions", ctypes.c_int64)]
+                ("knn_insertions", ctypes.c_int64), ("knn_max_candidates", ctypes.c_int64)]
 
     def as_dict(self):
         d = {f[0]: getattr(self, f[0]) for f in self._fields_}
