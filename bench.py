@@ -556,10 +556,11 @@ def torus_result(args):
 def mixture_modes(args, k=40, steps=5, warmup=2):
     """The headline workload: 16M mixture points (seed 4) at neighbour count k, alpha 0.65, 150 Taubin
     iterations. Paths (each warm-up + `steps` timed steps of grid + (proposal) + kNN + EGG + Taubin):
-    plain_own_cell (every query from its own cell, the reference schedule), plain_autotuned (the level the
-    autotuner picks for this cloud, tuned once outside the timing) and learned (the model's start level and
-    radius bound per query; model trained on the GPU, outside the timing, on the plain search's radii over
-    2M mixture points of seed 1000). Rows of the three are compared on the GPU after the timing."""
+    plain_own_cell (every query from its own cell, the reference schedule, one lane per query), plain_autotuned
+    (the level the autotuner picks for this cloud and the launch shape the shape tuner picks for it, tuned once
+    outside the timing) and learned (the model's start level per query and the launch shape tuned for it;
+    model trained on the GPU, outside the timing, on the plain search's radii over 2M mixture points of seed
+    1000). Rows of the three are compared on the GPU after the timing."""
     import torch
 
     from multigpuegtsmoothing_amd import datasets as D
@@ -581,11 +582,21 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
     g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
     t0 = time.perf_counter()
     lvl = int(g.autotune(k)[0])
+    # launch shape (lane per query / persistent / wave per query) for the tuned level and for the learned
+    # starts, each timed on the tuner's sample, outside the timed steps
+    g.set_option(E.OPT_START_LEVEL, lvl)
+    shapes = {"plain_own_cell": (0, None), "plain_autotuned": g.autotune_shape(k)}
+    g.close()
+    g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
+    model.propose(g, k)
+    shapes["learned"] = g.autotune_shape(k)
     tune_s = time.perf_counter() - t0
     g.close()
     holder = {}
 
     def make_step(mode):
+        shape = shapes[mode][0]
+
         def step():
             g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
             if args.no_graph:
@@ -594,6 +605,8 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
                 model.propose(g, k)
             elif mode == "plain_autotuned":
                 g.set_option(E.OPT_START_LEVEL, lvl)
+            g.set_option(E.OPT_KNN_PERSISTENT, 1 if shape == 1 else 0)
+            g.set_option(E.OPT_KNN_LANES, 64 if shape == 2 else 1)
             g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
             E.EGT_CUDA(g).PerformSmoothing(iters, 0.5, -0.53, 0.65)
             st = g.stats()
@@ -619,7 +632,10 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
                      "taubin_pass_ms": st["taubin_pass_ms"], "taubin_tile": st["taubin_tile"],
                      "egg_kept": st["egg_kept"],
                      "candidates_per_query": st["knn_candidates"] / n, "cells_per_query": st["knn_cells"] / n,
-                     "insertions_per_query": st["knn_insertions"] / n, "invalid_rows": st["invalid_rows"]}
+                     "insertions_per_query": st["knn_insertions"] / n, "invalid_rows": st["invalid_rows"],
+                     "knn_shape": ("lane per query", "persistent, lane per query", "wave per query")[shapes[mode][0]]}
+        if shapes[mode][1] is not None:
+            per[mode]["shape_sample_ms"] = [round(float(x), 4) for x in shapes[mode][1]]
         if mode == "learned":
             per[mode]["fallback_rate"] = st["learned_fallbacks"] / n
             per[mode]["bound_retry_rate"] = st["learned_bound_retries"] / n

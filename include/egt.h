@@ -102,10 +102,13 @@ typedef enum {
     EGT_OPT_TAUBIN_REGULARIZED = 6, /* Taubin weights: 0 = exp(-d^2) (default, as the reference runs); 1 = the
                                       reference's dormant isRegularized variant, EGT_CUDA.cu:263, 544: weight
                                       1 in the lambda pass, 1 / (d^2 + 1e-8) in the mu pass */
-    EGT_OPT_LEARNED_MODE = 7     /* kNN with the learned proposal: bit 0 = start at the proposed level, bit 1 =
+    EGT_OPT_LEARNED_MODE = 7,    /* kNN with the learned proposal: bit 0 = start at the proposed level, bit 1 =
                                     bound the search by the proposed radius (queries with fewer than k points
                                     inside it are searched again without the bound); default 1 (the level only:
                                     the bound measured slower). Rows do not depend on it */
+    EGT_OPT_KNN_LANES = 8        /* kNN self search: lanes per query, 1 = one lane per query with the list in its
+                                    registers (default), 64 = one wave per query with the list spread over the
+                                    lanes. Rows do not depend on it */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -225,6 +228,16 @@ egt_status egt_release_cached_memory(uint64_t* released, uint64_t* cached);
  * afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, float* trial_ms,
                         float* trial_cost);
+
+/* Launch-shape tuner (no reference counterpart): times the handle's current search (its start level and, with
+ * EGT_OPT_LEARNED, the proposal in place) with neighbour count k and radius thres on the autotuner's sample in
+ * each launch shape: 0 = one lane per query, one workgroup per 256 queries; 1 = one lane per query, one
+ * persistent work-queue launch; 2 = one wave per query (not offered with the radius bound). Each shape runs
+ * once to warm up and twice timed; the fastest (best of its two) is kept through EGT_OPT_KNN_PERSISTENT and
+ * EGT_OPT_KNN_LANES. Rows do not depend on the choice. *shape (optional) receives it, shape_ms (optional, 3
+ * floats) each shape's sample time in ms, -1 where not offered. The handle's neighbour rows are not valid
+ * afterwards: run egt_grid_knn_self. */
+egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms);
 
 /* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA
  * leaked (EGT_CUDA.cu:490-492). */

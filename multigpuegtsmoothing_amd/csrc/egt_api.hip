@@ -357,6 +357,10 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
             if (value < 1 || value > 3) return fail(EGT_ERR_INVALID_ARG, "learned mode must be 1, 2 or 3");
             h->learned_mode = (int)value;
             return EGT_OK;
+        case EGT_OPT_KNN_LANES:
+            if (value != 1 && value != 64) return fail(EGT_ERR_INVALID_ARG, "kNN lanes per query must be 1 or 64");
+            h->knn_lanes = (int)value;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }
@@ -380,6 +384,25 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
             if (trial_ms) trial_ms[i] = ms[i];
             if (trial_cost) trial_cost[i] = cost[i];
         }
+    });
+}
+
+egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "the shape tuner needs a one-cloud point-cloud handle");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    return guarded([&] {
+        float t = thres / (h->max_extent - h->min_extent);
+        t *= t;
+        if (h->search_stale) egt::refresh_search_structure(h);
+        float ms[3];
+        const int best = egt::autotune_shape(h, k, t, ms);
+        h->csr_valid = false;
+        if (shape) *shape = best;
+        if (shape_ms)
+            for (int i = 0; i < 3; ++i) shape_ms[i] = ms[i];
     });
 }
 

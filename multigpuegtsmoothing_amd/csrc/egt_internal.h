@@ -229,6 +229,7 @@ struct egt_ctx {
     int learned_mode = 1;  // with learned: 1 = start at the proposed level, 2 = bound the search by the proposed
                            // radius, 3 = both
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
+    int knn_lanes = 1;       // kNN self search: 1 = one lane per query, 64 = one wave per query
     int taubin_graph = 1;
     int taubin_tile = 1024;  // Taubin tile width: 1024 or 2048 points; 0 = tune on the next smoothing call
     int taubin_regularized = 0;  // isRegularized weights (1 / inverse-square), off as in the reference
@@ -283,6 +284,9 @@ void run_knn_query_original(egt_ctx* h, int32_t* out);
 void run_knn_clouds(egt_ctx* h);
 int auto_start_level(const egt_ctx* h, int k, int cloud = 0);
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost);
+// Times the current search in each launch shape on a sample and keeps the fastest: 0 = one lane per query,
+// 1 = the same as one persistent launch, 2 = one wave per query. shape_ms: 3 sample times, -1 if not offered.
+int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms);
 
 // learned proposal (egt_mlp.hip)
 int mlp_param_count();

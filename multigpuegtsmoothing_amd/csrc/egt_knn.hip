@@ -31,6 +31,8 @@ constexpr int KBLK = 256;
 // 16M-point cluster mixture from the own cell (76.2 ms at 48, 77.1 at 96; 78.6 with 96 above the fine level
 // and 48 below).
 constexpr uint32_t SPLIT_COUNT = 96;
+// The same for the wave-per-query search, whose batches of 64 make scanning a populous cell cheaper.
+constexpr uint32_t WAVE_SPLIT_COUNT = 96;
 // Keys (distance bits << 32 | sorted index) of non-negative distances, read as IEEE doubles, are positive
 // and order exactly as the integers do, so the list is held as doubles and merged with FP64 min / max (full
 // rate on gfx950; keys of zero distance are denormal doubles, which compute kernels keep). The empty key is
@@ -535,6 +537,392 @@ __global__ void __launch_bounds__(KBLK)
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Wave-per-query search (launch shape EGT_OPT_KNN_LANES = 64): the same exact search and replay as k_knn, with
+// the 64 lanes of a wave working on one query at a time. Per box: lane t looks up the t-th of the 27 cells
+// (nearest first) in parallel, populous cells are split into 8 children looked up in parallel too, and the
+// ranges of the cells that pass the bound are packed end to end into batches of 64 candidates (one per lane,
+// coalesced). The list is spread over the lanes, lane i holding the i-th smallest key (lanes past k hold the
+// next keys, which never reach a row). A batch's survivors go in one by one as max(L[i-1], min(key, L[i]))
+// with L[i-1] shifted in from the lane below, or, when many survive, by a bitonic sort of the batch and a
+// bitonic merge with the list. Each wave serves 16 Morton-consecutive queries of a 64-query chunk and keeps
+// the ranges of the last box it looked up for the next query with the same box.
+namespace wv {
+
+template <int CTRL, int ROWM, bool BC>
+__device__ inline int dpp(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, ROWM, 0xf, BC);
+}
+// Inclusive prefix sum over the wave: row_shr 1, 2, 4, 8 inside each row of 16 lanes (lanes shifted in from
+// outside the row read 0), then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then rows
+// 0-1's total into rows 2-3 (row_bcast:31).
+__device__ inline int scan_add(int x) {
+    x += dpp<0x111, 0xf, true>(x);
+    x += dpp<0x112, 0xf, true>(x);
+    x += dpp<0x114, 0xf, true>(x);
+    x += dpp<0x118, 0xf, true>(x);
+    x += dpp<0x142, 0xa, false>(x);
+    x += dpp<0x143, 0xc, false>(x);
+    return x;
+}
+__device__ inline int readlane(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ inline double readlane_d(double x, int l) {
+    const uint64_t u = key_u(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return key_d(((uint64_t)hi << 32) | lo);
+}
+// Lane i receives lane i - 1's value, lane 0 receives +0.0 (below every key). wave_shr:1.
+__device__ inline double shift_up(double x) {
+    const uint64_t u = key_u(x);
+    const uint32_t lo = (uint32_t)dpp<0x138, 0xf, true>((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)dpp<0x138, 0xf, true>((int)(uint32_t)(u >> 32));
+    return key_d(((uint64_t)hi << 32) | lo);
+}
+__device__ inline double shfl_xor_d(double x, int j) {
+    const uint64_t u = key_u(x);
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)u, j, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(u >> 32), j, 64);
+    return key_d(((uint64_t)hi << 32) | lo);
+}
+// Value of lane ^ J: quad_perm for 1 and 2, row_shl / row_shr inside rows of 16 for 4 and 8, else ds_bpermute.
+template <int J>
+__device__ inline int xor_lane(int x) {
+    if constexpr (J == 1) {
+        return dpp<0xB1, 0xf, true>(x);
+    } else if constexpr (J == 2) {
+        return dpp<0x4E, 0xf, true>(x);
+    } else if constexpr (J == 4 || J == 8) {
+        const int a = dpp<0x100 + J, 0xf, true>(x), b = dpp<0x110 + J, 0xf, true>(x);
+        return (__lane_id() & J) ? b : a;
+    } else {
+        return __shfl_xor(x, J, 64);
+    }
+}
+template <int J>
+__device__ inline double xor_lane_d(double x) {
+    const uint64_t u = key_u(x);
+    const uint32_t lo = (uint32_t)xor_lane<J>((int)(uint32_t)u), hi = (uint32_t)xor_lane<J>((int)(uint32_t)(u >> 32));
+    return key_d(((uint64_t)hi << 32) | lo);
+}
+// One compare-exchange stage of a bitonic network: partner lane ^ J; the lane keeps the smaller key when its
+// J bit equals its K2 bit (ascending blocks of K2; K2 = 64 and above sorts the whole wave ascending).
+template <int K2, int J>
+__device__ inline double bstage(double x) {
+    const int lane = __lane_id();
+    const double p = xor_lane_d<J>(x);
+    const bool up = K2 >= 64 || (lane & K2) == 0;
+    const bool low = (lane & J) == 0;
+    return (low == up) ? dmin(x, p) : dmax(x, p);
+}
+// Bitonic sort of the 64 lanes' keys, ascending by lane.
+__device__ inline double sort64(double x) {
+    x = bstage<2, 1>(x);
+    x = bstage<4, 2>(x); x = bstage<4, 1>(x);
+    x = bstage<8, 4>(x); x = bstage<8, 2>(x); x = bstage<8, 1>(x);
+    x = bstage<16, 8>(x); x = bstage<16, 4>(x); x = bstage<16, 2>(x); x = bstage<16, 1>(x);
+    x = bstage<32, 16>(x); x = bstage<32, 8>(x); x = bstage<32, 4>(x); x = bstage<32, 2>(x); x = bstage<32, 1>(x);
+    x = bstage<64, 32>(x); x = bstage<64, 16>(x); x = bstage<64, 8>(x); x = bstage<64, 4>(x); x = bstage<64, 2>(x);
+    x = bstage<64, 1>(x);
+    return x;
+}
+// The 64 smallest of the ascending list L and the ascending batch S, ascending: min(L[i], S[63 - i]) is a
+// bitonic sequence holding them, then a bitonic merge.
+__device__ inline double merge64(double L, double S) {
+    double x = dmin(L, shfl_xor_d(S, 63));
+    x = bstage<64, 32>(x); x = bstage<64, 16>(x); x = bstage<64, 8>(x); x = bstage<64, 4>(x); x = bstage<64, 2>(x);
+    x = bstage<64, 1>(x);
+    return x;
+}
+// Number of lanes whose (non-decreasing) value v is <= g, for g below the last lane's value: branch-free
+// binary search over the lanes.
+__device__ inline int count_le(int v, int g) {
+    int pos = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(v, pos + step - 1, 64) <= g) pos += step;
+    return pos;
+}
+
+}  // namespace wv
+
+// Cell test of k_knn (skip inside the previous box or beyond the bound), also returning the box distance.
+__device__ inline int wave_cell_test(const KnnParams& P, int lv, const int (&c)[3], const int (&plo)[3],
+                                     const int (&phi)[3], const float (&qv)[3], float bnd, float& dd) {
+    bool inside = true, overlap = true;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int lo = c[ax] << lv, hi = ((c[ax] + 1) << lv) - 1;
+        inside = inside && lo >= plo[ax] && hi <= phi[ax];
+        overlap = overlap && lo <= phi[ax] && hi >= plo[ax];
+    }
+    dd = 0.0f;
+    if (inside) return 0;
+    const float hc = P.ba / (float)(P.Bv >> lv);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float lo = P.a + (float)c[ax] * hc;
+        const float gap = fmaxf(fmaxf(lo - qv[ax], qv[ax] - (lo + hc)) - 1e-6f, 0.0f);
+        dd = __builtin_fmaf(gap, gap, dd);
+    }
+    if (dd > bnd) return 0;
+    return overlap ? 2 : 1;
+}
+
+__global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) k_knn_wave(KnnParams P) {
+    const int lane = __lane_id();
+    const int64_t v = blockIdx.x;
+    if (v >= P.nchunks) return;
+    const int64_t start = P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64;
+    const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[start] : 0u;
+    const CloudDev cd = P.clouds[cloud];
+    const int kq = cd.k;
+    const float thres_n = cd.thres_n;
+    const int64_t cend = cd.base + cd.n;
+    uint64_t ncand = 0;
+    uint32_t ncells = 0, nins = 0, ninvalid = 0, nlearned = 0, nfallback = 0, most = 0;
+    // ranges of the 27 cells of the last box looked up (lane t: the t-th cell nearest first)
+    int c_level = -1, c_cq0 = 0, c_cq1 = 0, c_cq2 = 0;
+    uint32_t rs = 0, rn = 0;
+    const int tt = lane < 27 ? (int)c_shell_order[lane] : 13;
+    const int dxo = (tt % 3) - 1, dyo = ((tt / 3) % 3) - 1, dzo = (tt / 9) - 1;
+    // wave index made uniform for the compiler, so the query's values live in scalar registers
+    const int64_t m0 = start + (int64_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 16;
+    const int64_t m1 = min<int64_t>(min<int64_t>(m0 + 16, start + 64), cend);
+    for (int64_t m = m0; m < m1; ++m) {
+        const uint64_t cand0 = ncand;
+        const float4 q = P.mpts[m];
+        const int32_t jq = __float_as_int(q.w);
+        const uint64_t qvc = P.mvcell[m];
+        const int vq[3] = {vcell_axis(qvc, 0), vcell_axis(qvc, 1), vcell_axis(qvc, 2)};
+        const float qv[3] = {q.x, q.y, q.z};
+        int own_level = P.start_level;
+        if (P.qlevel) {
+            own_level = (int)P.qlevel[m] + SUB;
+            ++nlearned;
+        }
+        int level = own_level < 0 ? 0 : (own_level >= P.nlev ? P.nlev - 1 : own_level);
+        const int first_level = level;
+        uint32_t fallback_q = 0;
+        int plo[3] = {P.Bv, P.Bv, P.Bv}, phi[3] = {-1, -1, -1};
+        double L = key_d(KEY_EMPTY);
+        for (;;) {
+            const int Bl = P.Bv >> level;
+            const int cq[3] = {vq[0] >> level, vq[1] >> level, vq[2] >> level};
+            const int c3[3] = {cq[0] + dxo, cq[1] + dyo, cq[2] + dzo};
+            const bool in_grid = lane < 27 && c3[0] >= 0 && c3[1] >= 0 && c3[2] >= 0 && c3[0] < Bl &&
+                                 c3[1] < Bl && c3[2] < Bl;
+            // early exit of k_knn: the reference's next stage does not exist or meets a cell of >= maxc points
+            if (level > first_level && level > SUB) {
+                const int rl = level - SUB;
+                if (rl >= P.stages) break;
+                if ((P.bigmask >> rl) & 1u) {
+                    const bool big = in_grid && tt != 13 && cell_count(P, cloud, rl, c3[0], c3[1], c3[2]) >= (uint32_t)P.maxc;
+                    if (__ballot(big)) break;
+                }
+            }
+            if (level != c_level || cq[0] != c_cq0 || cq[1] != c_cq1 || cq[2] != c_cq2) {
+                c_level = level;
+                c_cq0 = cq[0];
+                c_cq1 = cq[1];
+                c_cq2 = cq[2];
+                rs = 0;
+                rn = 0;
+                if (in_grid) {
+                    ++ncells;
+                    const uint2 r = cell_range(P, cloud, level, c3, ncells);
+                    rs = r.x;
+                    rn = r.y;
+                }
+            }
+            const uint64_t kth0 = key_u(wv::readlane_d(L, kq - 1));
+            const float bnd0 = kth0 != KEY_EMPTY ? __uint_as_float((uint32_t)(kth0 >> 32)) : thres_n;
+            float dd = 0.0f;
+            const int ct = (in_grid && rn > 0) ? wave_cell_test(P, level, c3, plo, phi, qv, bnd0, dd) : 0;
+            const bool split = ct != 0 && level > 0 && (rn > WAVE_SPLIT_COUNT || (ct == 2 && level <= SUB));
+            const int ent = ct ? (split ? 8 : 1) : 0;
+            const int eincl = wv::scan_add(ent);
+            const int total = wv::readlane(eincl, 63);
+            for (int w0 = 0; w0 < total; w0 += 64) {
+                // entry g of the box: a cell, or one of the 8 children of a split cell (nearest child first)
+                const int g = w0 + lane;
+                const bool act = g < total;
+                const int t = min(wv::count_le(eincl, g), 26);
+                const int excl_t = __shfl(eincl - ent, t, 64);
+                const uint32_t ps = (uint32_t)__shfl((int)rs, t, 64), pn = (uint32_t)__shfl((int)rn, t, 64);
+                const int pinfo = __shfl(ct | (split ? 4 : 0), t, 64);
+                const float pdd = __shfl(dd, t, 64);
+                const int ptt = (int)c_shell_order[t];
+                const int pc[3] = {cq[0] + (ptt % 3) - 1, cq[1] + ((ptt / 3) % 3) - 1, cq[2] + (ptt / 9) - 1};
+                uint32_t es = ps, en = pn;
+                int ect = pinfo & 3;
+                float edd = pdd;
+                if (act && (pinfo & 4)) {
+                    const int ch = g - excl_t;
+                    const float hcc = P.ba / (float)(P.Bv >> (level - 1));
+                    int near = 0;
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) near |= (qv[ax] >= P.a + (float)(2 * pc[ax] + 1) * hcc ? 1 : 0) << ax;
+                    const int chn = near ^ ((0x76534210u >> (4 * ch)) & 7);
+                    const int cc3[3] = {2 * pc[0] + (chn & 1), 2 * pc[1] + ((chn >> 1) & 1), 2 * pc[2] + (chn >> 2)};
+                    ect = wave_cell_test(P, level - 1, cc3, plo, phi, qv, bnd0, edd);
+                    en = 0;
+                    if (ect) {
+                        ++ncells;
+                        const uint2 rc = level - 1 >= SUB
+                                             ? level_lookup(P.htab, P.lev[level - 1 - SUB],
+                                                            cell_key(cloud, morton_code(cc3[0], cc3[1], cc3[2])))
+                                             : sub_range(P, ps, pn, level - 1, cc3, ncells);
+                        es = rc.x;
+                        en = rc.y;
+                    }
+                }
+                if (!act || !ect) en = 0;
+                const bool eov = ect == 2;
+                // the entries' ranges, end to end, 64 candidates per batch; an entry beyond the bound is dropped
+                uint32_t rem = en, pos = es;
+                for (;;) {
+                    const uint64_t kthb = key_u(wv::readlane_d(L, kq - 1));
+                    const float bnd = kthb != KEY_EMPTY ? __uint_as_float((uint32_t)(kthb >> 32)) : thres_n;
+                    if (edd > bnd) rem = 0;
+                    const int rincl = wv::scan_add((int)rem);
+                    const int tot = wv::readlane(rincl, 63);
+                    if (tot == 0) break;
+                    const int rexcl = rincl - (int)rem;
+                    // lane l takes slot l of the ranges laid end to end: the entries of this batch, one at a time
+                    int64_t p = -1;
+                    int eo = 0;
+                    for (uint64_t em = __ballot(rem > 0 && rexcl < 64); em; em &= em - 1) {
+                        const int e = __builtin_ctzll(em);
+                        const int ex = wv::readlane(rexcl, e), ei = wv::readlane(rincl, e);
+                        const uint32_t ep = (uint32_t)wv::readlane((int)pos, e);
+                        const int eov_e = wv::readlane((int)eov, e);
+                        if (lane >= ex && lane < ei) {
+                            p = (int64_t)ep + (lane - ex);
+                            eo = eov_e;
+                        }
+                    }
+                    const int take = min((int)rem, max(0, 64 - rexcl));
+                    if (!__ballot(take > 0)) break;  // every batch consumes; guards the loop's end
+                    pos += (uint32_t)take;
+                    rem -= (uint32_t)take;
+                    uint64_t kk = KEY_EMPTY;
+                    bool tested = false;
+                    if (p >= cd.base && p < cend) {
+                        const float4 e4 = P.mpts[p];
+                        const int32_t j = __float_as_int(e4.w);
+                        bool ok = j != jq;
+                        if (ok && eo) {
+                            const uint32_t cc = P.mcell[p];
+                            const int ex0 = (int)(cc & 1023u) << SUB, ey0 = (int)((cc >> 10) & 1023u) << SUB,
+                                      ez0 = (int)(cc >> 20) << SUB;
+                            if (ex0 >= plo[0] && ex0 <= phi[0] && ey0 >= plo[1] && ey0 <= phi[1] && ez0 >= plo[2] &&
+                                ez0 <= phi[2])
+                                ok = false;
+                        }
+                        if (ok) {
+                            tested = true;
+                            const float d = sqd(q.x, q.y, q.z, e4.x, e4.y, e4.z);
+                            if (d <= thres_n) kk = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)j;
+                        }
+                    }
+                    ncand += (uint64_t)__builtin_popcountll(__ballot(tested));
+                    const bool sv = kk < kthb;
+                    uint64_t smask = __ballot(sv);
+                    const int ns = __builtin_popcountll(smask);
+                    nins += (uint32_t)ns;
+                    if (ns > 16) {
+                        L = wv::merge64(L, wv::sort64(sv ? key_d(kk) : key_d(KEY_EMPTY)));
+                    } else {
+                        const double kd = key_d(kk);
+                        while (smask) {
+                            const int b = __builtin_ctzll(smask);
+                            smask &= smask - 1;
+                            const double c = wv::readlane_d(kd, b);
+                            L = dmax(wv::shift_up(L), dmin(c, L));
+                        }
+                    }
+                }
+            }
+            if (level >= P.nlev - 1) break;
+            const uint64_t lk = key_u(wv::readlane_d(L, kq - 1));
+            const bool full = lk != KEY_EMPTY;
+            const float bound = full ? __uint_as_float((uint32_t)(lk >> 32)) : thres_n;
+            const double R = __builtin_sqrt((double)bound) + 2e-6;
+            const double hl = (double)P.ba / (double)Bl;
+            const double qd[3] = {q.x, q.y, q.z};
+            bool exact = true;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                if (cq[ax] - 1 >= 1 && !(R < qd[ax] - ((double)P.a + (double)(cq[ax] - 1) * hl))) exact = false;
+                if (cq[ax] + 1 <= Bl - 2 && !(R < ((double)P.a + (double)(cq[ax] + 2) * hl) - qd[ax])) exact = false;
+            }
+            if (exact) break;
+            if (P.qlevel && level >= own_level) fallback_q = 1;
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                plo[ax] = max(0, (cq[ax] - 1) << level);
+                phi[ax] = min(P.Bv - 1, ((cq[ax] + 2) << level) - 1);
+            }
+            ++level;
+        }
+        most = max(most, (uint32_t)min<uint64_t>(ncand - cand0, 0xFFFFFFFFull));
+        // replay of the reference schedule on the exact list (lanes below k hold it)
+        const uint64_t mykey = key_u(L);
+        const bool listed = lane < kq && mykey != KEY_EMPTY;
+        const int c = __builtin_popcountll(__ballot(listed));
+        int mx = 0, my = 0, mz = 0;
+        if (listed) {
+            const uint32_t bm = (uint32_t)P.B - 1u;
+            const uint32_t kk = (uint32_t)P.key[(uint32_t)mykey];
+            mx = kk & bm;
+            my = (kk >> P.log2B) & bm;
+            mz = kk >> (2 * P.log2B);
+        }
+        const float kthd = c > 0 ? __uint_as_float((uint32_t)(key_u(wv::readlane_d(L, kq - 1)) >> 32)) : 0.0f;
+        bool valid = false;
+        for (int s = 0; s <= P.stages; ++s) {
+            const int lvl = s == 0 ? 0 : s - 1;
+            const int off = s == 0 ? 0 : 1;
+            const int cx = vq[0] >> (SUB + lvl), cy = vq[1] >> (SUB + lvl), cz = vq[2] >> (SUB + lvl);
+            if ((P.bigmask >> lvl) & 1u) {
+                bool big = false;
+                if (s == 0) {
+                    big = lane == 0 && cell_count(P, cloud, 0, cx, cy, cz) >= (uint32_t)P.maxc;
+                } else {
+                    big = lane < 27 && tt != 13 && cell_count(P, cloud, lvl, cx + dxo, cy + dyo, cz + dzo) >= (uint32_t)P.maxc;
+                }
+                if (__ballot(big)) break;
+            }
+            const bool out = listed && !((mx >> lvl) >= cx - off && (mx >> lvl) <= cx + off && (my >> lvl) >= cy - off &&
+                                         (my >> lvl) <= cy + off && (mz >> lvl) >= cz - off && (mz >> lvl) <= cz + off);
+            const bool inside = __ballot(out) == 0;
+            const int cst[3] = {cx, cy, cz};
+            if (inside && !ref_continue<false>(P, kq, thres_n, s, q.x, q.y, q.z, cst, c, kthd)) {
+                valid = true;
+                break;
+            }
+        }
+        int32_t* row = P.knn + cd.row_off + (int64_t)(jq - cd.base) * kq;
+        if (lane < kq) row[lane] = (valid && listed) ? (int32_t)(uint32_t)mykey - (int32_t)cd.base : -1;
+        if (lane == 0) P.kth[jq] = (valid && c > 0) ? kthd : -1.0f;
+        ninvalid += valid ? 0u : 1u;
+        nfallback += fallback_q;
+    }
+    uint32_t c4 = ncells;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c4 += __shfl_xor(c4, d, 64);
+    if (lane == 0) {
+        if (ncand) atomicAdd(&P.counters[0], (unsigned long long)ncand);
+        if (ninvalid) atomicAdd(&P.counters[1], (unsigned long long)ninvalid);
+        if (nlearned) atomicAdd(&P.counters[2], (unsigned long long)nlearned);
+        if (nfallback) atomicAdd(&P.counters[3], (unsigned long long)nfallback);
+        if (c4) atomicAdd(&P.counters[4], (unsigned long long)c4);
+        if (nins) atomicAdd(&P.counters[7], (unsigned long long)nins);
+        if (most) atomicMax(&P.counters[8], (unsigned long long)most);
+    }
+}
+
 // Query preparation: each query normalized with the reference cloud's extent (normalize_points
 // GridKNNCuda.cu:841-849: x' = (x - min) / (max - min)), its fine cell clamped to the grid and its virtual
 // cell inside that (the search boxes start there), and the Morton code of the fine cell as the sort key, so
@@ -630,6 +1018,18 @@ void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false, bo
         case 56: launch_knn<56>(P, s, query, list); break;
         default: launch_knn<64>(P, s, query, list); break;
     }
+}
+
+// Self search of a work list in the handle's launch shape: one wave per query (lanes = 64, not with the
+// radius bound) or one lane per query.
+void launch_self(const egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
+    if (h->knn_lanes == 64 && !P.qbound) {
+        if (P.nchunks < 1) return;
+        hipLaunchKernelGGL(k_knn_wave, dim3((unsigned)P.nchunks), dim3(KBLK), 0, s, P);
+        LAUNCH_CHECK();
+        return;
+    }
+    launch_any(P, K, s);
 }
 
 // Search parameters of the handle; with scratch, the per-point k-th distance, the counters and the bounded
@@ -743,7 +1143,7 @@ void run_knn_clouds(egt_ctx* h) {
             HIP_CHECK(hipMemsetAsync(queues, 0, sizeof(uint64_t), s));
             P.queue = queues;
         }
-        launch_any(P, cr.K, s);
+        launch_self(h, P, cr.K, s);
         if (P.qbound) {
             // queries with fewer than k points within their proposed radius: searched again without the
             // bound (list launch, each lane its own query), from the handle's start level
@@ -973,6 +1373,60 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
 // next to dense clusters), not by the bulk of the work. Rows do not depend on the choice. Levels -SUB .. top
 // are tried; trial_ms / trial_cost get each level's sample time and score at index level + SUB (-1 for
 // untried levels).
+// Launch-shape tuner: times the handle's current search (start level, learned proposal) on the same evenly
+// spread sample of 64-query chunks in each launch shape (one lane per query with one workgroup per 256
+// queries, the same as one persistent work-queue launch, one wave per query; the wave shape is not offered
+// with the radius bound), each shape twice after one warm-up launch, and keeps the fastest (best of its two
+// runs). Rows do not depend on the shape.
+int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
+    hipStream_t s = h->stream;
+    h->clouds[0].k = k;
+    h->clouds[0].thres_n = thres_n;
+    prepare_clouds(h);
+    KnnParams P = knn_params(h);
+    const int64_t total = (h->n + 63) / 64;
+    P.stride = (int)std::max<int64_t>(1, total / 8192);
+    P.nchunks = (total + P.stride - 1) / P.stride;
+    const int K = kclass(k);
+    unsigned long long* queue = P.counters + 5;
+    Event e0, e1;
+    auto launch = [&](int shape) {
+        KnnParams Q = P;
+        if (shape == 1) {
+            HIP_CHECK(hipMemsetAsync(queue, 0, sizeof(uint64_t), s));
+            Q.queue = queue;
+        }
+        h->knn_lanes = shape == 2 ? 64 : 1;
+        launch_self(h, Q, K, s);
+    };
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int shape = 0; shape < 3; ++shape) {
+        shape_ms[shape] = -1.0f;
+        if (shape == 2 && P.qbound) continue;
+        launch(shape);
+        float t = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_CHECK(hipEventRecord(e0, s));
+            launch(shape);
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            t = rep == 0 ? ms : std::min(t, ms);
+        }
+        shape_ms[shape] = t;
+        if (shape == 0 || t < best_ms) {
+            best = shape;
+            best_ms = t;
+        }
+    }
+    h->knn_persistent = best == 1 ? 1 : 0;
+    h->knn_lanes = best == 2 ? 64 : 1;
+    h->have_knn = false;  // the trial rows cover only the sample
+    return best;
+}
+
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost) {
     hipStream_t s = h->stream;
     h->clouds[0].k = k;

@@ -27,6 +27,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "UNSUPPO
 OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PERSISTENT, OPT_TAUBIN_TILE = 0, 1, 2, 3, 4, 5
 OPT_TAUBIN_REGULARIZED = 6
 OPT_LEARNED_MODE = 7         # learned proposal: 1 = start level (default), 2 = radius bound, 3 = both
+OPT_KNN_LANES = 8            # kNN self search: 1 = one lane per query (default), 64 = one wave per query
 START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
 SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
 AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
@@ -117,6 +118,7 @@ def _load():
         "egt_destroy": ([_H], st),
         "egt_get_stream": ([_H], ctypes.c_void_p),
         "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N, _F32N], st),
+        "egt_autotune_shape": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N], st),
         "egt_batch_create": ([_F32P, _F32P, _F32P, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
                               ctypes.c_int32, f32, i32, _PH], st),
         "egt_batch_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -292,6 +294,15 @@ class GridStructure(_Provider):
         cost = np.empty(AUTOTUNE_SLOTS, np.float32)
         _check(_lib.egt_autotune(self.handle, k, thres, ctypes.byref(lvl), ms, cost))
         return lvl.value, ms, cost
+
+    def autotune_shape(self, k, thres=1e8):
+        """Times the current search (start level, learned proposal) in each kNN launch shape on a sample and
+        keeps the fastest (no reference counterpart). Returns (shape, sample ms per shape): 0 = one lane per
+        query, 1 = the same as one persistent launch, 2 = one wave per query; -1 where not offered."""
+        shape = ctypes.c_int32()
+        ms = np.empty(3, np.float32)
+        _check(_lib.egt_autotune_shape(self.handle, k, thres, ctypes.byref(shape), ms))
+        return shape.value, ms
 
     def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
         _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))

@@ -24,15 +24,18 @@ def _clouds():
     return clouds, np.array(ks, np.int32)
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
-def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent):
-    """persistent: the kNN as one wave per 64-query chunk (default) or as a work-queue launch."""
+@pytest.mark.parametrize("persistent,lanes", [(0, 1), (1, 1), (0, 64)])
+def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent, lanes):
+    """The kNN as one wave per 64-query chunk with a lane per query (default), as a work-queue launch, or
+    with a wave per query."""
     E, O = egt_mod, oracle_mod
     clouds, ks = _clouds()
     lam, mu, alpha, its = 0.5, -0.53, 0.65, 12
-    _log("batch of", len(clouds), "clouds, sizes", [c[0].size for c in clouds], "persistent", persistent)
+    _log("batch of", len(clouds), "clouds, sizes", [c[0].size for c in clouds], "persistent", persistent,
+         "lanes", lanes)
     b = E.BatchGrid(clouds)
     b.set_option(E.OPT_KNN_PERSISTENT, persistent)
+    b.set_option(E.OPT_KNN_LANES, lanes)
     b.knn(ks)
     refs = []
     for c, pts in enumerate(clouds):
@@ -120,8 +123,16 @@ def test_learned_proposal_on_batch(egt_mod, oracle_mod):
     _log("learned batch: queries", st["learned_queries"], "fallbacks", st["learned_fallbacks"],
          "cells/query", st["knn_cells"] / n, "plain cells/query", plain.stats()["knn_cells"] / n)
     assert st["learned_queries"] == n
+    # the same proposal with a wave per query: each query starts at its own proposed level
+    wave = E.BatchGrid(evalc)
+    wave.set_option(E.OPT_KNN_LANES, 64)
+    model.propose(wave, keval)
+    wave.knn(keval)
+    _log("learned batch, wave per query: fallbacks", wave.stats()["learned_fallbacks"])
     for c in range(len(evalc)):
         np.testing.assert_array_equal(learned.GetKNNIndexes(c), plain.GetKNNIndexes(c), err_msg=f"cloud {c}")
+        np.testing.assert_array_equal(wave.GetKNNIndexes(c), plain.GetKNNIndexes(c), err_msg=f"wave {c}")
+    wave.close()
     og = O.Grid(*evalc[5])
     np.testing.assert_array_equal(learned.GetKNNIndexes(5), og.to_original(og.knn(int(keval[5]))[0]))
     plain.close()

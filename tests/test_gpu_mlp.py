@@ -106,8 +106,10 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
     rows, stats = {}, {}
     # learned_level: start at the proposed level; learned_bound: the autotuned level, candidates bounded by
     # the proposed radius; learned: both; learned_tight: both with the bound shrunk 8x, so most queries hold
-    # fewer than k points inside it and are searched again without it
-    modes = ("own_cell", "occupancy", "autotuned", "learned_level", "learned_bound", "learned", "learned_tight")
+    # fewer than k points inside it and are searched again without it; learned_wave: the proposed level with
+    # a wave per query
+    modes = ("own_cell", "occupancy", "autotuned", "learned_level", "learned_bound", "learned", "learned_tight",
+             "learned_wave")
     bsafe = model.bound_safety
     for mode in modes:
         g = E.GridStructure(evalc, 1, 1e-6, 1024, 4)
@@ -118,7 +120,9 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
         if mode.startswith("learned"):
             model.bound_safety = bsafe - 3.0 if mode == "learned_tight" else bsafe
             model.propose(g, k)
-            g.set_option(E.OPT_LEARNED_MODE, {"learned_level": 1, "learned_bound": 2}.get(mode, 3))
+            g.set_option(E.OPT_LEARNED_MODE, {"learned_level": 1, "learned_bound": 2, "learned_wave": 1}.get(mode, 3))
+            if mode == "learned_wave":  # a wave per query: each query starts at its own proposed level
+                g.set_option(E.OPT_KNN_LANES, 64)
             if mode == "learned":
                 lv = E.model_levels(g)
                 _log("  proposed levels histogram (from level -3)",

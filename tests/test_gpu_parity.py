@@ -16,8 +16,10 @@ def _log(*a):
     print(*a, flush=True)
 
 
-def _cloud(E, pts, k, bucket=1024, maxc=None, start_level=None, learned=None):
+def _cloud(E, pts, k, bucket=1024, maxc=None, start_level=None, learned=None, lanes=None):
     g = E.GridStructure(pts, 1, 1e-6, bucket, 4)
+    if lanes is not None:
+        g.set_option(E.OPT_KNN_LANES, lanes)  # 1 = one lane per query, 64 = one wave per query
     if maxc is not None:
         g.set_option(E.OPT_MAX_PTS_IN_CUBE, maxc)
     if start_level is not None:
@@ -26,8 +28,9 @@ def _cloud(E, pts, k, bucket=1024, maxc=None, start_level=None, learned=None):
     return g
 
 
+@pytest.mark.parametrize("lanes", [1, 64])
 @pytest.mark.parametrize("case", ["sphere20k_k40", "torus30k_k16", "sphere5k_k64", "lattice_k18", "dups_k8"])
-def test_knn_rows_bit_exact(egt_mod, oracle_mod, case):
+def test_knn_rows_bit_exact(egt_mod, oracle_mod, case, lanes):
     E, O = egt_mod, oracle_mod
     if case == "sphere20k_k40":
         pts, k = D.sphere(20_000, seed=11), 40
@@ -40,8 +43,8 @@ def test_knn_rows_bit_exact(egt_mod, oracle_mod, case):
     else:
         x, y, z = D.sphere(2_000, seed=14)
         pts, k = (np.concatenate([x, x[:300]]), np.concatenate([y, y[:300]]), np.concatenate([z, z[:300]])), 8
-    _log("case", case, "n", pts[0].size, "k", k)
-    g = _cloud(E, pts, k)
+    _log("case", case, "n", pts[0].size, "k", k, "lanes", lanes)
+    g = _cloud(E, pts, k, lanes=lanes)
     got = g.GetKNNIndexes()
     og = O.Grid(*pts)
     knn, kth, inv, st = og.knn(k)
@@ -65,17 +68,41 @@ def test_autotuned_start_level_keeps_rows(egt_mod, oracle_mod):
     g.close()
 
 
+def test_shape_tuner_keeps_rows(egt_mod, oracle_mod):
+    """The launch-shape tuner times the three shapes on its sample and keeps the fastest; rows equal the
+    oracle's after it and in each shape it can choose."""
+    E, O = egt_mod, oracle_mod
+    pts = D.mixture(60_000, seed=24)
+    og = O.Grid(*pts)
+    want = og.to_original(og.knn(40)[0])
+    g = E.GridStructure(pts, 1, 1e-6, 1024, 4)
+    shape, ms = g.autotune_shape(40)
+    _log("shape tuner choice", shape, "sample ms (lane, persistent, wave)", np.round(ms, 4).tolist())
+    assert shape in (0, 1, 2) and (ms > 0).all() and ms[shape] == ms.min()
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), want)
+    for lanes, persistent in ((1, 0), (1, 1), (64, 0)):
+        g.set_option(E.OPT_KNN_LANES, lanes)
+        g.set_option(E.OPT_KNN_PERSISTENT, persistent)
+        g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+        np.testing.assert_array_equal(g.GetKNNIndexes(), want, err_msg=f"lanes {lanes} persistent {persistent}")
+    g.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 64])
 @pytest.mark.parametrize("start_level", ["occupancy", -3, -2, -1, 0, 2, 5, 9])
-def test_knn_independent_of_start_level(egt_mod, oracle_mod, start_level):
+def test_knn_independent_of_start_level(egt_mod, oracle_mod, start_level, lanes):
     E, O = egt_mod, oracle_mod
     pts = D.sphere(8_000, seed=21)
-    g = _cloud(E, pts, 24, start_level=E.START_LEVEL_OCCUPANCY if start_level == "occupancy" else start_level)
+    g = _cloud(E, pts, 24, start_level=E.START_LEVEL_OCCUPANCY if start_level == "occupancy" else start_level,
+               lanes=lanes)
     og = O.Grid(*pts)
     knn, _, _, _ = og.knn(24)
     np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(knn))
 
 
-def test_knn_invalidation_rules(egt_mod, oracle_mod):
+@pytest.mark.parametrize("lanes", [1, 64])
+def test_knn_invalidation_rules(egt_mod, oracle_mod, lanes):
     """Rows the reference blanks: N <= k (never enough neighbours), a dense cell over the population limit
     (limit lowered to 40 so a 3k cloud exercises it), and isolated outliers."""
     E, O = egt_mod, oracle_mod
@@ -87,13 +114,13 @@ def test_knn_invalidation_rules(egt_mod, oracle_mod):
     pts = (x, y, z)
     og = O.Grid(*pts)
     for k, maxc in [(16, 100000), (16, 40), (40, 100000)]:
-        g = _cloud(E, pts, k, maxc=maxc)
+        g = _cloud(E, pts, k, maxc=maxc, lanes=lanes)
         knn, _, inv, _ = og.knn(k, maxc=maxc)
         _log("k", k, "maxc", maxc, "invalid", int(inv.sum()))
         assert inv.sum() > 0
         np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(knn))
     tiny = D.sphere(10, seed=32)
-    g = _cloud(E, tiny, 16)
+    g = _cloud(E, tiny, 16, lanes=lanes)
     assert (g.GetKNNIndexes() == -1).all()
 
 
@@ -168,15 +195,16 @@ def test_taubin_crowded_cells_bit_exact(egt_mod, oracle_mod, tile):
         assert st["taubin_fallback_tiles"] > 0
 
 
+@pytest.mark.parametrize("lanes", [1, 64])
 @pytest.mark.parametrize("start_level", [0, -1, -3])
 @pytest.mark.parametrize("maxc", [None, 3750])
-def test_knn_crowded_cells_sub_levels(egt_mod, oracle_mod, start_level, maxc):
+def test_knn_crowded_cells_sub_levels(egt_mod, oracle_mod, start_level, maxc, lanes):
     """Cells of about 3,800 points: the search splits them into sub-cells (binary-searched ranges inside the
     cell) or starts from boxes of 1/2 or 1/8 cells; with the population limit lowered to 3750 (between the
     sizes of those cells) the reference's rule blanks the rows that meet a cell over the limit."""
     E, O = egt_mod, oracle_mod
     pts = _crowded_blob()
-    g = _cloud(E, pts, 40, bucket=64, maxc=maxc, start_level=start_level)
+    g = _cloud(E, pts, 40, bucket=64, maxc=maxc, start_level=start_level, lanes=lanes)
     got = g.GetKNNIndexes()
     st = g.stats()
     g.close()
@@ -184,7 +212,7 @@ def test_knn_crowded_cells_sub_levels(egt_mod, oracle_mod, start_level, maxc):
     knn, _, inv, _ = og.knn(40, maxc=maxc or 100000)
     np.testing.assert_array_equal(got, og.to_original(knn))
     assert st["invalid_rows"] == int(inv.sum())
-    _log("crowded knn start", start_level, "maxc", maxc, "invalid", int(inv.sum()), "candidates/query",
+    _log("crowded knn lanes", lanes, "start", start_level, "maxc", maxc, "invalid", int(inv.sum()), "candidates/query",
          st["knn_candidates"] / pts[0].size, "cells/query", st["knn_cells"] / pts[0].size)
     if maxc is not None:
         assert 0 < inv.sum() < pts[0].size
