@@ -635,7 +635,7 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
                      "insertions_per_query": st["knn_insertions"] / n, "invalid_rows": st["invalid_rows"],
                      "knn_shape": ("lane per query", "persistent, lane per query", "wave per query")[shapes[mode][0]]}
         if shapes[mode][1] is not None:
-            per[mode]["shape_sample_ms"] = [round(float(x), 4) for x in shapes[mode][1]]
+            per[mode]["shape_tuner_ms"] = [round(float(x), 4) for x in shapes[mode][1]]
         if mode == "learned":
             per[mode]["fallback_rate"] = st["learned_fallbacks"] / n
             per[mode]["bound_retry_rate"] = st["learned_bound_retries"] / n

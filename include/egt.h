@@ -78,6 +78,7 @@ typedef struct {
                                       searched again without the bound */
     int64_t knn_insertions;    /* last kNN: keys inserted into the neighbour lists */
     int64_t knn_max_candidates; /* last kNN: candidate distances of its costliest query */
+    int64_t knn_long_queries;   /* last kNN: queries handed on to the wave-per-query pass (EGT_OPT_KNN_LONG_QUERY) */
 } egt_stats;
 
 /* EGT_OPT_START_LEVEL value: choose the level from the mean cell occupancy. */
@@ -106,9 +107,12 @@ typedef enum {
                                     bound the search by the proposed radius (queries with fewer than k points
                                     inside it are searched again without the bound); default 1 (the level only:
                                     the bound measured slower). Rows do not depend on it */
-    EGT_OPT_KNN_LANES = 8        /* kNN self search: lanes per query, 1 = one lane per query with the list in its
+    EGT_OPT_KNN_LANES = 8,       /* kNN self search: lanes per query, 1 = one lane per query with the list in its
                                     registers (default), 64 = one wave per query with the list spread over the
                                     lanes. Rows do not depend on it */
+    EGT_OPT_KNN_LONG_QUERY = 9   /* kNN self search with one lane per query: a query that has tested more
+                                    candidates than this is handed on to a wave-per-query pass (its wave no
+                                    longer waits for it); 0 = never; default 2048. Rows do not depend on it */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -230,13 +234,13 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
                         float* trial_cost);
 
 /* Launch-shape tuner (no reference counterpart): times the handle's current search (its start level and, with
- * EGT_OPT_LEARNED, the proposal in place) with neighbour count k and radius thres on the autotuner's sample in
- * each launch shape: 0 = one lane per query, one workgroup per 256 queries; 1 = one lane per query, one
- * persistent work-queue launch; 2 = one wave per query (not offered with the radius bound). Each shape runs
- * once to warm up and twice timed; the fastest (best of its two) is kept through EGT_OPT_KNN_PERSISTENT and
- * EGT_OPT_KNN_LANES. Rows do not depend on the choice. *shape (optional) receives it, shape_ms (optional, 3
- * floats) each shape's sample time in ms, -1 where not offered. The handle's neighbour rows are not valid
- * afterwards: run egt_grid_knn_self. */
+ * EGT_OPT_LEARNED, the proposal in place) with neighbour count k and radius thres over all queries in each
+ * launch shape: 0 = one lane per query, one workgroup per 256 queries; 1 = one lane per query, one
+ * persistent work-queue launch; 2 = one wave per query (not offered with the radius bound). Each shape is
+ * warmed up on the autotuner's sample, then timed once on the full search; the fastest is kept through
+ * EGT_OPT_KNN_PERSISTENT and EGT_OPT_KNN_LANES. Rows do not depend on the choice. *shape (optional) receives
+ * it, shape_ms (optional, 3 floats) each shape's time in ms, -1 where not offered. The handle's neighbour rows
+ * are not valid afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms);
 
 /* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA

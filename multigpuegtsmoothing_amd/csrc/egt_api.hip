@@ -361,6 +361,10 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
             if (value != 1 && value != 64) return fail(EGT_ERR_INVALID_ARG, "kNN lanes per query must be 1 or 64");
             h->knn_lanes = (int)value;
             return EGT_OK;
+        case EGT_OPT_KNN_LONG_QUERY:
+            if (value < 0 || value > UINT32_MAX) return fail(EGT_ERR_INVALID_ARG, "long-query cap must be in [0, 2^32)");
+            h->knn_long_cap = value;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }

@@ -230,6 +230,8 @@ struct egt_ctx {
                            // radius, 3 = both
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int knn_lanes = 1;       // kNN self search: 1 = one lane per query, 64 = one wave per query
+    int64_t knn_long_cap = 2048;  // lane-per-query search: candidates after which a query goes to a wave
+                                  // (0 = never)
     int taubin_graph = 1;
     int taubin_tile = 1024;  // Taubin tile width: 1024 or 2048 points; 0 = tune on the next smoothing call
     int taubin_regularized = 0;  // isRegularized weights (1 / inverse-square), off as in the reference

@@ -69,9 +69,9 @@ struct KnnParams {
     int32_t* __restrict__ knn;             // rows, cloud after cloud, local sorted indices
     float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
     unsigned long long* __restrict__ counters;  // [0] candidates, [1] invalid rows, [2] learned, [3] fallbacks,
-                                                // [4] cells looked up, [5] work queue, [6] bound retries,
-                                                // [7] keys inserted into the lists, [8] most candidates of a
-                                                // query
+                                                // [4] cells looked up, [5] work queue, [6] queries handed
+                                                // back (retry list length), [7] keys inserted into the lists,
+                                                // [8] most candidates of a query, [9] long queries handed on
     const LevelDev* __restrict__ lev;      // per-level table descriptors (device array)
     const float4* __restrict__ qpts;       // query search: queries in Morton order (x', y', z', bits(index))
     const unsigned long long* __restrict__ qvcell;  // query search: virtual cell, clamped to the grid
@@ -81,6 +81,8 @@ struct KnnParams {
     int Bv;            // cells per axis at internal level 0 (B << SUB)
     int nlev;          // internal levels: SUB sub-cell levels + the reference pyramid
     int start_level;   // internal level of the first box (SUB = the query's own fine cell)
+    uint32_t long_cap; // lane-per-query search: a query that has tested more candidates than this is handed
+                       // back to a wave-per-query pass (0 = never)
     uint32_t bigmask;  // levels holding a cell with >= maxc points (any cloud)
     float a, ba;
 };
@@ -216,7 +218,7 @@ __global__ void __launch_bounds__(KBLK)
     __attribute__((amdgpu_waves_per_eu(K <= 8 || (K <= 16 && !PERSIST) ? 4 : (K <= 40 && !PERSIST ? 3 : 1))))
     k_knn(KnnParams P) {
     uint64_t cand = 0;
-    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0, inserted = 0, most = 0;
+    uint32_t invalid_row = 0, learned = 0, fallback = 0, cells = 0, inserted = 0, most = 0, nlong = 0;
     // Each wave takes chunks of 64 Morton-consecutive queries of one cloud, either one fixed chunk or, in a
     // persistent launch, chunk after chunk from a work queue until the list is exhausted.
     for (int64_t v = next_chunk<PERSIST>(P, -1); v < P.nchunks; v = next_chunk<PERSIST>(P, v)) {
@@ -285,6 +287,7 @@ __global__ void __launch_bounds__(KBLK)
         level = level < 0 ? 0 : (level >= P.nlev ? P.nlev - 1 : level);
         const int first_level = level;
         uint32_t fallback_q = 0;
+        bool gave_up = false;  // long query, handed on to the wave-per-query pass
         int plo[3] = {P.Bv, P.Bv, P.Bv}, phi[3] = {-1, -1, -1};  // previous box (virtual coords), empty
         for (;;) {
             const int Bl = P.Bv >> level;
@@ -420,7 +423,12 @@ __global__ void __launch_bounds__(KBLK)
                     }
                     scan(start, start + count, ov);
                 }
+                if (!QUERY && !LIST && P.long_cap && cand - cand0 > P.long_cap) {
+                    gave_up = true;
+                    break;
+                }
             }
+            if (gave_up) break;
             flush();
             if (level >= P.nlev - 1) break;
             // exactness: every unscanned point is farther than the current bound
@@ -447,8 +455,9 @@ __global__ void __launch_bounds__(KBLK)
         }
 
         most = max(most, (uint32_t)min<uint64_t>(cand - cand0, 0xFFFFFFFFull));
-        const bool handed_back = P.qbound && (uint32_t)key_u(L[K - 1]) == 0xFFFFFFFFu;
+        const bool handed_back = gave_up || (P.qbound && (uint32_t)key_u(L[K - 1]) == 0xFFFFFFFFu);
         if (handed_back) P.retry[atomicAdd(&P.counters[6], 1ull)] = (uint32_t)m;
+        nlong += gave_up ? 1u : 0u;
         if (!handed_back) {
         // replay of the reference schedule on the exact list
         int c = 0;
@@ -515,10 +524,11 @@ __global__ void __launch_bounds__(KBLK)
     }
     // one atomic per wave
     unsigned long long c0 = cand;
-    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells, c7 = inserted, c8 = most;
+    uint32_t c1 = invalid_row, c2 = learned, c3 = fallback, c4 = cells, c7 = inserted, c8 = most, c9 = nlong;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         c0 += __shfl_xor(c0, d, 64);
+        c9 += __shfl_xor(c9, d, 64);
         c4 += __shfl_xor(c4, d, 64);
         c7 += __shfl_xor(c7, d, 64);
         c8 = max(c8, (uint32_t)__shfl_xor(c8, d, 64));
@@ -534,6 +544,7 @@ __global__ void __launch_bounds__(KBLK)
         if (c4) atomicAdd(&P.counters[4], (unsigned long long)c4);
         if (c7) atomicAdd(&P.counters[7], (unsigned long long)c7);
         if (c8) atomicMax(&P.counters[8], (unsigned long long)c8);
+        if (c9) atomicAdd(&P.counters[9], (unsigned long long)c9);
     }
 }
 
@@ -669,27 +680,40 @@ __device__ inline int wave_cell_test(const KnnParams& P, int lv, const int (&c)[
     return overlap ? 2 : 1;
 }
 
+// LIST: each wave takes one query of a list (long queries a lane-per-query search handed on), from the
+// handle's start level.
+template <bool LIST>
 __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) k_knn_wave(KnnParams P) {
     const int lane = __lane_id();
     const int64_t v = blockIdx.x;
     if (v >= P.nchunks) return;
-    const int64_t start = P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64;
-    const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[start] : 0u;
-    const CloudDev cd = P.clouds[cloud];
-    const int kq = cd.k;
-    const float thres_n = cd.thres_n;
-    const int64_t cend = cd.base + cd.n;
+    // wave index made uniform for the compiler, so the query's values live in scalar registers
+    const int wv_id = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int64_t m0, m1;
+    if (LIST) {
+        m0 = v * (KBLK / 64) + wv_id;
+        m1 = min<int64_t>(m0 + 1, P.nlist);
+    } else {
+        const int64_t start = P.chunks ? P.chunks[v * P.stride] : v * P.stride * 64;
+        const CloudDev& c0 = P.clouds[P.mcloud ? (uint32_t)P.mcloud[start] : 0u];
+        m0 = start + (int64_t)wv_id * 16;
+        m1 = min<int64_t>(min<int64_t>(m0 + 16, start + 64), c0.base + c0.n);
+    }
     uint64_t ncand = 0;
     uint32_t ncells = 0, nins = 0, ninvalid = 0, nlearned = 0, nfallback = 0, most = 0;
     // ranges of the 27 cells of the last box looked up (lane t: the t-th cell nearest first)
     int c_level = -1, c_cq0 = 0, c_cq1 = 0, c_cq2 = 0;
+    uint32_t c_cloud = 0;
     uint32_t rs = 0, rn = 0;
     const int tt = lane < 27 ? (int)c_shell_order[lane] : 13;
     const int dxo = (tt % 3) - 1, dyo = ((tt / 3) % 3) - 1, dzo = (tt / 9) - 1;
-    // wave index made uniform for the compiler, so the query's values live in scalar registers
-    const int64_t m0 = start + (int64_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * 16;
-    const int64_t m1 = min<int64_t>(min<int64_t>(m0 + 16, start + 64), cend);
-    for (int64_t m = m0; m < m1; ++m) {
+    for (int64_t mi = m0; mi < m1; ++mi) {
+        const int64_t m = LIST ? (int64_t)P.qlist[mi] : mi;
+        const uint32_t cloud = P.mcloud ? (uint32_t)P.mcloud[m] : 0u;
+        const CloudDev cd = P.clouds[cloud];
+        const int kq = cd.k;
+        const float thres_n = cd.thres_n;
+        const int64_t cend = cd.base + cd.n;
         const uint64_t cand0 = ncand;
         const float4 q = P.mpts[m];
         const int32_t jq = __float_as_int(q.w);
@@ -721,7 +745,8 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) 
                     if (__ballot(big)) break;
                 }
             }
-            if (level != c_level || cq[0] != c_cq0 || cq[1] != c_cq1 || cq[2] != c_cq2) {
+            if (level != c_level || cq[0] != c_cq0 || cq[1] != c_cq1 || cq[2] != c_cq2 || cloud != c_cloud) {
+                c_cloud = cloud;
                 c_level = level;
                 c_cq0 = cq[0];
                 c_cq1 = cq[1];
@@ -1025,11 +1050,49 @@ void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false, bo
 void launch_self(const egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
     if (h->knn_lanes == 64 && !P.qbound) {
         if (P.nchunks < 1) return;
-        hipLaunchKernelGGL(k_knn_wave, dim3((unsigned)P.nchunks), dim3(KBLK), 0, s, P);
+        hipLaunchKernelGGL(k_knn_wave<false>, dim3((unsigned)P.nchunks), dim3(KBLK), 0, s, P);
         LAUNCH_CHECK();
         return;
     }
     launch_any(P, K, s);
+}
+
+// One neighbour-count class: the search in the handle's launch shape, then the queries it handed back searched
+// again in Morton order (the list fills in the order the waves reach their atomics, so it is sorted first):
+// queries with fewer than k points within their radius bound, without the bound, one lane per query; long
+// queries (more candidates than the cap), one wave per query. Both from the handle's start level. Returns
+// how many were handed back.
+int64_t search_class(egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
+    launch_self(h, P, K, s);
+    if (!P.qbound && !P.long_cap) return 0;
+    uint64_t nb = 0;
+    HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (nb == 0) return 0;
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < h->n) ++bits;
+    k_copy_u32<<<ceil_div((int64_t)nb, KBLK), KBLK, 0, s>>>(P.retry, (int64_t)nb, P.retry + h->n);
+    LAUNCH_CHECK();
+    radix_sort_pairs(P.retry, P.retry + h->n, (int64_t)nb, bits, s, h->scratch);
+    KnnParams R = P;
+    R.qbound = nullptr;
+    R.qlevel = nullptr;
+    R.chunks = nullptr;
+    R.queue = nullptr;
+    R.long_cap = 0;
+    R.qlist = P.retry;
+    R.nlist = (int64_t)nb;
+    R.stride = 1;
+    if (P.long_cap) {
+        R.nchunks = ceil_div((int64_t)nb, KBLK / 64);
+        hipLaunchKernelGGL(k_knn_wave<true>, dim3((unsigned)R.nchunks), dim3(KBLK), 0, s, R);
+        LAUNCH_CHECK();
+    } else {
+        R.nchunks = ((int64_t)nb + 63) / 64;
+        launch_any(R, K, s, false, true);
+    }
+    HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));
+    return (int64_t)nb;
 }
 
 // Search parameters of the handle; with scratch, the per-point k-th distance, the counters and the bounded
@@ -1045,13 +1108,15 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     P.qbound = (h->learned && (h->learned_mode & 2) && h->qbound.p) ? h->qbound.p : nullptr;
     P.qlist = nullptr;
     P.nlist = 0;
+    // a lane-per-query search hands long queries to a wave-per-query pass
+    P.long_cap = h->knn_lanes == 64 ? 0u : (uint32_t)h->knn_long_cap;
     // the hand-back list and, after it, room for the values of its sort
-    P.retry = (scratch && P.qbound) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
+    P.retry = (scratch && (P.qbound || P.long_cap)) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
     P.clouds = h->dclouds.p;
     P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     P.knn = h->knn.p;
     P.kth = scratch ? h->scratch.get<float>(15, h->n) : nullptr;
-    P.counters = scratch ? reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 9)) : nullptr;
+    P.counters = scratch ? reinterpret_cast<unsigned long long*>(h->scratch.get<uint64_t>(14, 10)) : nullptr;
     P.queue = nullptr;
     P.bigmask = 0;
     for (int l = 0; l < h->nlev; ++l)
@@ -1131,7 +1196,7 @@ void run_knn_clouds(egt_ctx* h) {
     KnnParams P = knn_params(h);
     const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
     h->work_classes = classes;
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));
     unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
@@ -1143,44 +1208,17 @@ void run_knn_clouds(egt_ctx* h) {
             HIP_CHECK(hipMemsetAsync(queues, 0, sizeof(uint64_t), s));
             P.queue = queues;
         }
-        launch_self(h, P, cr.K, s);
-        if (P.qbound) {
-            // queries with fewer than k points within their proposed radius: searched again without the
-            // bound (list launch, each lane its own query), from the handle's start level
-            uint64_t nb = 0;
-            HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            if (nb > 0) {
-                // in Morton order again, so a wave's 64 queries are spatial neighbours (the list fills in the
-                // order the waves reach their atomics)
-                int bits = 1;
-                while (bits < 31 && ((int64_t)1 << bits) < h->n) ++bits;
-                k_copy_u32<<<ceil_div((int64_t)nb, KBLK), KBLK, 0, s>>>(P.retry, (int64_t)nb, P.retry + h->n);
-                LAUNCH_CHECK();
-                radix_sort_pairs(P.retry, P.retry + h->n, (int64_t)nb, bits, s, h->scratch);
-                KnnParams R = P;
-                R.qbound = nullptr;
-                R.qlevel = nullptr;
-                R.chunks = nullptr;
-                R.queue = nullptr;
-                R.qlist = P.retry;
-                R.nlist = (int64_t)nb;
-                R.nchunks = ((int64_t)nb + 63) / 64;
-                R.stride = 1;
-                launch_any(R, cr.K, s, false, true);
-                HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));
-                handed_back += (int64_t)nb;
-            }
-        }
+        handed_back += search_class(h, P, cr.K, s);
     }
     HIP_CHECK(hipEventRecord(e1, s));
-    uint64_t hc[9];
+    uint64_t hc[10];
     HIP_CHECK(hipMemcpyAsync(hc, P.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     HIP_CHECK(hipEventElapsedTime(&h->stats.knn_ms, e0, e1));
-    h->stats.learned_bound_retries = handed_back;
+    h->stats.learned_bound_retries = handed_back - (int64_t)hc[9];
     h->stats.knn_insertions = (int64_t)hc[7];
     h->stats.knn_max_candidates = (int64_t)hc[8];
+    h->stats.knn_long_queries = (int64_t)hc[9];
     h->stats.knn_candidates = (int64_t)hc[0];
     h->stats.invalid_rows = (int64_t)hc[1];
     h->stats.learned_queries = (int64_t)hc[2];
@@ -1208,7 +1246,8 @@ void run_knn_query(egt_ctx* h, const float* dqx, const float* dqy, const float* 
     h->qpts.alloc((size_t)std::max<int64_t>(nq, 1));
     h->qvcell.alloc((size_t)std::max<int64_t>(nq, 1));
     KnnParams P = knn_params(h);
-    HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
+    P.long_cap = 0;
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     CloudDev qcd{};
@@ -1332,6 +1371,7 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
     CloudDev* qcloud = reinterpret_cast<CloudDev*>(w + 14 * NQ + 2 * QNA_COUNTERS);
     hipStream_t s = h->stream;
     KnnParams P = knn_params(h, false);
+    P.long_cap = 0;
     CloudDev qcd{};
     qcd.base = 0;
     qcd.n = nq;
@@ -1373,11 +1413,13 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
 // next to dense clusters), not by the bulk of the work. Rows do not depend on the choice. Levels -SUB .. top
 // are tried; trial_ms / trial_cost get each level's sample time and score at index level + SUB (-1 for
 // untried levels).
-// Launch-shape tuner: times the handle's current search (start level, learned proposal) on the same evenly
-// spread sample of 64-query chunks in each launch shape (one lane per query with one workgroup per 256
-// queries, the same as one persistent work-queue launch, one wave per query; the wave shape is not offered
-// with the radius bound), each shape twice after one warm-up launch, and keeps the fastest (best of its two
-// runs). Rows do not depend on the shape.
+// Launch-shape tuner: times the handle's current search (start level, learned proposal) over all queries in
+// each launch shape (one lane per query with one workgroup per 256 queries, the same as one persistent
+// work-queue launch, one wave per query; the wave shape is not offered with the radius bound), each after a
+// warm-up on the autotuner's sample, and keeps the fastest. A sample's time would mislead here: it is set by
+// its slowest wave, and one query per lane-per-query wave can test tens of thousands of candidates (on a
+// 16M-point cluster mixture the 3% sample took 29 ms in that shape against 3.8 ms with a wave per query, while
+// the full searches took 65 and 76 ms). Rows do not depend on the shape.
 int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     hipStream_t s = h->stream;
     h->clouds[0].k = k;
@@ -1385,36 +1427,36 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     prepare_clouds(h);
     KnnParams P = knn_params(h);
     const int64_t total = (h->n + 63) / 64;
-    P.stride = (int)std::max<int64_t>(1, total / 8192);
-    P.nchunks = (total + P.stride - 1) / P.stride;
+    KnnParams W = P;  // warm-up: the autotuner's sample
+    W.stride = (int)std::max<int64_t>(1, total / 8192);
+    W.nchunks = (total + W.stride - 1) / W.stride;
+    P.stride = 1;
+    P.nchunks = total;
     const int K = kclass(k);
     unsigned long long* queue = P.counters + 5;
     Event e0, e1;
-    auto launch = [&](int shape) {
-        KnnParams Q = P;
+    auto launch = [&](const KnnParams& B, int shape) {
+        KnnParams Q = B;
         if (shape == 1) {
             HIP_CHECK(hipMemsetAsync(queue, 0, sizeof(uint64_t), s));
             Q.queue = queue;
         }
         h->knn_lanes = shape == 2 ? 64 : 1;
-        launch_self(h, Q, K, s);
+        if (shape == 2) Q.long_cap = 0;
+        search_class(h, Q, K, s);  // with the hand-back pass of the lane-per-query shapes
     };
     int best = 0;
     float best_ms = 0.0f;
     for (int shape = 0; shape < 3; ++shape) {
         shape_ms[shape] = -1.0f;
         if (shape == 2 && P.qbound) continue;
-        launch(shape);
+        launch(W, shape);
+        HIP_CHECK(hipEventRecord(e0, s));
+        launch(P, shape);
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipStreamSynchronize(s));
         float t = 0.0f;
-        for (int rep = 0; rep < 2; ++rep) {
-            HIP_CHECK(hipEventRecord(e0, s));
-            launch(shape);
-            HIP_CHECK(hipEventRecord(e1, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            float ms = 0.0f;
-            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            t = rep == 0 ? ms : std::min(t, ms);
-        }
+        HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
         shape_ms[shape] = t;
         if (shape == 0 || t < best_ms) {
             best = shape;
@@ -1436,6 +1478,7 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     h->learned = 0;
     KnnParams P = knn_params(h);
     h->learned = saved_learned;
+    P.long_cap = 0;  // every query's whole work counts
     const int64_t total = (h->n + 63) / 64;
     P.stride = (int)std::max<int64_t>(1, total / 8192);
     P.nchunks = (total + P.stride - 1) / P.stride;
@@ -1447,7 +1490,7 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
     double best_cost = 0.0;
     for (int l = -SUB; l <= top; ++l) {
         P.start_level = l + SUB;
-        HIP_CHECK(hipMemsetAsync(P.counters, 0, 9 * sizeof(uint64_t), s));
+        HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));
         HIP_CHECK(hipEventRecord(e0, s));
         launch_any(P, K, s);
         HIP_CHECK(hipEventRecord(e1, s));

@@ -28,6 +28,7 @@ OPT_START_LEVEL, OPT_LEARNED, OPT_TAUBIN_GRAPH, OPT_MAX_PTS_IN_CUBE, OPT_KNN_PER
 OPT_TAUBIN_REGULARIZED = 6
 OPT_LEARNED_MODE = 7         # learned proposal: 1 = start level (default), 2 = radius bound, 3 = both
 OPT_KNN_LANES = 8            # kNN self search: 1 = one lane per query (default), 64 = one wave per query
+OPT_KNN_LONG_QUERY = 9       # lane-per-query search: candidates after which a query goes to a wave (2048; 0 = never)
 START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
 SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
 AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
@@ -52,7 +53,8 @@ class Stats(ctypes.Structure):
                 ("query_invalid_rows", ctypes.c_int64), ("query_ms", ctypes.c_float), ("query_k", ctypes.c_int32),
                 ("taubin_pass_ms", ctypes.c_float), ("taubin_setup_ms", ctypes.c_float),
                 ("taubin_overflow_tiles", ctypes.c_int32), ("learned_bound_retries", ctypes.c_int64),
-                ("knn_insertions", ctypes.c_int64), ("knn_max_candidates", ctypes.c_int64)]
+                ("knn_insertions", ctypes.c_int64), ("knn_max_candidates", ctypes.c_int64),
+                ("knn_long_queries", ctypes.c_int64)]
 
     def as_dict(self):
         d = {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -296,8 +298,8 @@ class GridStructure(_Provider):
         return lvl.value, ms, cost
 
     def autotune_shape(self, k, thres=1e8):
-        """Times the current search (start level, learned proposal) in each kNN launch shape on a sample and
-        keeps the fastest (no reference counterpart). Returns (shape, sample ms per shape): 0 = one lane per
+        """Times the current search (start level, learned proposal) in each kNN launch shape over all queries
+        and keeps the fastest (no reference counterpart). Returns (shape, ms per shape): 0 = one lane per
         query, 1 = the same as one persistent launch, 2 = one wave per query; -1 where not offered."""
         shape = ctypes.c_int32()
         ms = np.empty(3, np.float32)

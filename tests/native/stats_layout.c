@@ -13,6 +13,6 @@ int main(void) {
     F(normals_ms); F(grid_ms); F(propose_ms); F(knn_cells); F(taubin_tile_ms); F(taubin_tile);
     F(taubin_fallback_tiles); F(query_count); F(query_candidates); F(query_invalid_rows); F(query_ms);
     F(query_k); F(taubin_pass_ms); F(taubin_setup_ms); F(taubin_overflow_tiles); F(learned_bound_retries);
-    F(knn_insertions); F(knn_max_candidates);
+    F(knn_insertions); F(knn_max_candidates); F(knn_long_queries);
     return 0;
 }

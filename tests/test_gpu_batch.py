@@ -24,19 +24,26 @@ def _clouds():
     return clouds, np.array(ks, np.int32)
 
 
-@pytest.mark.parametrize("persistent,lanes", [(0, 1), (1, 1), (0, 64)])
-def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent, lanes):
-    """The kNN as one wave per 64-query chunk with a lane per query (default), as a work-queue launch, or
-    with a wave per query."""
+@pytest.mark.parametrize("persistent,lanes,cap", [(0, 1, None), (1, 1, None), (0, 64, None), (0, 1, 48)])
+def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent, lanes, cap):
+    """The kNN as one wave per 64-query chunk with a lane per query (default), as a work-queue launch, with
+    a wave per query, or with most queries handed on to the wave-per-query pass (cap of 48 candidates; the
+    pass then takes queries of every cloud from one list)."""
     E, O = egt_mod, oracle_mod
     clouds, ks = _clouds()
     lam, mu, alpha, its = 0.5, -0.53, 0.65, 12
     _log("batch of", len(clouds), "clouds, sizes", [c[0].size for c in clouds], "persistent", persistent,
-         "lanes", lanes)
+         "lanes", lanes, "cap", cap)
     b = E.BatchGrid(clouds)
     b.set_option(E.OPT_KNN_PERSISTENT, persistent)
     b.set_option(E.OPT_KNN_LANES, lanes)
+    if cap is not None:
+        b.set_option(E.OPT_KNN_LONG_QUERY, cap)
     b.knn(ks)
+    if cap is not None:
+        long_q = b.stats()["knn_long_queries"]
+        _log("queries handed on", long_q)
+        assert long_q > int(b.sizes.sum()) // 4
     refs = []
     for c, pts in enumerate(clouds):
         ref = O.pipeline(*pts, int(ks[c]), its, lam, mu, alpha, normals=True)

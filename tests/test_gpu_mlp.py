@@ -113,6 +113,7 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
     bsafe = model.bound_safety
     for mode in modes:
         g = E.GridStructure(evalc, 1, 1e-6, 1024, 4)
+        g.set_option(E.OPT_KNN_LONG_QUERY, 0)  # every query's whole work in the counters compared below
         if mode == "occupancy":
             g.set_option(E.OPT_START_LEVEL, E.START_LEVEL_OCCUPANCY)  # one global level from the mean occupancy
         if mode in ("autotuned", "learned_bound"):

@@ -89,6 +89,30 @@ def test_shape_tuner_keeps_rows(egt_mod, oracle_mod):
     g.close()
 
 
+@pytest.mark.parametrize("cap", [0, 1, 200, 2048])
+def test_long_queries_handed_to_waves(egt_mod, oracle_mod, cap):
+    """A lane-per-query search hands queries past cap candidates to a wave-per-query pass (cap 1: nearly every
+    query; 0: none). Crowded cells (about 3,800 points each) make long queries; rows equal the oracle's."""
+    E, O = egt_mod, oracle_mod
+    pts = _crowded_blob()
+    g = E.GridStructure(pts, 1, 1e-6, 64, 4)
+    g.set_option(E.OPT_KNN_LONG_QUERY, cap)
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    got = g.GetKNNIndexes()
+    st = g.stats()
+    g.close()
+    og = O.Grid(*pts, B=64)
+    np.testing.assert_array_equal(got, og.to_original(og.knn(40)[0]))
+    n = pts[0].size
+    _log("cap", cap, "handed on", st["knn_long_queries"], "of", n, "max candidates", st["knn_max_candidates"])
+    if cap == 0:
+        assert st["knn_long_queries"] == 0
+    elif cap == 1:  # every query that tests more than one candidate
+        assert 0.99 * n < st["knn_long_queries"] <= n
+    else:
+        assert st["knn_long_queries"] <= n
+
+
 @pytest.mark.parametrize("lanes", [1, 64])
 @pytest.mark.parametrize("start_level", ["occupancy", -3, -2, -1, 0, 2, 5, 9])
 def test_knn_independent_of_start_level(egt_mod, oracle_mod, start_level, lanes):
