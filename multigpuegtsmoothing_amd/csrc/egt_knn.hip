@@ -64,7 +64,8 @@ struct KnnParams {
                                            // null for positions 0, 64, 128, ...
     const uint32_t* __restrict__ qlist;    // list launch: Morton positions of the queries, 64 per wave
     int64_t nlist;                         // its length
-    uint32_t* __restrict__ retry;          // bounded search: queries to search again without the bound
+    uint32_t* __restrict__ retry;          // queries handed back (fewer than k within the radius bound, or long)
+    int64_t retry_cap;                     // entries of retry
     unsigned long long* __restrict__ queue;  // work-queue counter (persistent launch) or null
     int32_t* __restrict__ knn;             // rows, cloud after cloud, local sorted indices
     float* __restrict__ kth;               // per sorted point: k-th distance of valid rows, else -1
@@ -456,7 +457,10 @@ __global__ void __launch_bounds__(KBLK)
 
         most = max(most, (uint32_t)min<uint64_t>(cand - cand0, 0xFFFFFFFFull));
         const bool handed_back = gave_up || (P.qbound && (uint32_t)key_u(L[K - 1]) == 0xFFFFFFFFu);
-        if (handed_back) P.retry[atomicAdd(&P.counters[6], 1ull)] = (uint32_t)m;
+        if (handed_back) {
+            const unsigned long long at = atomicAdd(&P.counters[6], 1ull);
+            if (at < (unsigned long long)P.retry_cap) P.retry[at] = (uint32_t)m;
+        }
         nlong += gave_up ? 1u : 0u;
         if (!handed_back) {
         // replay of the reference schedule on the exact list
@@ -1063,12 +1067,15 @@ void launch_self(const egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
 // queries (more candidates than the cap), one wave per query. Both from the handle's start level. Returns
 // how many were handed back.
 int64_t search_class(egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
+    const bool hand_back = P.qbound || P.long_cap;
+    if (hand_back) HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));  // the list's length
     launch_self(h, P, K, s);
-    if (!P.qbound && !P.long_cap) return 0;
+    if (!hand_back) return 0;
     uint64_t nb = 0;
     HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     if (nb == 0) return 0;
+    if ((int64_t)nb > P.retry_cap) throw std::runtime_error("kNN hand-back list longer than its buffer");
     int bits = 1;
     while (bits < 31 && ((int64_t)1 << bits) < h->n) ++bits;
     k_copy_u32<<<ceil_div((int64_t)nb, KBLK), KBLK, 0, s>>>(P.retry, (int64_t)nb, P.retry + h->n);
@@ -1091,7 +1098,6 @@ int64_t search_class(egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
         R.nchunks = ((int64_t)nb + 63) / 64;
         launch_any(R, K, s, false, true);
     }
-    HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));
     return (int64_t)nb;
 }
 
@@ -1112,6 +1118,7 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     P.long_cap = h->knn_lanes == 64 ? 0u : (uint32_t)h->knn_long_cap;
     // the hand-back list and, after it, room for the values of its sort
     P.retry = (scratch && (P.qbound || P.long_cap)) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
+    P.retry_cap = P.retry ? h->n : 0;
     P.clouds = h->dclouds.p;
     P.mcloud = h->nclouds > 1 ? h->mcloud.p : nullptr;
     P.knn = h->knn.p;
@@ -1426,6 +1433,7 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     h->clouds[0].thres_n = thres_n;
     prepare_clouds(h);
     KnnParams P = knn_params(h);
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));  // scratch holds earlier contents
     const int64_t total = (h->n + 63) / 64;
     KnnParams W = P;  // warm-up: the autotuner's sample
     W.stride = (int)std::max<int64_t>(1, total / 8192);

@@ -89,6 +89,23 @@ def test_shape_tuner_keeps_rows(egt_mod, oracle_mod):
     g.close()
 
 
+def test_shape_tuner_after_long_queries(egt_mod, oracle_mod):
+    """The shape tuner on a handle whose previous search handed nearly every query on (cap 1): its own launches
+    hand queries back too, and their list starts empty; rows equal the oracle's afterwards."""
+    E, O = egt_mod, oracle_mod
+    pts = _crowded_blob()
+    g = E.GridStructure(pts, 1, 1e-6, 64, 4)
+    g.set_option(E.OPT_KNN_LONG_QUERY, 1)
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    shape, ms = g.autotune_shape(40)
+    _log("shape tuner after long queries: choice", shape, "ms", np.round(ms, 3).tolist())
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    got = g.GetKNNIndexes()
+    g.close()
+    og = O.Grid(*pts, B=64)
+    np.testing.assert_array_equal(got, og.to_original(og.knn(40)[0]))
+
+
 @pytest.mark.parametrize("cap", [0, 1, 200, 2048])
 def test_long_queries_handed_to_waves(egt_mod, oracle_mod, cap):
     """A lane-per-query search hands queries past cap candidates to a wave-per-query pass (cap 1: nearly every
