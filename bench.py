@@ -716,7 +716,9 @@ def headline_main(args):
     roof = {"kernel": f"k_taubin_tile<{L['taubin_tile']}>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("mixture"),
             "bytes_per_launch": taub_bytes, "avg_launch_ms": L["taubin_pass_ms"],
-            "knn": {"kernel": "k_knn<40> (+ proposal)", "bound": "valu/latency", "achieved_GBs": knn_gbs,
+            "knn": {"kernel": ("k_knn_wave" if L.get("knn_shape") == "wave per query" else "k_knn<40>")
+                              + (" (+ proposal)" if main_mode == "learned" else ""),
+                    "shape": L.get("knn_shape"), "bound": "valu/latency", "achieved_GBs": knn_gbs,
                     "frac": knn_gbs / HBM_PEAK_GBS, "bytes_per_query": 12 + 4 * k},
             "grid": {"kernels": "grid build (cube, key, sorts, gathers, level tables)", "bound": "hbm",
                      "achieved_GBs": grid_gbs, "frac": grid_gbs / HBM_PEAK_GBS, "bytes_per_point": 32}}

@@ -33,6 +33,10 @@ constexpr int KBLK = 256;
 constexpr uint32_t SPLIT_COUNT = 96;
 // The same for the wave-per-query search, whose batches of 64 make scanning a populous cell cheaper.
 constexpr uint32_t WAVE_SPLIT_COUNT = 96;
+// Wave-per-query search: a batch with more survivors than this is sorted and merged into the list, fewer are
+// inserted one at a time. Measured on a 16M-point cluster mixture (learned starts 59.6 / 58.7 / 57.9 ms at 16,
+// 36 and never; from level 1, 65.7 / 67.1 / 74.3 ms) and a 4M-point torus (level 2: 10.0 / 10.3 / 12.3 ms).
+constexpr int WAVE_SORT_MIN = 16;
 // Keys (distance bits << 32 | sorted index) of non-negative distances, read as IEEE doubles, are positive
 // and order exactly as the integers do, so the list is held as doubles and merged with FP64 min / max (full
 // rate on gfx950; keys of zero distance are denormal doubles, which compute kernels keep). The empty key is
@@ -860,8 +864,9 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) 
                     uint64_t smask = __ballot(sv);
                     const int ns = __builtin_popcountll(smask);
                     nins += (uint32_t)ns;
-                    if (ns > 16) {
-                        L = wv::merge64(L, wv::sort64(sv ? key_d(kk) : key_d(KEY_EMPTY)));
+                    if (ns > WAVE_SORT_MIN) {
+                        const double S = wv::sort64(sv ? key_d(kk) : key_d(KEY_EMPTY));
+                        L = key_u(wv::readlane_d(L, 0)) == KEY_EMPTY ? S : wv::merge64(L, S);  // empty list: S
                     } else {
                         const double kd = key_d(kk);
                         while (smask) {
