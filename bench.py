@@ -52,7 +52,7 @@ def parse():
                         "mixture: the 16M-point cluster mixture, plain vs learned vs autotuned at k=40 and k=64")
     p.add_argument("--clouds", type=int, default=512)
     p.add_argument("--mixture-ks", default="40,64", help="mixture workload: neighbour counts to run")
-    p.add_argument("--no-extras", action="store_true", help="headline: skip the batch and torus keys")
+    p.add_argument("--no-extras", action="store_true", help="headline: skip the k=64, batch and torus keys")
     p.add_argument("--no-per-cloud", action="store_true",
                    help="headline / batch: skip the one-handle-per-cloud plain path")
     p.add_argument("--mixture-iters", type=int, default=150, help="headline: Taubin iterations (150)")
@@ -368,7 +368,8 @@ def batch_result(args, steps, warmup, per_cloud=True):
     cell tables for all clouds) + (learned proposal) + kNN (one launch per neighbour-count class over 64-query
     chunks) + EGG + Taubin (one captured graph over all clouds). Coordinates are resident in HBM before timing.
     The learned batch (model trained on the GPU on 64 clouds of other seeds) and the plain batch (every query
-    from its own cell) are each timed over `steps` steps; beside them, when per_cloud, the plain GPU path of
+    from its own cell), each with the kNN launch shape per neighbour-count class that the class tuner picks
+    for it (outside the timing), are each timed over `steps` steps; beside them, when per_cloud, the plain GPU path of
     the reference interface (one handle per cloud, plain search, EGG, Taubin, cloud after cloud), timed once
     after a warm-up pass."""
     import torch
@@ -393,10 +394,20 @@ def batch_result(args, steps, warmup, per_cloud=True):
     train_s = time.perf_counter() - t0
     tb.close()
     del tclouds
+    # launch shape per neighbour-count class, tuned once for the learned starts and once for the plain search,
+    # outside the timed steps
+    shapes = {}
+    for learned in (True, False):
+        b = E.BatchGrid(sizes, device=dev)
+        if learned:
+            model.propose(b, ks)
+        shapes[learned] = b.autotune_shape(ks)
+        b.close()
     torch.cuda.synchronize()
 
     def step(learned=True):
         b = E.BatchGrid(sizes, device=dev)
+        b.set_option(E.OPT_KNN_CLASS_SHAPES, shapes[learned][0])
         if learned:
             model.propose(b, ks)
         b.knn(ks)
@@ -427,7 +438,11 @@ def batch_result(args, steps, warmup, per_cloud=True):
            "plain_knn_ms": pst["knn_ms"], "learned_knn_ms": st["knn_ms"] + st["propose_ms"],
            "knn_speedup_learned": pst["knn_ms"] / (st["knn_ms"] + st["propose_ms"]),
            "learned_end_to_end_speedup": plain_dt / dt,
-           "train": {"clouds": 64, "seconds": train_s, "mse": mse}}
+           "train": {"clouds": 64, "seconds": train_s, "mse": mse},
+           "knn_class_shapes": {path: {f"k<={8 * (ci + 1)}": {"shape": ("lane per query", "persistent", "wave per query")[
+               ((packed >> (2 * ci)) & 3) - 1], "ms": [round(float(x), 3) for x in ms[ci]]}
+               for ci in range(8) if (packed >> (2 * ci)) & 3}
+               for path, (packed, ms) in (("learned", shapes[True]), ("plain", shapes[False]))}}
     if per_cloud:
         # The plain GPU path on the same clouds: one GridStructure per cloud, the plain (own-cell) search, EGG
         # and Taubin, cloud after cloud (what a caller of the reference's interface does with many clouds).
@@ -738,6 +753,15 @@ def headline_main(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_mixture(k, iters)
     if not args.no_extras:
+        # the same cloud at k=64 (the metric's second neighbour count), fewer steps
+        r64 = mixture_modes(args, 64, min(steps, 3), min(warmup, 1))
+        out["mixture_k64"] = {"ms_per_step": {m: v["ms_per_step"] for m, v in r64["modes"].items()},
+                              "knn_ms": {m: v["knn_ms"] for m, v in r64["modes"].items()},
+                              "knn_shape": {m: v["knn_shape"] for m, v in r64["modes"].items()},
+                              "learned_end_to_end_speedup": r64.get("learned_end_to_end_speedup"),
+                              "learned_knn_speedup": r64.get("learned_knn_speedup"),
+                              "best_plain": r64.get("best_plain"), "rows_equal": r64["rows_equal"],
+                              "start_level_autotuned": r64["start_level_autotuned"]}
         out["batch_512_clouds"] = batch_result(args, min(steps, 3), min(warmup, 1),
                                                per_cloud=not args.no_per_cloud)
         for key in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data"):

@@ -110,9 +110,14 @@ typedef enum {
     EGT_OPT_KNN_LANES = 8,       /* kNN self search: lanes per query, 1 = one lane per query with the list in its
                                     registers (default), 64 = one wave per query with the list spread over the
                                     lanes. Rows do not depend on it */
-    EGT_OPT_KNN_LONG_QUERY = 9   /* kNN self search with one lane per query: a query that has tested more
+    EGT_OPT_KNN_LONG_QUERY = 9,  /* kNN self search with one lane per query: a query that has tested more
                                     candidates than this is handed on to a wave-per-query pass (its wave no
                                     longer waits for it); 0 = never; default 2048. Rows do not depend on it */
+    EGT_OPT_KNN_CLASS_SHAPES = 10 /* kNN self search: launch shape per neighbour-count class (k <= 8, 16, ..,
+                                    64 are classes 0..7): bits 2i..2i+1 hold class i's shape + 1 (1 = one lane
+                                    per query, 2 = the same as one persistent launch, 3 = one wave per query,
+                                    0 = the handle-wide EGT_OPT_KNN_LANES / EGT_OPT_KNN_PERSISTENT; default 0).
+                                    egt_batch_autotune_shape returns such a value. Rows do not depend on it */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -242,6 +247,15 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
  * it, shape_ms (optional, 3 floats) each shape's time in ms, -1 where not offered. The handle's neighbour rows
  * are not valid afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms);
+
+/* Per-class launch-shape tuner for a batch (no reference counterpart): with neighbour counts k[c] per cloud
+ * and radius thres (as egt_batch_knn) and the handle's start level or learned proposal in place, times each
+ * neighbour-count class's full search in each launch shape (as egt_autotune_shape: lane per query,
+ * persistent, wave per query), each after a warm-up launch, and keeps the fastest per class. *shapes
+ * (optional) receives the choices packed as EGT_OPT_KNN_CLASS_SHAPES takes them, for later handles; class_ms
+ * (optional, 24 floats) each class's time per shape (index 3 class + shape), -1 where not run. The handle's
+ * neighbour rows are not valid afterwards: run egt_batch_knn. */
+egt_status egt_batch_autotune_shape(egt_handle h, const int32_t* k, float thres, int64_t* shapes, float* class_ms);
 
 /* Provider destructors, GridKNNCuda.cu:1005-1021, GridMeshCuda1.cu:542-551; also frees what EGT_CUDA
  * leaked (EGT_CUDA.cu:490-492). */

@@ -365,6 +365,10 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
             if (value < 0 || value > UINT32_MAX) return fail(EGT_ERR_INVALID_ARG, "long-query cap must be in [0, 2^32)");
             h->knn_long_cap = value;
             return EGT_OK;
+        case EGT_OPT_KNN_CLASS_SHAPES:
+            if (value < 0 || value >= (1 << 16)) return fail(EGT_ERR_INVALID_ARG, "class shapes: 8 fields of 2 bits");
+            for (int c = 0; c < 8; ++c) h->class_shape[c] = (int)((value >> (2 * c)) & 3) - 1;
+            return EGT_OK;
     }
     return fail(EGT_ERR_INVALID_ARG, "unknown option");
 }
@@ -488,6 +492,32 @@ egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres) {
         h->k = kmax;
         h->csr_valid = false;
         h->smoothed = false;
+    });
+}
+
+egt_status egt_batch_autotune_shape(egt_handle h, const int32_t* k, float thres, int64_t* shapes, float* class_ms) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 2) return fail(EGT_ERR_STATE, "egt_batch_autotune_shape needs a batch handle");
+    if (!k) return fail(EGT_ERR_INVALID_ARG, "null k array");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    for (int c = 0; c < h->nclouds; ++c) {
+        if (k[c] < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+        if (k[c] > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    }
+    return guarded([&] {
+        if (h->search_stale) egt::refresh_search_structure(h);
+        for (int c = 0; c < h->nclouds; ++c) {
+            egt::CloudDev& cd = h->clouds[c];
+            cd.k = k[c];
+            float t = thres / cd.extent;
+            cd.thres_n = t * t;
+        }
+        float ms[24];
+        const int64_t packed = egt::autotune_class_shapes(h, ms);
+        h->csr_valid = false;
+        if (shapes) *shapes = packed;
+        if (class_ms)
+            for (int i = 0; i < 24; ++i) class_ms[i] = ms[i];
     });
 }
 

@@ -230,6 +230,8 @@ struct egt_ctx {
                            // radius, 3 = both
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int knn_lanes = 1;       // kNN self search: 1 = one lane per query, 64 = one wave per query
+    int class_shape[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // per neighbour-count class (k <= 8, 16, .., 64):
+                                                           // 0 lane, 1 persistent, 2 wave; -1 handle-wide
     int64_t knn_long_cap = 2048;  // lane-per-query search: candidates after which a query goes to a wave
                                   // (0 = never)
     int taubin_graph = 1;
@@ -289,6 +291,9 @@ int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, floa
 // Times the current search in each launch shape on a sample and keeps the fastest: 0 = one lane per query,
 // 1 = the same as one persistent launch, 2 = one wave per query. shape_ms: 3 sample times, -1 if not offered.
 int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms);
+// The same per neighbour-count class of the handle's clouds (k and radius already set): ms is 8 x 3, the
+// return value the choices packed as EGT_OPT_KNN_CLASS_SHAPES takes them.
+int64_t autotune_class_shapes(egt_ctx* h, float* ms);
 
 // learned proposal (egt_mlp.hip)
 int mlp_param_count();

@@ -1059,27 +1059,40 @@ void launch_any(const KnnParams& P, int K, hipStream_t s, bool query = false, bo
     }
 }
 
-// Self search of a work list in the handle's launch shape: one wave per query (lanes = 64, not with the
-// radius bound) or one lane per query.
-void launch_self(const egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
-    if (h->knn_lanes == 64 && !P.qbound) {
-        if (P.nchunks < 1) return;
-        hipLaunchKernelGGL(k_knn_wave<false>, dim3((unsigned)P.nchunks), dim3(KBLK), 0, s, P);
-        LAUNCH_CHECK();
-        return;
-    }
-    launch_any(P, K, s);
+// Launch shape of a neighbour-count class: the handle's per-class choice (from the class tuner or
+// EGT_OPT_KNN_CLASS_SHAPES), else the handle-wide options. 0 = one lane per query, 1 = the same as one
+// persistent launch, 2 = one wave per query.
+int class_shape(const egt_ctx* h, int K) {
+    const int c = h->class_shape[K / 8 - 1];
+    if (c >= 0) return c;
+    return h->knn_lanes == 64 ? 2 : (h->knn_persistent ? 1 : 0);
 }
 
-// One neighbour-count class: the search in the handle's launch shape, then the queries it handed back searched
-// again in Morton order (the list fills in the order the waves reach their atomics, so it is sorted first):
-// queries with fewer than k points within their radius bound, without the bound, one lane per query; long
-// queries (more candidates than the cap), one wave per query. Both from the handle's start level. Returns
-// how many were handed back.
-int64_t search_class(egt_ctx* h, const KnnParams& P, int K, hipStream_t s) {
+// One neighbour-count class: the search in the given launch shape (the wave shape is not offered with the
+// radius bound: such a class runs one lane per query), then the queries it handed back searched again in
+// Morton order (the list fills in the order the waves reach their atomics, so it is sorted first): queries
+// with fewer than k points within their radius bound, without the bound, one lane per query; long queries
+// (more candidates than the cap), one wave per query. Both from the handle's start level. Returns how many
+// were handed back.
+int64_t search_class(egt_ctx* h, KnnParams P, int K, hipStream_t s, int shape) {
+    if (shape == 2 && P.qbound) shape = 0;
+    if (shape == 1) {  // persistent: the work-queue counter starts at zero
+        P.queue = P.counters + 5;
+        HIP_CHECK(hipMemsetAsync(P.queue, 0, sizeof(uint64_t), s));
+    } else {
+        P.queue = nullptr;
+    }
+    if (shape == 2) P.long_cap = 0;
     const bool hand_back = P.qbound || P.long_cap;
     if (hand_back) HIP_CHECK(hipMemsetAsync(P.counters + 6, 0, sizeof(uint64_t), s));  // the list's length
-    launch_self(h, P, K, s);
+    if (shape == 2) {
+        if (P.nchunks > 0) {
+            hipLaunchKernelGGL(k_knn_wave<false>, dim3((unsigned)P.nchunks), dim3(KBLK), 0, s, P);
+            LAUNCH_CHECK();
+        }
+    } else {
+        launch_any(P, K, s);
+    }
     if (!hand_back) return 0;
     uint64_t nb = 0;
     HIP_CHECK(hipMemcpyAsync(&nb, P.counters + 6, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
@@ -1214,18 +1227,13 @@ void run_knn_clouds(egt_ctx* h) {
     const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
     h->work_classes = classes;
     HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));
-    unsigned long long* queues = P.counters + 5;  // one counter, reset before each class
     Event e0, e1;
     HIP_CHECK(hipEventRecord(e0, s));
     int64_t handed_back = 0;
     for (const WorkClass& cr : classes) {
         P.chunks = h->nclouds > 1 ? h->work_lists.p + cr.first : nullptr;
         P.nchunks = cr.count;
-        if (h->knn_persistent) {
-            HIP_CHECK(hipMemsetAsync(queues, 0, sizeof(uint64_t), s));
-            P.queue = queues;
-        }
-        handed_back += search_class(h, P, cr.K, s);
+        handed_back += search_class(h, P, cr.K, s, class_shape(h, cr.K));
     }
     HIP_CHECK(hipEventRecord(e1, s));
     uint64_t hc[10];
@@ -1451,17 +1459,9 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     P.stride = 1;
     P.nchunks = total;
     const int K = kclass(k);
-    unsigned long long* queue = P.counters + 5;
     Event e0, e1;
     auto launch = [&](const KnnParams& B, int shape) {
-        KnnParams Q = B;
-        if (shape == 1) {
-            HIP_CHECK(hipMemsetAsync(queue, 0, sizeof(uint64_t), s));
-            Q.queue = queue;
-        }
-        h->knn_lanes = shape == 2 ? 64 : 1;
-        if (shape == 2) Q.long_cap = 0;
-        search_class(h, Q, K, s);  // with the hand-back pass of the lane-per-query shapes
+        search_class(h, B, K, s, shape);  // with the hand-back pass of the lane-per-query shapes
     };
     int best = 0;
     float best_ms = 0.0f;
@@ -1483,8 +1483,50 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     }
     h->knn_persistent = best == 1 ? 1 : 0;
     h->knn_lanes = best == 2 ? 64 : 1;
+    h->class_shape[K / 8 - 1] = -1;  // the handle-wide options now say it
     h->have_knn = false;  // the trial rows cover only the sample
     return best;
+}
+
+// Per-class launch-shape tuner (batches, where the neighbour-count classes differ): for each class of the
+// current clouds, times the class's full search in each shape after one warm-up launch of that shape, and
+// keeps the fastest in the handle's per-class choice. ms: 8 classes x 3 shapes, -1 where not run. Returns the
+// choices packed as EGT_OPT_KNN_CLASS_SHAPES takes them.
+int64_t autotune_class_shapes(egt_ctx* h, float* ms) {
+    hipStream_t s = h->stream;
+    prepare_clouds(h);
+    KnnParams P = knn_params(h);
+    HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));  // scratch holds earlier contents
+    const std::vector<WorkClass> classes = build_work_lists(h, h->work_lists);
+    for (int i = 0; i < 24; ++i) ms[i] = -1.0f;
+    Event e0, e1;
+    int64_t packed = 0;
+    for (const WorkClass& cr : classes) {
+        P.chunks = h->nclouds > 1 ? h->work_lists.p + cr.first : nullptr;
+        P.nchunks = cr.count;
+        const int ci = cr.K / 8 - 1;
+        int best = 0;
+        float best_ms = 0.0f;
+        for (int shape = 0; shape < 3; ++shape) {
+            if (shape == 2 && P.qbound) continue;
+            search_class(h, P, cr.K, s, shape);
+            HIP_CHECK(hipEventRecord(e0, s));
+            search_class(h, P, cr.K, s, shape);
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            float t = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+            ms[ci * 3 + shape] = t;
+            if (shape == 0 || t < best_ms) {
+                best = shape;
+                best_ms = t;
+            }
+        }
+        h->class_shape[ci] = best;
+        packed |= (int64_t)(best + 1) << (2 * ci);
+    }
+    h->have_knn = false;  // as after the other tuners: the caller runs the search for its rows and statistics
+    return packed;
 }
 
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost) {

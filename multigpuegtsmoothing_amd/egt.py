@@ -29,6 +29,7 @@ OPT_TAUBIN_REGULARIZED = 6
 OPT_LEARNED_MODE = 7         # learned proposal: 1 = start level (default), 2 = radius bound, 3 = both
 OPT_KNN_LANES = 8            # kNN self search: 1 = one lane per query (default), 64 = one wave per query
 OPT_KNN_LONG_QUERY = 9       # lane-per-query search: candidates after which a query goes to a wave (2048; 0 = never)
+OPT_KNN_CLASS_SHAPES = 10    # per neighbour-count class launch shapes, as BatchGrid.autotune_shape returns them
 START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
 SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
 AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
@@ -121,6 +122,7 @@ def _load():
         "egt_get_stream": ([_H], ctypes.c_void_p),
         "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N, _F32N], st),
         "egt_autotune_shape": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N], st),
+        "egt_batch_autotune_shape": ([_H, _I32P, f32, ctypes.POINTER(i64), _F32N], st),
         "egt_batch_create": ([_F32P, _F32P, _F32P, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
                               ctypes.c_int32, f32, i32, _PH], st),
         "egt_batch_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -398,6 +400,16 @@ class BatchGrid(_Provider):
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (len(self.sizes),)), np.int32)
         _check(_lib.egt_batch_knn(self.handle, ks, thres))
         self.k = ks
+
+    def autotune_shape(self, k, thres=1e8):
+        """Times each neighbour-count class's search in each kNN launch shape and keeps the fastest per class
+        (no reference counterpart). Returns (packed choices for OPT_KNN_CLASS_SHAPES, ms array [8 classes, 3
+        shapes: lane per query, persistent, wave per query], -1 where not run). Run knn() afterwards."""
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (len(self.sizes),)), np.int32)
+        packed = ctypes.c_int64()
+        ms = np.empty(24, np.float32)
+        _check(_lib.egt_batch_autotune_shape(self.handle, ks, thres, ctypes.byref(packed), ms))
+        return packed.value, ms.reshape(8, 3)
 
     def GetKNNIndexes(self, cloud):
         out = np.empty((int(self.sizes[cloud]), int(self.k[cloud])), np.int32)

@@ -67,6 +67,44 @@ def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent, lan
     b.close()
 
 
+def test_batch_class_shape_tuner_keeps_rows(egt_mod):
+    """The per-class shape tuner times each neighbour-count class in each launch shape and keeps the fastest;
+    rows after it, and on a new handle given its choices through OPT_KNN_CLASS_SHAPES, equal the default
+    batch's (itself checked against the oracle above). Every packed field is a shape the tuner ran."""
+    E = egt_mod
+    clouds, ks = _clouds()
+    ref = E.BatchGrid(clouds)
+    ref.knn(ks)
+    want = [ref.GetKNNIndexes(c) for c in range(len(clouds))]
+    ref.close()
+    b = E.BatchGrid(clouds)
+    packed, ms = b.autotune_shape(ks)
+    _log("class shapes", hex(packed), "ms per class (lane, persistent, wave)", np.round(ms, 3).tolist())
+    classes = sorted({(int(k) + 7) // 8 - 1 for k in ks})
+    for ci in range(8):
+        field = (packed >> (2 * ci)) & 3
+        if ci in classes:
+            assert field in (1, 2, 3) and ms[ci, field - 1] == ms[ci][ms[ci] >= 0].min()
+        else:
+            assert field == 0 and (ms[ci] < 0).all()
+    b.knn(ks)
+    for c in range(len(clouds)):
+        np.testing.assert_array_equal(b.GetKNNIndexes(c), want[c], err_msg=f"cloud {c}")
+    b.close()
+    # every class forced to each shape through the option
+    for shape in (1, 2, 3):
+        f = E.BatchGrid(clouds)
+        f.set_option(E.OPT_KNN_CLASS_SHAPES, sum(shape << (2 * ci) for ci in range(8)))
+        f.knn(ks)
+        for c in range(len(clouds)):
+            np.testing.assert_array_equal(f.GetKNNIndexes(c), want[c], err_msg=f"shape {shape} cloud {c}")
+        f.close()
+    bad = E.BatchGrid(clouds[:2])
+    with pytest.raises(E.EGTError):
+        bad.set_option(E.OPT_KNN_CLASS_SHAPES, 1 << 16)
+    bad.close()
+
+
 def test_batch_invalid_rows_and_single_cloud(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     # a dense blob (cells above the 1e5 rule are not needed: an isolated outlier is blanked too)
