@@ -396,13 +396,21 @@ def batch_result(args, steps, warmup, per_cloud=True):
     del tclouds
     # launch shape per neighbour-count class, tuned once for the learned starts and once for the plain search,
     # outside the timed steps
+    # (the learned batch: the proposal's level alone or with its radius bound, whichever totals less)
     shapes = {}
-    for learned in (True, False):
-        b = E.BatchGrid(sizes, device=dev)
-        if learned:
-            model.propose(b, ks)
-        shapes[learned] = b.autotune_shape(ks)
-        b.close()
+    b = E.BatchGrid(sizes, device=dev)
+    model.propose(b, ks)
+    tried = {}
+    for lmode in (1, 3):
+        b.set_option(E.OPT_LEARNED_MODE, lmode)
+        packed, ms = b.autotune_shape(ks)
+        tried[lmode] = (float(sum(ms[ci][ms[ci] >= 0].min() for ci in range(8) if (ms[ci] >= 0).any())), packed, ms)
+    b.close()
+    learned_mode = min(tried, key=lambda m: tried[m][0])
+    shapes[True] = tried[learned_mode][1:]
+    b = E.BatchGrid(sizes, device=dev)
+    shapes[False] = b.autotune_shape(ks)
+    b.close()
     torch.cuda.synchronize()
 
     def step(learned=True):
@@ -410,6 +418,7 @@ def batch_result(args, steps, warmup, per_cloud=True):
         b.set_option(E.OPT_KNN_CLASS_SHAPES, shapes[learned][0])
         if learned:
             model.propose(b, ks)
+            b.set_option(E.OPT_LEARNED_MODE, learned_mode)
         b.knn(ks)
         E.EGT_CUDA(b).PerformSmoothing(iters, 0.5, -0.53, 0.65)
         st = b.stats()
@@ -439,6 +448,7 @@ def batch_result(args, steps, warmup, per_cloud=True):
            "knn_speedup_learned": pst["knn_ms"] / (st["knn_ms"] + st["propose_ms"]),
            "learned_end_to_end_speedup": plain_dt / dt,
            "train": {"clouds": 64, "seconds": train_s, "mse": mse},
+           "learned_mode": {1: "proposed level", 3: "proposed level and radius bound"}[learned_mode],
            "knn_class_shapes": {path: {f"k<={8 * (ci + 1)}": {"shape": ("lane per query", "persistent", "wave per query")[
                ((packed >> (2 * ci)) & 3) - 1], "ms": [round(float(x), 3) for x in ms[ci]]}
                for ci in range(8) if (packed >> (2 * ci)) & 3}
@@ -573,7 +583,8 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
     iterations. Paths (each warm-up + `steps` timed steps of grid + (proposal) + kNN + EGG + Taubin):
     plain_own_cell (every query from its own cell, the reference schedule, one lane per query), plain_autotuned
     (the level the autotuner picks for this cloud and the launch shape the shape tuner picks for it, tuned once
-    outside the timing) and learned (the model's start level per query and the launch shape tuned for it;
+    outside the timing) and learned (the model's start level per query, alone or with its radius bound, and the
+    launch shape, whichever pair the shape tuner times fastest;
     model trained on the GPU, outside the timing, on the plain search's radii over 2M mixture points of seed
     1000). Rows of the three are compared on the GPU after the timing."""
     import torch
@@ -602,9 +613,16 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
     g.set_option(E.OPT_START_LEVEL, lvl)
     shapes = {"plain_own_cell": (0, None), "plain_autotuned": g.autotune_shape(k)}
     g.close()
+    # the learned path: the proposal's level alone or with its radius bound, each in its fastest shape
     g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
     model.propose(g, k)
-    shapes["learned"] = g.autotune_shape(k)
+    tried = {}
+    for lmode in (1, 3):
+        g.set_option(E.OPT_LEARNED_MODE, lmode)
+        shape, ms = g.autotune_shape(k)
+        tried[lmode] = (float(ms[shape]), shape, ms)
+    learned_mode = min(tried, key=lambda m: tried[m][0])
+    shapes["learned"] = tried[learned_mode][1:]
     tune_s = time.perf_counter() - t0
     g.close()
     holder = {}
@@ -618,6 +636,7 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
                 g.set_option(E.OPT_TAUBIN_GRAPH, 0)
             if mode == "learned":
                 model.propose(g, k)
+                g.set_option(E.OPT_LEARNED_MODE, learned_mode)
             elif mode == "plain_autotuned":
                 g.set_option(E.OPT_START_LEVEL, lvl)
             g.set_option(E.OPT_KNN_PERSISTENT, 1 if shape == 1 else 0)
@@ -654,6 +673,8 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
         if mode == "learned":
             per[mode]["fallback_rate"] = st["learned_fallbacks"] / n
             per[mode]["bound_retry_rate"] = st["learned_bound_retries"] / n
+            per[mode]["learned_mode"] = {1: "proposed level", 3: "proposed level and radius bound"}[learned_mode]
+            per[mode]["tuned_ms_by_learned_mode"] = {str(m): [round(float(x), 3) for x in v[2]] for m, v in tried.items()}
     first = next(iter(rows))
     equal = bool(all(torch.equal(rows[first], rows[m]) for m in rows))
     del rows

@@ -105,8 +105,9 @@ typedef enum {
                                       1 in the lambda pass, 1 / (d^2 + 1e-8) in the mu pass */
     EGT_OPT_LEARNED_MODE = 7,    /* kNN with the learned proposal: bit 0 = start at the proposed level, bit 1 =
                                     bound the search by the proposed radius (queries with fewer than k points
-                                    inside it are searched again without the bound); default 1 (the level only:
-                                    the bound measured slower). Rows do not depend on it */
+                                    inside it are searched again without the bound); default 1 (the level only).
+                                    Which is faster depends on the cloud and k (try both with
+                                    egt_autotune_shape). Rows do not depend on it */
     EGT_OPT_KNN_LANES = 8,       /* kNN self search: lanes per query, 1 = one lane per query with the list in its
                                     registers (default), 64 = one wave per query with the list spread over the
                                     lanes. Rows do not depend on it */
@@ -241,10 +242,10 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
 /* Launch-shape tuner (no reference counterpart): times the handle's current search (its start level and, with
  * EGT_OPT_LEARNED, the proposal in place) with neighbour count k and radius thres over all queries in each
  * launch shape: 0 = one lane per query, one workgroup per 256 queries; 1 = one lane per query, one
- * persistent work-queue launch; 2 = one wave per query (not offered with the radius bound). Each shape is
+ * persistent work-queue launch; 2 = one wave per query. Each shape is
  * warmed up on the autotuner's sample, then timed once on the full search; the fastest is kept through
  * EGT_OPT_KNN_PERSISTENT and EGT_OPT_KNN_LANES. Rows do not depend on the choice. *shape (optional) receives
- * it, shape_ms (optional, 3 floats) each shape's time in ms, -1 where not offered. The handle's neighbour rows
+ * it, shape_ms (optional, 3 floats) each shape's time in ms. The handle's neighbour rows
  * are not valid afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms);
 
@@ -326,7 +327,8 @@ egt_status egt_model_train_batch(egt_model m, egt_handle h, const int32_t* k, in
 egt_status egt_model_get_safety(egt_model m, float* safety);
 egt_status egt_model_set_safety(egt_model m, float safety);
 /* Margin (log2 units) of the radius bound: the bound is the predicted radius times 2^margin. Training sets it
- * to the 0.99 quantile of the residuals (at least the level margin). */
+ * to the 0.99 quantile of the residuals (at least the level margin) plus 0.6, so that about 0.05 % of the
+ * queries fall outside their bound and are searched again. */
 egt_status egt_model_get_bound_safety(egt_model m, float* safety);
 egt_status egt_model_set_bound_safety(egt_model m, float safety);
 /* Proposes a start level and a radius bound for every query of h for neighbour count k and switches h to

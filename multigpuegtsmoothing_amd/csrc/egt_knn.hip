@@ -742,7 +742,11 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) 
         const int first_level = level;
         uint32_t fallback_q = 0;
         int plo[3] = {P.Bv, P.Bv, P.Bv}, phi[3] = {-1, -1, -1};
-        double L = key_d(KEY_EMPTY);
+        // Radius bound (as k_knn): every lane starts with the placeholder key at the proposed radius, so only
+        // candidates inside it survive and cells beyond it are pruned; a list whose k-th entry is still the
+        // placeholder holds fewer than k points within the radius and is handed back.
+        double L = key_d(P.qbound ? (((uint64_t)__float_as_uint(fminf(P.qbound[m], thres_n)) << 32) | 0xFFFFFFFFull)
+                                  : KEY_EMPTY);
         for (;;) {
             const int Bl = P.Bv >> level;
             const int cq[3] = {vq[0] >> level, vq[1] >> level, vq[2] >> level};
@@ -906,6 +910,13 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) 
             ++level;
         }
         most = max(most, (uint32_t)min<uint64_t>(ncand - cand0, 0xFFFFFFFFull));
+        if (P.qbound && (uint32_t)key_u(wv::readlane_d(L, kq - 1)) == 0xFFFFFFFFu) {
+            if (lane == 0) {
+                const unsigned long long at = atomicAdd(&P.counters[6], 1ull);
+                if (at < (unsigned long long)P.retry_cap) P.retry[at] = (uint32_t)m;
+            }
+            continue;
+        }
         // replay of the reference schedule on the exact list (lanes below k hold it)
         const uint64_t mykey = key_u(L);
         const bool listed = lane < kq && mykey != KEY_EMPTY;
@@ -1068,14 +1079,12 @@ int class_shape(const egt_ctx* h, int K) {
     return h->knn_lanes == 64 ? 2 : (h->knn_persistent ? 1 : 0);
 }
 
-// One neighbour-count class: the search in the given launch shape (the wave shape is not offered with the
-// radius bound: such a class runs one lane per query), then the queries it handed back searched again in
-// Morton order (the list fills in the order the waves reach their atomics, so it is sorted first): queries
-// with fewer than k points within their radius bound, without the bound, one lane per query; long queries
-// (more candidates than the cap), one wave per query. Both from the handle's start level. Returns how many
-// were handed back.
+// One neighbour-count class: the search in the given launch shape, then the queries it handed back (fewer
+// than k points within their radius bound, or more candidates than the long-query cap) searched again in
+// Morton order (the list fills in the order the waves reach their atomics, so it is sorted first), without
+// the bound, from the handle's start level: one wave per query when the class ran in the wave shape or the
+// cap is on, else one lane per query. Returns how many were handed back.
 int64_t search_class(egt_ctx* h, KnnParams P, int K, hipStream_t s, int shape) {
-    if (shape == 2 && P.qbound) shape = 0;
     if (shape == 1) {  // persistent: the work-queue counter starts at zero
         P.queue = P.counters + 5;
         HIP_CHECK(hipMemsetAsync(P.queue, 0, sizeof(uint64_t), s));
@@ -1113,7 +1122,7 @@ int64_t search_class(egt_ctx* h, KnnParams P, int K, hipStream_t s, int shape) {
     R.qlist = P.retry;
     R.nlist = (int64_t)nb;
     R.stride = 1;
-    if (P.long_cap) {
+    if (P.long_cap || shape == 2) {
         R.nchunks = ceil_div((int64_t)nb, KBLK / 64);
         hipLaunchKernelGGL(k_knn_wave<true>, dim3((unsigned)R.nchunks), dim3(KBLK), 0, s, R);
         LAUNCH_CHECK();
@@ -1137,8 +1146,9 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     P.qbound = (h->learned && (h->learned_mode & 2) && h->qbound.p) ? h->qbound.p : nullptr;
     P.qlist = nullptr;
     P.nlist = 0;
-    // a lane-per-query search hands long queries to a wave-per-query pass
-    P.long_cap = h->knn_lanes == 64 ? 0u : (uint32_t)h->knn_long_cap;
+    // a lane-per-query search hands long queries to a wave-per-query pass (search_class clears the cap for a
+    // class that runs in the wave shape)
+    P.long_cap = (uint32_t)h->knn_long_cap;
     // the hand-back list and, after it, room for the values of its sort
     P.retry = (scratch && (P.qbound || P.long_cap)) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
     P.retry_cap = P.retry ? h->n : 0;
@@ -1440,7 +1450,7 @@ void run_knn_query_noalloc(egt_ctx* h, const float* dqx, const float* dqy, const
 // untried levels).
 // Launch-shape tuner: times the handle's current search (start level, learned proposal) over all queries in
 // each launch shape (one lane per query with one workgroup per 256 queries, the same as one persistent
-// work-queue launch, one wave per query; the wave shape is not offered with the radius bound), each after a
+// work-queue launch, one wave per query), each after a
 // warm-up on the autotuner's sample, and keeps the fastest. A sample's time would mislead here: it is set by
 // its slowest wave, and one query per lane-per-query wave can test tens of thousands of candidates (on a
 // 16M-point cluster mixture the 3% sample took 29 ms in that shape against 3.8 ms with a wave per query, while
@@ -1467,7 +1477,6 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms) {
     float best_ms = 0.0f;
     for (int shape = 0; shape < 3; ++shape) {
         shape_ms[shape] = -1.0f;
-        if (shape == 2 && P.qbound) continue;
         launch(W, shape);
         HIP_CHECK(hipEventRecord(e0, s));
         launch(P, shape);
@@ -1508,7 +1517,6 @@ int64_t autotune_class_shapes(egt_ctx* h, float* ms) {
         int best = 0;
         float best_ms = 0.0f;
         for (int shape = 0; shape < 3; ++shape) {
-            if (shape == 2 && P.qbound) continue;
             search_class(h, P, cr.K, s, shape);
             HIP_CHECK(hipEventRecord(e0, s));
             search_class(h, P, cr.K, s, shape);

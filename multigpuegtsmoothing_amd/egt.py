@@ -302,7 +302,7 @@ class GridStructure(_Provider):
     def autotune_shape(self, k, thres=1e8):
         """Times the current search (start level, learned proposal) in each kNN launch shape over all queries
         and keeps the fastest (no reference counterpart). Returns (shape, ms per shape): 0 = one lane per
-        query, 1 = the same as one persistent launch, 2 = one wave per query; -1 where not offered."""
+        query, 1 = the same as one persistent launch, 2 = one wave per query."""
         shape = ctypes.c_int32()
         ms = np.empty(3, np.float32)
         _check(_lib.egt_autotune_shape(self.handle, k, thres, ctypes.byref(shape), ms))

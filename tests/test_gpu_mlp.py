@@ -106,10 +106,10 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
     rows, stats = {}, {}
     # learned_level: start at the proposed level; learned_bound: the autotuned level, candidates bounded by
     # the proposed radius; learned: both; learned_tight: both with the bound shrunk 8x, so most queries hold
-    # fewer than k points inside it and are searched again without it; learned_wave: the proposed level with
-    # a wave per query
+    # fewer than k points inside it and are searched again without it; learned_wave*: the same with a wave per
+    # query
     modes = ("own_cell", "occupancy", "autotuned", "learned_level", "learned_bound", "learned", "learned_tight",
-             "learned_wave")
+             "learned_wave", "learned_wave_bound", "learned_wave_tight")
     bsafe = model.bound_safety
     for mode in modes:
         g = E.GridStructure(evalc, 1, 1e-6, 1024, 4)
@@ -119,10 +119,10 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
         if mode in ("autotuned", "learned_bound"):
             g.set_option(E.OPT_START_LEVEL, tuned)
         if mode.startswith("learned"):
-            model.bound_safety = bsafe - 3.0 if mode == "learned_tight" else bsafe
+            model.bound_safety = bsafe - 3.0 if mode.endswith("tight") else bsafe
             model.propose(g, k)
             g.set_option(E.OPT_LEARNED_MODE, {"learned_level": 1, "learned_bound": 2, "learned_wave": 1}.get(mode, 3))
-            if mode == "learned_wave":  # a wave per query: each query starts at its own proposed level
+            if mode.startswith("learned_wave"):  # a wave per query: each query starts at its own proposed level
                 g.set_option(E.OPT_KNN_LANES, 64)
             if mode == "learned":
                 lv = E.model_levels(g)
@@ -150,6 +150,8 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
     assert rate < 0.5
     # the shrunk bound hands most queries back, and their second search still gives the exact rows
     assert stats["learned_tight"]["learned_bound_retries"] > n // 2
+    assert stats["learned_wave_tight"]["learned_bound_retries"] > n // 2
+    assert stats["learned_wave_bound"]["knn_insertions"] < stats["learned_wave"]["knn_insertions"]
     assert stats["learned"]["learned_bound_retries"] < 0.25 * n
     # the bound admits only candidates within the proposed radius into the lists
     assert stats["learned_bound"]["knn_insertions"] < stats["autotuned"]["knn_insertions"]

@@ -106,6 +106,24 @@ def test_shape_tuner_after_long_queries(egt_mod, oracle_mod):
     np.testing.assert_array_equal(got, og.to_original(og.knn(40)[0]))
 
 
+def test_long_query_cap_follows_the_class_shape(egt_mod, oracle_mod):
+    """With the wave shape chosen handle-wide and the k <= 40 class forced to one lane per query, that class
+    still hands its long queries on (the cap belongs to the lane shape, whatever the handle-wide choice)."""
+    E, O = egt_mod, oracle_mod
+    pts = _crowded_blob()
+    g = E.GridStructure(pts, 1, 1e-6, 64, 4)
+    g.set_option(E.OPT_KNN_LANES, 64)
+    g.set_option(E.OPT_KNN_CLASS_SHAPES, 1 << (2 * 4))  # class 4 (k <= 40): shape 0 + 1
+    g.GRIDCUDAKNNSELF_COMPACT(40, 1e8)
+    got = g.GetKNNIndexes()
+    st = g.stats()
+    g.close()
+    _log("lane class under a wave handle: handed on", st["knn_long_queries"])
+    assert st["knn_long_queries"] > 0
+    og = O.Grid(*pts, B=64)
+    np.testing.assert_array_equal(got, og.to_original(og.knn(40)[0]))
+
+
 @pytest.mark.parametrize("cap", [0, 1, 200, 2048])
 def test_long_queries_handed_to_waves(egt_mod, oracle_mod, cap):
     """A lane-per-query search hands queries past cap candidates to a wave-per-query pass (cap 1: nearly every
