@@ -94,8 +94,9 @@ def gpu_step(E, torch, dev_xyz, args, start_level):
 
 def learned_vs_plain(E, dev_xyz, args, start_level=None):
     """kNN stage of the same workload with the plain start (the query's own cell, as the reference), with the
-    learned proposal (model trained on the GPU on a torus drawn with a different seed) and, when given, with
-    the autotuned start level. Rows must be identical; the learned time includes the proposal kernel."""
+    learned proposal (model trained on the GPU on a torus drawn with a different seed; level offset, proposal
+    use and launch shape tuned by the library's learned-start tuner) and, when given, with the autotuned start
+    level. Rows must be identical; the learned time includes the proposal kernel."""
     from multigpuegtsmoothing_amd import datasets as D
 
     k = args.k
@@ -107,16 +108,25 @@ def learned_vs_plain(E, dev_xyz, args, start_level=None):
     train.close()
     res = {}
     rows = {}
+    # the learned start's level offset, proposal use and launch shape, tuned once by the library
+    g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
+    model.propose(g, k)
+    level_offset, learned_mode, lshape, _ = g.autotune_learned(k)
+    g.close()
     modes = ("plain", "learned") + (("autotuned",) if start_level is not None else ())
     for mode in modes:
         g = E.GridStructure(dev_xyz, 1, 1e-6, 1024, 4)
         if mode == "autotuned":
             g.set_option(E.OPT_START_LEVEL, start_level)
+        if mode == "learned":
+            g.set_option(E.OPT_KNN_LANES, 64 if lshape == 2 else 1)
         best = None
         for _ in range(3):
             prop = 0.0
             if mode == "learned":
                 model.propose(g, k)
+                g.set_option(E.OPT_LEARNED_MODE, learned_mode)
+                g.set_option(E.OPT_LEARNED_LEVEL_OFFSET, level_offset)
                 prop = g.stats()["propose_ms"]
             g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
             st = g.stats()
@@ -128,6 +138,8 @@ def learned_vs_plain(E, dev_xyz, args, start_level=None):
         rows[mode] = g.GetKNNIndexes()
         g.close()
     model.close()
+    res["learned"].update(level_offset=level_offset, learned_mode=learned_mode,
+                          shape="wave per query" if lshape == 2 else "lane per query")
     out = {"plain": res["plain"], "learned": res["learned"],
            "knn_speedup": res["plain"]["knn_ms"] / res["learned"]["knn_ms"],
            "bit_exact": bool(np.array_equal(rows["plain"], rows["learned"])),
@@ -401,13 +413,16 @@ def batch_result(args, steps, warmup, per_cloud=True):
     b = E.BatchGrid(sizes, device=dev)
     model.propose(b, ks)
     tried = {}
-    for lmode in (1, 3):
-        b.set_option(E.OPT_LEARNED_MODE, lmode)
-        packed, ms = b.autotune_shape(ks)
-        tried[lmode] = (float(sum(ms[ci][ms[ci] >= 0].min() for ci in range(8) if (ms[ci] >= 0).any())), packed, ms)
+    for off in (0, 1):
+        for lmode in (1, 3):
+            b.set_option(E.OPT_LEARNED_MODE, lmode)
+            b.set_option(E.OPT_LEARNED_LEVEL_OFFSET, off)
+            packed, ms = b.autotune_shape(ks)
+            tried[(off, lmode)] = (float(sum(ms[ci][ms[ci] >= 0].min() for ci in range(8) if (ms[ci] >= 0).any())),
+                                  packed, ms)
     b.close()
-    learned_mode = min(tried, key=lambda m: tried[m][0])
-    shapes[True] = tried[learned_mode][1:]
+    level_offset, learned_mode = min(tried, key=lambda m: tried[m][0])
+    shapes[True] = tried[(level_offset, learned_mode)][1:]
     b = E.BatchGrid(sizes, device=dev)
     shapes[False] = b.autotune_shape(ks)
     b.close()
@@ -419,6 +434,7 @@ def batch_result(args, steps, warmup, per_cloud=True):
         if learned:
             model.propose(b, ks)
             b.set_option(E.OPT_LEARNED_MODE, learned_mode)
+            b.set_option(E.OPT_LEARNED_LEVEL_OFFSET, level_offset)
         b.knn(ks)
         E.EGT_CUDA(b).PerformSmoothing(iters, 0.5, -0.53, 0.65)
         st = b.stats()
@@ -449,6 +465,7 @@ def batch_result(args, steps, warmup, per_cloud=True):
            "learned_end_to_end_speedup": plain_dt / dt,
            "train": {"clouds": 64, "seconds": train_s, "mse": mse},
            "learned_mode": {1: "proposed level", 3: "proposed level and radius bound"}[learned_mode],
+           "learned_level_offset": level_offset,
            "knn_class_shapes": {path: {f"k<={8 * (ci + 1)}": {"shape": ("lane per query", "persistent", "wave per query")[
                ((packed >> (2 * ci)) & 3) - 1], "ms": [round(float(x), 3) for x in ms[ci]]}
                for ci in range(8) if (packed >> (2 * ci)) & 3}
@@ -583,8 +600,8 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
     iterations. Paths (each warm-up + `steps` timed steps of grid + (proposal) + kNN + EGG + Taubin):
     plain_own_cell (every query from its own cell, the reference schedule, one lane per query), plain_autotuned
     (the level the autotuner picks for this cloud and the launch shape the shape tuner picks for it, tuned once
-    outside the timing) and learned (the model's start level per query, alone or with its radius bound, and the
-    launch shape, whichever pair the shape tuner times fastest;
+    outside the timing) and learned (the model's start level per query, offset by 0 or +1, alone or with its
+    radius bound, in the launch shape: whichever the library's learned-start tuner times fastest;
     model trained on the GPU, outside the timing, on the plain search's radii over 2M mixture points of seed
     1000). Rows of the three are compared on the GPU after the timing."""
     import torch
@@ -613,16 +630,12 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
     g.set_option(E.OPT_START_LEVEL, lvl)
     shapes = {"plain_own_cell": (0, None), "plain_autotuned": g.autotune_shape(k)}
     g.close()
-    # the learned path: the proposal's level alone or with its radius bound, each in its fastest shape
+    # the learned path: level offset, the proposal's level alone or with its radius bound, and the shape,
+    # tuned together by the library's learned-start tuner
     g = E.GridStructure(dev, 1, 1e-6, 1024, 4)
     model.propose(g, k)
-    tried = {}
-    for lmode in (1, 3):
-        g.set_option(E.OPT_LEARNED_MODE, lmode)
-        shape, ms = g.autotune_shape(k)
-        tried[lmode] = (float(ms[shape]), shape, ms)
-    learned_mode = min(tried, key=lambda m: tried[m][0])
-    shapes["learned"] = tried[learned_mode][1:]
+    level_offset, learned_mode, lshape, lms = g.autotune_learned(k)
+    shapes["learned"] = (lshape, None)
     tune_s = time.perf_counter() - t0
     g.close()
     holder = {}
@@ -637,6 +650,7 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
             if mode == "learned":
                 model.propose(g, k)
                 g.set_option(E.OPT_LEARNED_MODE, learned_mode)
+                g.set_option(E.OPT_LEARNED_LEVEL_OFFSET, level_offset)
             elif mode == "plain_autotuned":
                 g.set_option(E.OPT_START_LEVEL, lvl)
             g.set_option(E.OPT_KNN_PERSISTENT, 1 if shape == 1 else 0)
@@ -674,7 +688,10 @@ def mixture_modes(args, k=40, steps=5, warmup=2):
             per[mode]["fallback_rate"] = st["learned_fallbacks"] / n
             per[mode]["bound_retry_rate"] = st["learned_bound_retries"] / n
             per[mode]["learned_mode"] = {1: "proposed level", 3: "proposed level and radius bound"}[learned_mode]
-            per[mode]["tuned_ms_by_learned_mode"] = {str(m): [round(float(x), 3) for x in v[2]] for m, v in tried.items()}
+            per[mode]["learned_level_offset"] = level_offset
+            per[mode]["learned_tuner_ms"] = {f"offset {o}, {u}, {sh}": round(float(lms[o, mi, si]), 3)
+                                             for o in range(2) for mi, u in enumerate(("level", "level+bound"))
+                                             for si, sh in enumerate(("lane", "wave"))}
     first = next(iter(rows))
     equal = bool(all(torch.equal(rows[first], rows[m]) for m in rows))
     del rows

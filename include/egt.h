@@ -114,11 +114,14 @@ typedef enum {
     EGT_OPT_KNN_LONG_QUERY = 9,  /* kNN self search with one lane per query: a query that has tested more
                                     candidates than this is handed on to a wave-per-query pass (its wave no
                                     longer waits for it); 0 = never; default 2048. Rows do not depend on it */
-    EGT_OPT_KNN_CLASS_SHAPES = 10 /* kNN self search: launch shape per neighbour-count class (k <= 8, 16, ..,
+    EGT_OPT_KNN_CLASS_SHAPES = 10, /* kNN self search: launch shape per neighbour-count class (k <= 8, 16, ..,
                                     64 are classes 0..7): bits 2i..2i+1 hold class i's shape + 1 (1 = one lane
                                     per query, 2 = the same as one persistent launch, 3 = one wave per query,
                                     0 = the handle-wide EGT_OPT_KNN_LANES / EGT_OPT_KNN_PERSISTENT; default 0).
                                     egt_batch_autotune_shape returns such a value. Rows do not depend on it */
+    EGT_OPT_LEARNED_LEVEL_OFFSET = 11 /* kNN with the learned proposal: added to every proposed start level,
+                                    -3..3 (default 0; +1 = one level coarser than proposed).
+                                    egt_autotune_learned picks it. Rows do not depend on it */
 } egt_option;
 
 const char* egt_last_error_string(void);
@@ -248,6 +251,17 @@ egt_status egt_autotune(egt_handle h, int k, float thres, int32_t* start_level, 
  * it, shape_ms (optional, 3 floats) each shape's time in ms. The handle's neighbour rows
  * are not valid afterwards: run egt_grid_knn_self. */
 egt_status egt_autotune_shape(egt_handle h, int k, float thres, int32_t* shape, float* shape_ms);
+
+/* Learned-start tuner (no reference counterpart): with a proposal in place (egt_model_propose), times the full
+ * search with neighbour count k and radius thres for each level offset (0, +1), each use of the proposal
+ * (EGT_OPT_LEARNED_MODE 1 = its level, 3 = its level and radius bound) and each of the lane-per-query and
+ * wave-per-query shapes, each after a warm-up on the autotuner's sample, and keeps the fastest through
+ * EGT_OPT_LEARNED_LEVEL_OFFSET, EGT_OPT_LEARNED_MODE and EGT_OPT_KNN_LANES. Rows do not depend on the choice.
+ * offset, mode, shape (optional) receive it (shape 0 = lane per query, 2 = wave per query); ms (optional, 8
+ * floats) the times, index (offset x 2 + (mode == 3)) x 2 + (shape == 2). One-cloud handles. The handle's
+ * neighbour rows are not valid afterwards: run egt_grid_knn_self. */
+egt_status egt_autotune_learned(egt_handle h, int k, float thres, int32_t* offset, int32_t* mode, int32_t* shape,
+                                float* ms);
 
 /* Per-class launch-shape tuner for a batch (no reference counterpart): with neighbour counts k[c] per cloud
  * and radius thres (as egt_batch_knn) and the handle's start level or learned proposal in place, times each

@@ -365,6 +365,10 @@ egt_status egt_set_option(egt_handle h, egt_option opt, int64_t value) {
             if (value < 0 || value > UINT32_MAX) return fail(EGT_ERR_INVALID_ARG, "long-query cap must be in [0, 2^32)");
             h->knn_long_cap = value;
             return EGT_OK;
+        case EGT_OPT_LEARNED_LEVEL_OFFSET:
+            if (value < -3 || value > 3) return fail(EGT_ERR_INVALID_ARG, "learned level offset must be in [-3, 3]");
+            h->learned_level_offset = (int)value;
+            return EGT_OK;
         case EGT_OPT_KNN_CLASS_SHAPES:
             if (value < 0 || value >= (1 << 16)) return fail(EGT_ERR_INVALID_ARG, "class shapes: 8 fields of 2 bits");
             for (int c = 0; c < 8; ++c) h->class_shape[c] = (int)((value >> (2 * c)) & 3) - 1;
@@ -492,6 +496,30 @@ egt_status egt_batch_knn(egt_handle h, const int32_t* k, float thres) {
         h->k = kmax;
         h->csr_valid = false;
         h->smoothed = false;
+    });
+}
+
+egt_status egt_autotune_learned(egt_handle h, int k, float thres, int32_t* offset, int32_t* mode, int32_t* shape,
+                                float* ms) {
+    if (need(h)) return EGT_ERR_INVALID_ARG;
+    if (h->kind != 0) return fail(EGT_ERR_STATE, "the learned-start tuner needs a one-cloud point-cloud handle");
+    if (!h->learned || !h->qlevel.p) return fail(EGT_ERR_STATE, "no learned proposal in place (egt_model_propose)");
+    if (k < 1) return fail(EGT_ERR_INVALID_ARG, "k must be >= 1");
+    if (k > 64) return fail(EGT_ERR_UNSUPPORTED, "k > 64 is not supported");
+    if (!(thres > 0.0f)) return fail(EGT_ERR_INVALID_ARG, "thres must be > 0");
+    return guarded([&] {
+        float t = thres / (h->max_extent - h->min_extent);
+        t *= t;
+        if (h->search_stale) egt::refresh_search_structure(h);
+        float m8[8];
+        int o = 0, md = 1, sh = 0;
+        egt::autotune_learned(h, k, t, &o, &md, &sh, m8);
+        h->csr_valid = false;
+        if (offset) *offset = o;
+        if (mode) *mode = md;
+        if (shape) *shape = sh;
+        if (ms)
+            for (int i = 0; i < 8; ++i) ms[i] = m8[i];
     });
 }
 

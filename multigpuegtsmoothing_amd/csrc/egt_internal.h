@@ -230,6 +230,7 @@ struct egt_ctx {
                            // radius, 3 = both
     int knn_persistent = 0;  // kNN as one persistent work-queue launch per neighbour-count class
     int knn_lanes = 1;       // kNN self search: 1 = one lane per query, 64 = one wave per query
+    int learned_level_offset = 0;  // added to the proposed start levels
     int class_shape[8] = {-1, -1, -1, -1, -1, -1, -1, -1};  // per neighbour-count class (k <= 8, 16, .., 64):
                                                            // 0 lane, 1 persistent, 2 wave; -1 handle-wide
     int64_t knn_long_cap = 2048;  // lane-per-query search: candidates after which a query goes to a wave
@@ -294,6 +295,9 @@ int autotune_shape(egt_ctx* h, int k, float thres_n, float* shape_ms);
 // The same per neighbour-count class of the handle's clouds (k and radius already set): ms is 8 x 3, the
 // return value the choices packed as EGT_OPT_KNN_CLASS_SHAPES takes them.
 int64_t autotune_class_shapes(egt_ctx* h, float* ms);
+// With a proposal in place: level offset (0, 1) x proposal use (1, 3) x shape (lane, wave), full searches timed;
+// keeps the fastest in the handle's options. ms: 8 times, index (offset * 2 + mode index) * 2 + shape index.
+void autotune_learned(egt_ctx* h, int k, float thres_n, int* best_off, int* best_mode, int* best_shape, float* ms);
 
 // learned proposal (egt_mlp.hip)
 int mlp_param_count();

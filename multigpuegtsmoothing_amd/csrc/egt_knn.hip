@@ -86,6 +86,7 @@ struct KnnParams {
     int Bv;            // cells per axis at internal level 0 (B << SUB)
     int nlev;          // internal levels: SUB sub-cell levels + the reference pyramid
     int start_level;   // internal level of the first box (SUB = the query's own fine cell)
+    int level_offset;  // added to every proposed start level
     uint32_t long_cap; // lane-per-query search: a query that has tested more candidates than this is handed
                        // back to a wave-per-query pass (0 = never)
     uint32_t bigmask;  // levels holding a cell with >= maxc points (any cloud)
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(KBLK)
     int own_level = P.start_level;
     int wave_level = own_level;
     if (P.qlevel) {
-        own_level = active ? (int)P.qlevel[m] + SUB : 127;
+        own_level = active ? (int)P.qlevel[m] + SUB + P.level_offset : 127;
         wave_level = own_level;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) wave_level = min(wave_level, __shfl_xor(wave_level, d, 64));
@@ -735,7 +736,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) 
         const float qv[3] = {q.x, q.y, q.z};
         int own_level = P.start_level;
         if (P.qlevel) {
-            own_level = (int)P.qlevel[m] + SUB;
+            own_level = (int)P.qlevel[m] + SUB + P.level_offset;
             ++nlearned;
         }
         int level = own_level < 0 ? 0 : (own_level >= P.nlev ? P.nlev - 1 : own_level);
@@ -1149,6 +1150,7 @@ KnnParams knn_params(egt_ctx* h, bool scratch = true) {
     // a lane-per-query search hands long queries to a wave-per-query pass (search_class clears the cap for a
     // class that runs in the wave shape)
     P.long_cap = (uint32_t)h->knn_long_cap;
+    P.level_offset = h->learned_level_offset;
     // the hand-back list and, after it, room for the values of its sort
     P.retry = (scratch && (P.qbound || P.long_cap)) ? h->scratch.get<uint32_t>(37, 2 * (size_t)h->n) : nullptr;
     P.retry_cap = P.retry ? h->n : 0;
@@ -1535,6 +1537,62 @@ int64_t autotune_class_shapes(egt_ctx* h, float* ms) {
     }
     h->have_knn = false;  // as after the other tuners: the caller runs the search for its rows and statistics
     return packed;
+}
+
+// Learned-start tuner: with a proposal in place, times the full search for each level offset (0, +1: the
+// proposed levels one level coarser), each use of the proposal (its level alone, or with its radius bound) and
+// each of the lane-per-query and wave-per-query shapes, each after a warm-up on the autotuner's sample, and
+// keeps the fastest. ms: [offset][mode][shape] = 2 x 2 x 2. Measured on a 4M-point torus: the offset +1 takes
+// the learned search from 11.2 to 9.1 ms (the margin the model adds for its level is tighter than this cloud
+// needs); on a 16M-point cluster mixture from 54.1 to 52.3 ms with the bound.
+void autotune_learned(egt_ctx* h, int k, float thres_n, int* best_off, int* best_mode, int* best_shape, float* ms) {
+    hipStream_t s = h->stream;
+    h->clouds[0].k = k;
+    h->clouds[0].thres_n = thres_n;
+    prepare_clouds(h);
+    const int K = kclass(k);
+    const int64_t total = (h->n + 63) / 64;
+    Event e0, e1;
+    float best_ms = 0.0f;
+    *best_off = 0;
+    *best_mode = 1;
+    *best_shape = 0;
+    for (int off = 0; off < 2; ++off) {
+        for (int mi = 0; mi < 2; ++mi) {
+            h->learned_level_offset = off;
+            h->learned_mode = mi == 0 ? 1 : 3;
+            KnnParams P = knn_params(h);
+            HIP_CHECK(hipMemsetAsync(P.counters, 0, 10 * sizeof(uint64_t), s));
+            KnnParams W = P;
+            W.stride = (int)std::max<int64_t>(1, total / 8192);
+            W.nchunks = (total + W.stride - 1) / W.stride;
+            P.stride = 1;
+            P.nchunks = total;
+            for (int si = 0; si < 2; ++si) {
+                const int shape = si == 0 ? 0 : 2;
+                search_class(h, W, K, s, shape);
+                HIP_CHECK(hipEventRecord(e0, s));
+                search_class(h, P, K, s, shape);
+                HIP_CHECK(hipEventRecord(e1, s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                float t = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+                ms[(off * 2 + mi) * 2 + si] = t;
+                if ((off | mi | si) == 0 || t < best_ms) {
+                    best_ms = t;
+                    *best_off = off;
+                    *best_mode = h->learned_mode;
+                    *best_shape = shape;
+                }
+            }
+        }
+    }
+    h->learned_level_offset = *best_off;
+    h->learned_mode = *best_mode;
+    h->knn_lanes = *best_shape == 2 ? 64 : 1;
+    h->knn_persistent = 0;
+    h->class_shape[K / 8 - 1] = -1;
+    h->have_knn = false;  // as after the other tuners: the caller runs the search for its rows and statistics
 }
 
 int autotune_start_level(egt_ctx* h, int k, float thres_n, float* trial_ms, float* trial_cost) {

@@ -30,6 +30,7 @@ OPT_LEARNED_MODE = 7         # learned proposal: 1 = start level (default), 2 = 
 OPT_KNN_LANES = 8            # kNN self search: 1 = one lane per query (default), 64 = one wave per query
 OPT_KNN_LONG_QUERY = 9       # lane-per-query search: candidates after which a query goes to a wave (2048; 0 = never)
 OPT_KNN_CLASS_SHAPES = 10    # per neighbour-count class launch shapes, as BatchGrid.autotune_shape returns them
+OPT_LEARNED_LEVEL_OFFSET = 11  # added to the proposed start levels (-3..3)
 START_LEVEL_OCCUPANCY = 127  # OPT_START_LEVEL value: one level from the mean cell occupancy
 SUB_LEVELS = 3               # start levels -1, -2, -3: boxes of cells 1/2, 1/4, 1/8 of a grid cell
 AUTOTUNE_SLOTS = 14          # autotuner trial arrays: levels -3 .. 10
@@ -123,6 +124,8 @@ def _load():
         "egt_autotune": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N, _F32N], st),
         "egt_autotune_shape": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), _F32N], st),
         "egt_batch_autotune_shape": ([_H, _I32P, f32, ctypes.POINTER(i64), _F32N], st),
+        "egt_autotune_learned": ([_H, i32, f32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(ctypes.c_int32), _F32N], st),
         "egt_batch_create": ([_F32P, _F32P, _F32P, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"),
                               ctypes.c_int32, f32, i32, _PH], st),
         "egt_batch_create_device": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -307,6 +310,16 @@ class GridStructure(_Provider):
         ms = np.empty(3, np.float32)
         _check(_lib.egt_autotune_shape(self.handle, k, thres, ctypes.byref(shape), ms))
         return shape.value, ms
+
+    def autotune_learned(self, k, thres=1e8):
+        """With a proposal in place: times level offsets (0, +1) x proposal use (level; level and radius
+        bound) x shapes (lane, wave per query) over the full search and keeps the fastest (no reference
+        counterpart). Returns (offset, learned mode, shape, ms array [offset, mode 1/3, lane/wave])."""
+        off, mode, shape = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        ms = np.empty(8, np.float32)
+        _check(_lib.egt_autotune_learned(self.handle, k, thres, ctypes.byref(off), ctypes.byref(mode),
+                                         ctypes.byref(shape), ms))
+        return off.value, mode.value, shape.value, ms.reshape(2, 2, 2)
 
     def GRIDCUDAKNNSELF_COMPACT(self, k, thres=1e8):
         _check(_lib.egt_grid_knn_self(self.handle, int(k), float(thres)))

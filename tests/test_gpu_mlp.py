@@ -164,6 +164,24 @@ def test_learned_start_is_bit_exact_and_reports_fallbacks(egt_mod, oracle_mod):
 
     assert work(stats["learned_level"]) < work(stats["autotuned"])
     assert stats["learned"]["knn_candidates"] < 0.5 * stats["occupancy"]["knn_candidates"]
+    # the learned-start tuner (level offset x proposal use x shape) and level offsets set by hand
+    g = E.GridStructure(evalc, 1, 1e-6, 1024, 4)
+    model.propose(g, k)
+    off, lmode, shape, ms = g.autotune_learned(k)
+    _log("  learned tuner: offset", off, "mode", lmode, "shape", shape, "ms", np.round(ms, 3).tolist())
+    assert off in (0, 1) and lmode in (1, 3) and shape in (0, 2)
+    assert ms[off, lmode // 2, shape // 2] == ms.min()
+    g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), want, err_msg="tuned learned start")
+    for o in (-2, 2):
+        for lanes in (1, 64):
+            g.set_option(E.OPT_LEARNED_LEVEL_OFFSET, o)
+            g.set_option(E.OPT_KNN_LANES, lanes)
+            g.GRIDCUDAKNNSELF_COMPACT(k, 1e8)
+            np.testing.assert_array_equal(g.GetKNNIndexes(), want, err_msg=f"offset {o} lanes {lanes}")
+    with pytest.raises(E.EGTError):
+        g.set_option(E.OPT_LEARNED_LEVEL_OFFSET, 4)
+    g.close()
     model.close()
 
 
