@@ -17,6 +17,11 @@
 // replayed on that exact list, stage by stage (own cell, then ring 1 at bucket counts B, B/2, ..., 8), to
 // decide whether the reference would have blanked the row (cell with >= maxc points, or still searching
 // after the bucket-count-8 ring).
+//
+// Two launch shapes run this search: one lane per query (k_knn, the list in the lane's registers) and one
+// wave per query (k_knn_wave, the list spread over the lanes, one key per lane), chosen per neighbour-count
+// class by the options or the tuners below. A lane-per-query search hands queries that exceed a candidate
+// cap, and queries with fewer than k points inside a learned radius bound, to a wave-per-query pass.
 #include <algorithm>
 #include <mutex>
 #include "egt_device.h"
