@@ -3,8 +3,8 @@
 // Reference: taubin_step_GKNN EGT_CUDA.cu:229-276, iteration loop :545-634 (one iteration = a lambda pass
 // then a mu pass; weights w = exp(-|pn - pi|^2), isRegularized = 0 :544).
 //
-// Points live in Morton order as float4 (x, y, z, pad) in two buffers, so a wave's 64 points are spatial
-// neighbours and their neighbour gathers (one 16-byte load each) mostly hit the same cache lines. Blocks
+// Points live in Morton order as packed x, y, z (12 bytes) in two buffers, so a wave's 64 points are spatial
+// neighbours and their gathers mostly hit the same cache lines; a tile stages them in LDS as float4. Blocks
 // are remapped so each of the 8 XCDs walks chunks of consecutive Morton tiles (workgroups are dealt to XCDs
 // round robin), which keeps a block's neighbours in its own XCD's L2. The 2*iterations passes are recorded
 // into hipGraphs and replayed. The exp is egt_expf, so results match the host oracle bit for bit; the
@@ -23,6 +23,16 @@ constexpr int NXCD = 8;
 // per tile) is spread over all XCDs instead of holding back the one whose contiguous range it falls in.
 // The grid is padded to a multiple of NXCD * XCHUNK blocks, over which the map is a bijection.
 constexpr int XCHUNK = 16;
+// The passes' ping-pong points: x, y, z packed in 12 bytes (one dwordx3 access), a quarter less traffic than
+// float4 in the staging loads, the halo gathers and the stores; LDS keeps float4 for aligned 16-byte reads.
+struct __attribute__((aligned(4))) Pt3 {
+    float x, y, z;
+};
+__device__ inline float4 ld3(const Pt3* __restrict__ a, int64_t i) {
+    const Pt3 v = a[i];
+    return make_float4(v.x, v.y, v.z, 0.0f);
+}
+__device__ inline void st3(Pt3* __restrict__ a, int64_t i, const float4 v) { a[i] = Pt3{v.x, v.y, v.z}; }
 __device__ inline int64_t xcd_tile() {
     const int b = blockIdx.x;
     const int x = b % NXCD, j = b / NXCD;
@@ -287,7 +297,7 @@ __device__ inline void taubin_fetch(int g, int C0, int W0, int W1, const uint32_
 template <bool REG, bool OVF>
 __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uint32_t* sb0, const uint32_t* sb1,
                                        const uint32_t* dummy, int lane, int W0, int W1, int r0, int r1, float scale,
-                                       const float4* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
+                                       const Pt3* __restrict__ gin, const int32_t* __restrict__ hal, uint32_t lim,
                                        uint32_t tile, uint32_t (&ba)[TCH / 2], float4& res0, float4& res1) {
     constexpr int CH = TCH, CP = CH / 2;
     const int C0 = (W0 + CH - 1) / CH, C = C0 + (W1 + CH - 1) / CH;  // chunks of each row (wave-uniform)
@@ -313,7 +323,7 @@ __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uin
         for (int u = 0; u < CH; ++u) {
             if (u0 + u < W) {  // wave-uniform
                 if (!OVF || c[u] < lim) qs[u] = src[c[u]];
-                else qs[u] = gin[hal[c[u] - tile]];
+                else qs[u] = ld3(gin, hal[c[u] - tile]);
             }
         }
 #pragma unroll
@@ -351,7 +361,7 @@ __device__ inline void taubin_two_rows(const float4* __restrict__ src, const uin
 // slices in ascending width), so the waves of a block carry similar work; lane l of slice s handles the
 // row perm[64 s + l]. Results go back through LDS so the global writes are coalesced.
 template <int TILE, bool REG>
-__global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const float4* __restrict__ in, float4* __restrict__ out,
+__global__ void __launch_bounds__(tile_threads<TILE>()) __attribute__((amdgpu_waves_per_eu(8))) k_taubin_tile(const Pt3* __restrict__ in, Pt3* __restrict__ out,
                                                        const int32_t* __restrict__ rowptr,
                                                        const int32_t* __restrict__ col,
                                                        const uint16_t* __restrict__ scol,
@@ -374,10 +384,10 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     if (H < 0) {
         // halo larger than the tile, or a row that lists itself: CSR rows with global gathers
         for (int64_t i = base + threadIdx.x; i < end; i += TBLK_TILE) {
-            const float4 p = in[i];
+            const float4 p = ld3(in, i);
             float cgx = 0.0f, cgy = 0.0f, cgz = 0.0f, sum = 0.0f;
             for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-                const float4 q = in[col[e]];
+                const float4 q = ld3(in, col[e]);
                 const float dist = sqd(p.x, p.y, p.z, q.x, q.y, q.z);
                 const float w = taubin_weight<REG>(dist, scale);
                 cgx = __builtin_fmaf(w, q.x - p.x, cgx);
@@ -385,7 +395,7 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
                 cgz = __builtin_fmaf(w, q.z - p.z, cgz);
                 sum += w;
             }
-            out[i] = taubin_result(p, cgx, cgy, cgz, sum, scale);
+            st3(out, i, taubin_result(p, cgx, cgy, cgz, sum, scale));
         }
         return;
     }
@@ -425,22 +435,23 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     int32_t hidx[HP];
 #pragma unroll
     for (int k = 0; k < HP; ++k) hidx[k] = hal[min((int)threadIdx.x + k * TBLK_TILE, max(HL - 1, 0))];
-    float4 tv[TP];
+    Pt3 tv[TP];
 #pragma unroll
     for (int k = 0; k < TP; ++k) tv[k] = in[base + min((int)threadIdx.x + k * TBLK_TILE, np - 1)];
     // the first 64 words of the column array (it is allocated with at least that many)
     const uint32_t* dummy = reinterpret_cast<const uint32_t*>(scol);
     uint32_t ba[TCH / 2];
     taubin_fetch(0, (W[0] + TCH - 1) / TCH, W[0], W[1], sb[0], sb[1], dummy, lane, ba);
-    float4 hv[HP];
+    Pt3 hv[HP];
 #pragma unroll
     for (int k = 0; k < HP; ++k) hv[k] = in[(int)threadIdx.x + k * TBLK_TILE < HL ? (int64_t)hidx[k] : base];
+    // x, y, z into the float4 entries (their w is never read)
 #pragma unroll
-    for (int k = 0; k < TP; ++k) lds[threadIdx.x + k * TBLK_TILE] = tv[k];
+    for (int k = 0; k < TP; ++k) *reinterpret_cast<Pt3*>(&lds[threadIdx.x + k * TBLK_TILE]) = tv[k];
 #pragma unroll
     for (int k = 0; k < HP; ++k) {
         const int h = (int)threadIdx.x + k * TBLK_TILE;
-        lds[TILE + (h < HL ? h : HL)] = hv[k];
+        *reinterpret_cast<Pt3*>(&lds[TILE + (h < HL ? h : HL)]) = hv[k];
     }
     __syncthreads();
     float4 res[2];
@@ -455,21 +466,28 @@ __global__ void __launch_bounds__(tile_threads<TILE>()) k_taubin_tile(const floa
     for (int r = 0; r < 2; ++r)
         if (row[r] >= 0) lds[row[r]] = res[r];
     __syncthreads();
-    for (int i = threadIdx.x; i < np; i += TBLK_TILE) out[base + i] = lds[i];
+    for (int i = threadIdx.x; i < np; i += TBLK_TILE) out[base + i] = *reinterpret_cast<const Pt3*>(&lds[i]);
 }
 
 // Writes the smoothed points back to the sorted-space SoA arrays and refreshes the Morton copy.
-__global__ void __launch_bounds__(TBLK) k_writeback(const float4* __restrict__ p, int64_t n,
+__global__ void __launch_bounds__(TBLK) k_writeback(const Pt3* __restrict__ p, int64_t n,
                                                      float4* __restrict__ mpts, float* __restrict__ x,
                                                      float* __restrict__ y, float* __restrict__ z) {
     const int64_t m = (int64_t)blockIdx.x * TBLK + threadIdx.x;
     if (m >= n) return;
-    const float4 v = p[m];
+    const float4 v = ld3(p, m);
     const int32_t s = __float_as_int(mpts[m].w);
     mpts[m] = make_float4(v.x, v.y, v.z, __int_as_float(s));
     x[s] = v.x;
     y[s] = v.y;
     z[s] = v.z;
+}
+
+// The Morton-order points (float4, w = sorted index) as packed x, y, z for the passes.
+__global__ void __launch_bounds__(TBLK) k_pack3(const float4* __restrict__ mpts, int64_t n, Pt3* __restrict__ p) {
+    const int64_t m = (int64_t)blockIdx.x * TBLK + threadIdx.x;
+    if (m >= n) return;
+    st3(p, m, mpts[m]);
 }
 
 }  // namespace
@@ -532,7 +550,7 @@ TilePlan prepare_tiles(egt_ctx* h) {
 }
 
 template <int TILE, bool REG>
-void launch_tile(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
+void launch_tile(egt_ctx* h, const TilePlan& tp, const Pt3* in, Pt3* out, float scale) {
     const size_t lds_bytes = (size_t)(tp.tile + tp.maxH + 1) * sizeof(float4);  // + the halo's spill entry
     const dim3 tgrid(ceil_div(tp.ntiles, NXCD * XCHUNK) * NXCD * XCHUNK), tblk(tile_threads<TILE>());
     k_taubin_tile<TILE, REG><<<tgrid, tblk, lds_bytes, h->stream>>>(in, out, h->rowptr.p, h->col.p, tp.scol, tp.toff,
@@ -540,7 +558,7 @@ void launch_tile(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, 
                                                                     tp.ntiles);
 }
 
-void launch_pass(egt_ctx* h, const TilePlan& tp, const float4* in, float4* out, float scale) {
+void launch_pass(egt_ctx* h, const TilePlan& tp, const Pt3* in, Pt3* out, float scale) {
     const bool reg = h->taubin_regularized != 0;
     if (tp.tile == 2048)
         reg ? launch_tile<2048, true>(h, tp, in, out, scale) : launch_tile<2048, false>(h, tp, in, out, scale);
@@ -552,17 +570,17 @@ TilePlan prepare(egt_ctx* h, int tile) { return tile == 2048 ? prepare_tiles<204
 
 // Tile-width autotuner: prepares each candidate width and times three lambda/mu pass pairs on copies of
 // the points (the smoothing itself is untouched), keeping the faster width for this handle.
-int tune_tile(egt_ctx* h, const float4* A, float lambda, float mu) {
+int tune_tile(egt_ctx* h, const Pt3* A, float lambda, float mu) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
-    float4* C = h->scratch.get<float4>(22, n);
-    float4* D = h->scratch.get<float4>(23, n);
+    Pt3* C = h->scratch.get<Pt3>(22, n);
+    Pt3* D = h->scratch.get<Pt3>(23, n);
     Event e0, e1;
     int best = 2048;
     float best_ms = 0.0f;
     for (int tile : {1024, 2048}) {
         const TilePlan tp = prepare(h, tile);
-        HIP_CHECK(hipMemcpyAsync(C, A, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(C, A, n * sizeof(Pt3), hipMemcpyDeviceToDevice, s));
         launch_pass(h, tp, C, D, lambda);  // warm-up
         HIP_CHECK(hipEventRecord(e0, s));
         for (int r = 0; r < 3; ++r) {
@@ -588,9 +606,12 @@ int tune_tile(egt_ctx* h, const float4* A, float lambda, float mu) {
 void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     const int64_t n = h->n;
     hipStream_t s = h->stream;
-    float4* A = h->scratch.get<float4>(12, n);
-    float4* Bf = h->scratch.get<float4>(13, n);
-    HIP_CHECK(hipMemcpyAsync(A, h->mpts.p, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    Pt3* A = h->scratch.get<Pt3>(12, n);
+    Pt3* Bf = h->scratch.get<Pt3>(13, n);
+    if (n > 0) {
+        k_pack3<<<ceil_div(n, TBLK), TBLK, 0, s>>>(h->mpts.p, n, A);
+        LAUNCH_CHECK();
+    }
     if (h->taubin_tile == 0 && iters > 0) h->taubin_tile = tune_tile(h, A, lambda, mu);
     const int tile = h->taubin_tile == 1024 ? 1024 : 2048;
     Event e0, e1, ep;
@@ -600,7 +621,7 @@ void run_taubin(egt_ctx* h, int iters, float lambda, float mu) {
     h->stats.taubin_tile = tile;
     h->stats.taubin_fallback_tiles = (int32_t)tp.fallback_tiles;
     h->stats.taubin_overflow_tiles = (int32_t)tp.overflow_tiles;
-    auto pass = [&](const float4* in, float4* out, float scale) { launch_pass(h, tp, in, out, scale); };
+    auto pass = [&](const Pt3* in, Pt3* out, float scale) { launch_pass(h, tp, in, out, scale); };
     if (iters > 0) {
         if (h->taubin_graph) {
             // A graph of up to GRAPH_ITERS iterations (2 GRAPH_ITERS passes; each iteration ends back in A) is
