@@ -52,7 +52,9 @@ __device__ inline uint64_t key_u(double d) { return __builtin_bit_cast(uint64_t,
 // Per-lane buffer of keys awaiting insertion (4) and candidates fetched per step (4). Measured on a 4M-point
 // torus at start level 2 and on a 16M-point mixture with the FP64 insertion: k=40 8.3 ms with a buffer of 4
 // against 8.7 with 8, 9.0 with 2 and 13.9 with 16 (register pressure); fetching 2 or 8 per step runs slower
-// (9.0, 14.0 ms); k=16 4.4 ms fetching 4 against 4.7 fetching 2.
+// (9.0, 14.0 ms); k=16 4.4 ms fetching 4 against 4.7 fetching 2. Deferred flushes (a buffer of 5, 6 or 8
+// flushed once a lane holds more than 1, 2 or 4 keys; or fetching 2) insert as many keys and run slower on the
+// mixture's learned path too: 50.0-53.6 ms against 49.8.
 template <int K>
 constexpr int knn_buffer() { return 4; }
 template <int K>
