@@ -54,6 +54,25 @@ def test_knn_rows_bit_exact(egt_mod, oracle_mod, case, lanes):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("k", [10, 18])
+def test_lattice_rows_follow_bucket_count(egt_mod, oracle_mod, k):
+    """On a 12^3 lattice the rows at bucketCount 8 differ from those at 1024 (tie order, tied neighbours listed,
+    valid rows: all follow B); the GPU matches the oracle at each B, in both launch shapes."""
+    E, O = egt_mod, oracle_mod
+    pts = D.tiny_lattice(12)
+    got = {}
+    for B in (1024, 8):
+        og = O.Grid(*pts, B=B)
+        want = og.to_original(og.knn(k)[0])
+        for lanes in (1, 64):
+            _log("lattice k", k, "B", B, "lanes", lanes)
+            g = _cloud(E, pts, k, bucket=B, lanes=lanes)
+            got[B] = g.GetKNNIndexes()
+            g.close()
+            np.testing.assert_array_equal(got[B], want)
+    assert (got[1024] != got[8]).any()
+
+
 def test_autotuned_start_level_keeps_rows(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     pts = D.torus(60_000, seed=23)

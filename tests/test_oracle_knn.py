@@ -143,3 +143,33 @@ def test_knn_and_egg_subsets_equal_full_rows(oracle_mod):
     np.testing.assert_array_equal(sinv, inv[sel])
     egg_full = O.egg(g.xs, g.ys, g.zs, full, 0.65)
     np.testing.assert_array_equal(O.egg_subset(g.xs, g.ys, g.zs, sel, sub, 0.65), egg_full[sel])
+
+
+@pytest.mark.parametrize("k", [10, 18])
+def test_rows_depend_on_bucket_count(oracle_mod, k):
+    """Rows are a function of bucketCount B, not only of the points: ties are ordered by sorted index, and the
+    sorted order is the row-major order of the B^3 grid (GridKNNCuda.cu:1042-1128); the validity replay follows
+    the reference's stages, whose count and boxes come from B (:694-741). On a 12^3 lattice (every distance
+    tied several times) at B = 1024 and B = 8: the rows valid under both list the same distances, but tied
+    neighbours come in another order, and at k = 10 (the cut falls inside the 12 tied edge neighbours) other
+    tied neighbours are listed; the valid rows themselves differ too."""
+    pts = D.tiny_lattice(12)
+    x, y, z = (a.astype(np.float64) for a in pts)
+    rows, inv, perm = {}, {}, {}
+    for B in (1024, 8):
+        g = oracle_mod.Grid(*pts, B=B)
+        knn, _, iv, _ = g.knn(k)
+        rows[B], inv[B], perm[B] = g.to_original(knn), iv, g.perm.copy()
+    assert (perm[1024] != perm[8]).any()
+    assert (inv[1024] != inv[8]).any()
+    both = (inv[1024] == 0) & (inv[8] == 0)
+    a, b = rows[1024][both], rows[8][both]
+    i = np.repeat(np.nonzero(both)[0], k).reshape(-1, k)
+
+    def d2(r):
+        return (x[r] - x[i]) ** 2 + (y[r] - y[i]) ** 2 + (z[r] - z[i]) ** 2
+
+    np.testing.assert_array_equal(np.sort(d2(a), axis=1), np.sort(d2(b), axis=1))
+    assert (a != b).any(axis=1).sum() > 0.2 * both.sum()
+    differ_sets = sum(set(u) != set(v) for u, v in zip(a, b))
+    assert (differ_sets > 0) == (k == 10)
