@@ -202,6 +202,30 @@ def test_knn_invalidation_rules(egt_mod, oracle_mod, lanes):
     assert (g.GetKNNIndexes() == -1).all()
 
 
+def test_taubin_graph_replays_match_eager(egt_mod):
+    """The captured Taubin graphs against eager launches: three smoothing calls of 60 iterations on one handle
+    (two replays of the 25-iteration graph, then the 10-iteration one), with an eager kNN between the calls;
+    every call's output is bit-identical with graphs on and off."""
+    E = egt_mod
+    pts = D.sphere(30_000, seed=45)
+    outs = {}
+    for graph in (0, 1):
+        g = _cloud(E, pts, 16)
+        g.set_option(E.OPT_TAUBIN_GRAPH, graph)
+        res = []
+        for call in range(3):
+            _log("graph", graph, "call", call)
+            E.EGT_CUDA(g).PerformSmoothing(60, 0.5, -0.53, 0.65)
+            res.append(E.getOutputPoints(g))
+            g.GRIDCUDAKNNSELF_COMPACT(16, 1e8)
+        g.close()
+        outs[graph] = res
+    for call, (a, b) in enumerate(zip(outs[0], outs[1])):
+        for u, v in zip(a, b):
+            np.testing.assert_array_equal(u, v, err_msg=f"call {call}")
+    assert not np.array_equal(outs[1][0][0], outs[1][2][0])  # the calls kept smoothing
+
+
 def test_egg_and_taubin_bit_exact(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     pts = D.sphere(20_000, seed=41)
