@@ -703,8 +703,13 @@ __device__ inline int wave_cell_test(const KnnParams& P, int lv, const int (&c)[
 // mark per cell in LDS, then a running maximum over the lanes; 72 VGPRs, 7 waves per SIMD) on a 16M-point
 // cluster mixture: 67.7 against 75.8 ms from the own cell, 59.3 against 56.7 ms from learned starts, the case
 // this shape is picked for; a hybrid (the loop up to 4 cells) took 60.1 against 58.5 ms there.
+// Occupancy: the kernel is VALU-bound, and its wave-uniform values outgrow the scalar registers. Held to 8 waves
+// per SIMD it spilled 166 of them into vector lanes (778 v_readlane in the code); at 6 (75 VGPRs) 98 spill
+// (387 v_readlane), and the search runs faster with the same rows: the mixture at k=64 from learned starts
+// 68.7 -> 62.7 ms, from the own cell 97.2 -> 89.6; k=40 from the own cell 82.1 -> 75.8 (learned starts 55.9 ->
+// 56.3); a 4M-point torus at level 2 11.2 -> 10.2 (k=40) and 13.4 -> 12.0 ms (k=64). 7 waves: in between.
 template <bool LIST>
-__global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(8))) k_knn_wave(KnnParams P) {
+__global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(6))) k_knn_wave(KnnParams P) {
     const int lane = __lane_id();
     const int64_t v = blockIdx.x;
     if (v >= P.nchunks) return;
