@@ -662,6 +662,22 @@ __device__ inline double merge64(double L, double S) {
     x = bstage<64, 1>(x);
     return x;
 }
+// Running maximum over the wave of non-negative values (the same DPP steps as scan_add).
+__device__ inline int scan_max(int x) {
+    x = max(x, dpp<0x111, 0xf, true>(x));
+    x = max(x, dpp<0x112, 0xf, true>(x));
+    x = max(x, dpp<0x114, 0xf, true>(x));
+    x = max(x, dpp<0x118, 0xf, true>(x));
+    x = max(x, dpp<0x142, 0xa, false>(x));
+    x = max(x, dpp<0x143, 0xc, false>(x));
+    return x;
+}
+// Orders a wave's LDS accesses before and after it.
+__device__ inline void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 // Number of lanes whose (non-decreasing) value v is <= g, for g below the last lane's value: branch-free
 // binary search over the lanes.
 __device__ inline int count_le(int v, int g) {
@@ -699,10 +715,13 @@ __device__ inline int wave_cell_test(const KnnParams& P, int lv, const int (&c)[
 
 // LIST: each wave takes one query of a list (long queries a lane-per-query search handed on), from the
 // handle's start level.
-// Lanes are mapped to a batch's cells one cell at a time (readlanes). Measured against a constant-cost map (a
-// mark per cell in LDS, then a running maximum over the lanes; 72 VGPRs, 7 waves per SIMD) on a 16M-point
-// cluster mixture: 67.7 against 75.8 ms from the own cell, 59.3 against 56.7 ms from learned starts, the case
-// this shape is picked for; a hybrid (the loop up to 4 cells) took 60.1 against 58.5 ms there.
+// Lanes are mapped to a batch's entries at a constant cost: each entry with points left marks, in LDS, the slot
+// its run starts at, and a running maximum over the lanes carries it to the slots after it. Against mapping
+// the entries one at a time (four readlanes each), at 6 waves per SIMD on a 16M-point cluster mixture: k=64
+// from learned starts 59.1 against 62.8 ms, from the own cell 81.7 against 89.4; k=40 from the own cell 69.5
+// against 75.8, from learned starts 60.5 against 58.0 (there a lane per query is faster still); a 4M-point
+// torus at level 2 9.7 against 10.2 (k=40) and 11.0 against 12.0 ms (k=64). A hybrid (one at a time up to 4
+// entries) fell in between on every case.
 // Occupancy: the kernel is VALU-bound, and its wave-uniform values outgrow the scalar registers. Held to 8 waves
 // per SIMD it spilled 166 of them into vector lanes (778 v_readlane in the code); at 6 (75 VGPRs) 98 spill
 // (387 v_readlane), and the search runs faster with the same rows: the mixture at k=64 from learned starts
@@ -715,6 +734,7 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(6))) 
     if (v >= P.nchunks) return;
     // wave index made uniform for the compiler, so the query's values live in scalar registers
     const int wv_id = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    __shared__ int wmark[KBLK / 64][64];  // per wave: the entry whose run starts at each slot of a batch
     int64_t m0, m1;
     if (LIST) {
         m0 = v * (KBLK / 64) + wv_id;
@@ -845,17 +865,20 @@ __global__ void __launch_bounds__(KBLK) __attribute__((amdgpu_waves_per_eu(6))) 
                     if (tot == 0) break;
                     const int rexcl = rincl - (int)rem;
                     // lane l takes slot l of the ranges laid end to end: the entries of this batch, one at a time
+                    // lane l takes slot l of the entries' ranges laid end to end
                     int64_t p = -1;
                     int eo = 0;
-                    for (uint64_t em = __ballot(rem > 0 && rexcl < 64); em; em &= em - 1) {
-                        const int e = __builtin_ctzll(em);
-                        const int ex = wv::readlane(rexcl, e), ei = wv::readlane(rincl, e);
-                        const uint32_t ep = (uint32_t)wv::readlane((int)pos, e);
-                        const int eov_e = wv::readlane((int)eov, e);
-                        if (lane >= ex && lane < ei) {
-                            p = (int64_t)ep + (lane - ex);
-                            eo = eov_e;
-                        }
+                    {
+                        int* mk = wmark[wv_id];
+                        mk[lane] = 0;
+                        wv::lds_sync();
+                        if (rem > 0 && rexcl < 64) mk[rexcl] = lane + 1;
+                        wv::lds_sync();
+                        const int e = max(wv::scan_max(mk[lane]) - 1, 0);
+                        wv::lds_sync();
+                        const int eb = __shfl((int)pos - rexcl, e, 64);
+                        eo = __shfl((int)eov, e, 64);
+                        p = lane < tot ? (int64_t)eb + lane : -1;
                     }
                     const int take = min((int)rem, max(0, 64 - rexcl));
                     if (!__ballot(take > 0)) break;  // every batch consumes; guards the loop's end
