@@ -39,8 +39,10 @@ constexpr uint32_t SPLIT_COUNT = 96;
 // The same for the wave-per-query search, whose batches of 64 make scanning a populous cell cheaper.
 constexpr uint32_t WAVE_SPLIT_COUNT = 96;
 // Wave-per-query search: a batch with more survivors than this is sorted and merged into the list, fewer are
-// inserted one at a time. Measured on a 16M-point cluster mixture (learned starts 59.6 / 58.7 / 57.9 ms at 16,
-// 36 and never; from level 1, 65.7 / 67.1 / 74.3 ms) and a 4M-point torus (level 2: 10.0 / 10.3 / 12.3 ms).
+// inserted one at a time. Measured on a 16M-point cluster mixture at 16 / 8 / 32 / never: k=64 from learned
+// starts 59.0 / 60.0 / 60.0 / 75.4 ms, from the own cell 81.7 / 82.4 / 82.4 / 89.9; k=40 from the own cell
+// 69.7 / 70.1 / 70.1 / 72.8, from learned starts 60.4 / 60.0 / 59.9 / 56.6 (there a lane per query is faster
+// still); a 4M-point torus at level 2, k=64: 11.0 / 11.2 / 11.2 / 15.2 ms.
 constexpr int WAVE_SORT_MIN = 16;
 // Keys (distance bits << 32 | sorted index) of non-negative distances, read as IEEE doubles, are positive
 // and order exactly as the integers do, so the list is held as doubles and merged with FP64 min / max (full
