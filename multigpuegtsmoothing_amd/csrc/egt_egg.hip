@@ -220,6 +220,33 @@ __global__ void __launch_bounds__(256) k_csr_fill(const int32_t* __restrict__ mr
     const int k_l = cd.k;
     if (lane < nr) rowptr[m0 + lane] = (int32_t)o_l;
     if (m0 + nr == n && lane == 0) rowptr[n] = (int32_t)total;
+    // The rows of consecutive Morton positions lie back to back in mrow. When the wave's rows share one k (all
+    // but the chunks where a cloud ends), they are read as one block, 64 slots per load (a row per load left
+    // 64 - k lanes idle: 48 at k=16); slot idx of the block is slot idx - r k of row r = floor(idx / k), and
+    // the row's offset and keep mask come from lane r. On a 16M-point mixture, EGG + CSR 4.52 -> 3.90 ms at k=16
+    // and 13.25 -> 12.76 ms at k=40 (k=64: unchanged, a row already fills the wave).
+    const int k0 = __builtin_amdgcn_readfirstlane(k_l);
+    if (!__ballot(lane < nr && k_l != k0)) {
+        const int64_t rb0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rb_l, 0)) |
+                                      ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rb_l >> 32), 0)
+                                       << 32));
+        const int T = nr * k0;  // at most 16 x 64 slots
+        // (idx + 0.5) / k is at least 0.5 / k >= 1/128 away from an integer and below 1024, far more than the
+        // rounding of the product, so the truncation is the exact row
+        const float rk = 1.0f / (float)k0;
+        for (int base = 0; base < T; base += 64) {
+            const int idx = base + lane;
+            const bool in = idx < T;
+            const int r = min((int)(((float)idx + 0.5f) * rk), nr - 1);
+            const int slot = idx - r * k0;
+            const int32_t v = mrow[rb0 + (in ? idx : 0)];
+            const uint32_t o = (uint32_t)__shfl((int)o_l, r, 64);
+            const uint64_t keep = ((uint64_t)(uint32_t)__shfl((int)(uint32_t)keep_l, r, 64)) |
+                                  ((uint64_t)(uint32_t)__shfl((int)(keep_l >> 32), r, 64) << 32);
+            if (in && ((keep >> slot) & 1ull)) col[o + (uint32_t)__popcll(keep & ((1ull << slot) - 1ull))] = v;
+        }
+        return;
+    }
     const uint64_t lt = (1ull << lane) - 1ull;  // lanes below this one
 #pragma unroll
     for (int i = 0; i < CSR_ROWS; ++i) {
