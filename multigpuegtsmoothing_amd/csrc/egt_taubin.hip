@@ -87,6 +87,40 @@ __device__ inline void block_bitonic_sort(uint32_t* a, int n2) {
     }
 }
 
+// The same sort for n2 <= PBLK keys, one per thread, held in a register: stages whose partner lies within the
+// wave (j < 64) exchange through lane shuffles without a barrier, the others through LDS with two barriers.
+// For 1024 keys that is 10 LDS stages (20 barriers) instead of 55 barrier stages. Keys of threads at or past n2
+// never meet the others (n2 is a power of two, so i ^ j < n2 for i, j < n2). Falls back to block_bitonic_sort
+// for longer arrays. Taubin set-up (tile preparation and graphs): 16M-point mixture 3.47 -> 3.30 ms, 4M-point
+// torus 0.86 -> 0.79 ms, same tiles.
+__device__ inline void block_sort(uint32_t* a, int n2) {
+    if (n2 > PBLK) {
+        block_bitonic_sort(a, n2);
+        return;
+    }
+    const int i = threadIdx.x;
+    uint32_t x = i < n2 ? a[i] : 0xFFFFFFFFu;
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint32_t p;
+            if (j >= 64) {
+                __syncthreads();  // the previous LDS stage's reads are done
+                if (i < n2) a[i] = x;
+                __syncthreads();
+                p = i < n2 ? a[i ^ j] : x;
+            } else {
+                p = (uint32_t)__shfl_xor((int)x, j, 64);
+            }
+            const bool up = (i & k) == 0, low = (i & j) == 0;
+            const uint32_t lo = x < p ? x : p, hi = x < p ? p : x;  // unsigned compare
+            x = (low == up) ? lo : hi;
+        }
+    }
+    __syncthreads();
+    if (i < n2) a[i] = x;
+    __syncthreads();
+}
+
 // LDS hash set of a tile's external columns: open addressing, linear probing, 2 * halo_max slots. An insert
 // is only started while the count is at most halo_max, so the set never holds more than halo_max + PBLK
 // entries and every probe sequence reaches an empty slot.
@@ -169,7 +203,7 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     __syncthreads();
     for (int i = (int)H + threadIdx.x; i < n2; i += PBLK) uniq[i] = HS_EMPTY;
     __syncthreads();
-    block_bitonic_sort(uniq, n2);
+    block_sort(uniq, n2);
     for (uint32_t h = threadIdx.x; h < H; h += PBLK) halo[t * halo_max<TILE>() + h] = (int32_t)uniq[h];
     for (int32_t e = eb + threadIdx.x; e < ee; e += PBLK) {
         const uint32_t c = (uint32_t)col[e];
@@ -197,7 +231,7 @@ __global__ void __launch_bounds__(PBLK) k_tile_prep(const int32_t* __restrict__ 
     }
     for (int s = threadIdx.x; s < TILE / 64; s += PBLK) swid[s] = 0;
     __syncthreads();
-    block_bitonic_sort(hs, TILE);
+    block_sort(hs, TILE);
     for (int j = threadIdx.x; j < np; j += PBLK) {
         const uint32_t li = hs[j] & 0xFFFu;
         perm[t * TILE + j] = (uint16_t)li;
