@@ -67,6 +67,27 @@ def test_batch_matches_separate_oracle_runs(egt_mod, oracle_mod, persistent, lan
     b.close()
 
 
+def test_batch_same_k_neighbouring_clouds(egt_mod, oracle_mod):
+    """Neighbouring clouds with the same k and sizes that are not multiples of 16: the CSR fill reads the 16 rows
+    of a chunk that crosses a cloud boundary as one block (the rows of consecutive Morton positions lie back to
+    back across clouds). EGG rows and smoothed coordinates equal the oracle's, cloud by cloud."""
+    E, O = egt_mod, oracle_mod
+    clouds = [D.sphere(1003, seed=950), D.torus(2011, seed=951), D.superquadric(517, seed=952), D.sphere(77, seed=953)]
+    ks = np.array([40, 40, 40, 16], np.int32)
+    lam, mu, alpha, its = 0.5, -0.53, 0.65, 8
+    b = E.BatchGrid(clouds)
+    b.knn(ks)
+    E.EGT_CUDA(b).PerformSmoothing(its, lam, mu, alpha)
+    out = np.stack(E.getOutputPoints(b), 1)
+    for c, (pts, o) in enumerate(zip(clouds, b.split(out))):
+        _log("cloud", c, "n", pts[0].size, "k", int(ks[c]))
+        ref = O.pipeline(*pts, int(ks[c]), its, lam, mu, alpha, normals=False)
+        og = ref["grid"]
+        np.testing.assert_array_equal(b.GetKNNIndexes(c), og.to_original(ref["egg"]), err_msg=f"egg {c}")
+        np.testing.assert_array_equal(o, np.stack(ref["output"], 1), err_msg=f"coords {c}")
+    b.close()
+
+
 def test_batch_class_shape_tuner_keeps_rows(egt_mod):
     """The per-class shape tuner times each neighbour-count class in each launch shape and keeps the fastest;
     rows after it, and on a new handle given its choices through OPT_KNN_CLASS_SHAPES, equal the default
