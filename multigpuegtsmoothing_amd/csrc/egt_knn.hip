@@ -36,8 +36,19 @@ constexpr int KBLK = 256;
 // 16M-point cluster mixture from the own cell (76.2 ms at 48, 77.1 at 96; 78.6 with 96 above the fine level
 // and 48 below).
 constexpr uint32_t SPLIT_COUNT = 96;
-// The same for the wave-per-query search, whose batches of 64 make scanning a populous cell cheaper.
-constexpr uint32_t WAVE_SPLIT_COUNT = 96;
+// The same for the wave-per-query search. Its batches of 64 and constant-cost lane map scan a populous cell
+// as cheaply as its children, so it splits only much larger cells. On a 16M-point cluster mixture:
+//   threshold         96    192    384   1024   4096  never
+//   k=40, own cell   69.9   66.9   62.5   59.8   60.5   65.9 ms
+//   k=64, own cell   81.7   79.7   75.8   74.8   75.8   84.9 ms
+//   k=64, learned    59.4   57.4   57.5   57.5   57.6   67.0 ms
+//   512-cloud batch, learned (the class tuner's best total over all classes):
+//                   127.8      -  127.4  130.5      -      - ms
+// (k=40 from learned starts and a 4M-point torus at level 2: within 1 %; same rows; the columns come from
+// four boxes, matched through their common runs). The sparse batch clouds start at coarse levels from
+// learned proposals, where splitting lets the radius bound prune whole children: 384 keeps the batch and
+// most of the gain from the own cell.
+constexpr uint32_t WAVE_SPLIT_COUNT = 384;
 // Wave-per-query search: a batch with more survivors than this is sorted and merged into the list, fewer are
 // inserted one at a time. Measured on a 16M-point cluster mixture at 16 / 8 / 32 / never: k=64 from learned
 // starts 59.0 / 60.0 / 60.0 / 75.4 ms, from the own cell 81.7 / 82.4 / 82.4 / 89.9; k=40 from the own cell
