@@ -34,7 +34,9 @@ constexpr int KBLK = 256;
 // Cells with more points than this are scanned child by child. Measured on a 4M-point torus at start level 2
 // (k=40: 14.0 ms at 48, 12.2 at 64, 11.9 at 96, 12.0 at 128; k=64: 31.7 at 48, 26.5 at 96) and on a
 // 16M-point cluster mixture from the own cell (76.2 ms at 48, 77.1 at 96; 78.6 with 96 above the fine level
-// and 48 below).
+// and 48 below). With the final kernels on the 16M mixture at k=40 (own cell / learned proposal, the tuner's
+// fastest mode): 60.3 / 53.8 ms at 64, 62.0 / 52.3 at 96, 63.9 / 55.0 at 192, 67.3 / 61.1 at 384; the
+// learned start, which the mixture runs with, settles it at 96 (torus level 2: 8.34, 8.24, 8.38, 8.49).
 constexpr uint32_t SPLIT_COUNT = 96;
 // The same for the wave-per-query search. Its batches of 64 and constant-cost lane map scan a populous cell
 // as cheaply as its children, so it splits only much larger cells. On a 16M-point cluster mixture:
