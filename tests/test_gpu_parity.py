@@ -366,6 +366,52 @@ def test_normals_parity(egt_mod, oracle_mod):
     np.testing.assert_array_equal(nz, rz)
 
 
+def _duplicated_sphere(n=6_000, seed=57):
+    """A sphere whose every 5th point appears twice and every 50th three times, shuffled: a point's twins are
+    its nearest neighbours at distance 0 (ties broken by sorted index), the EGG keeps them (half-distance
+    under 1e-6, EGT_CUDA.cu:136-170) and they give the orientation's forest many equal edge weights."""
+    x, y, z = D.sphere(n, seed=seed)
+    idx = np.concatenate([np.arange(n), np.arange(0, n, 5), np.arange(0, n, 50)])
+    idx = idx[np.random.default_rng(seed).permutation(idx.size)]
+    return x[idx].copy(), y[idx].copy(), z[idx].copy()
+
+
+@pytest.mark.parametrize("lanes", [1, 64])
+def test_duplicate_points_rows_bit_exact(egt_mod, oracle_mod, lanes):
+    E, O = egt_mod, oracle_mod
+    pts = _duplicated_sphere()
+    og = O.Grid(*pts)
+    for k in (16, 40):
+        g = _cloud(E, pts, k, lanes=lanes)
+        want = og.knn(k)[0]
+        _log("duplicates: n", pts[0].size, "k", k, "lanes", lanes)
+        np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(want))
+        g.close()
+
+
+def test_duplicate_points_pipeline_bit_exact(egt_mod, oracle_mod):
+    E, O = egt_mod, oracle_mod
+    pts = _duplicated_sphere()
+    k, iters, lam, mu, alpha = 40, 12, 0.5, -0.53, 0.65
+    g = _cloud(E, pts, k)
+    E.EGT_CUDA(g).PerformSmoothing(iters, lam, mu, alpha)
+    rows = g.GetKNNIndexes()
+    out = E.getOutputPoints(g)
+    nrm = E.Normal_CUDA(g).GetNormals()
+    st = g.stats()
+    ref = O.pipeline(*pts, k, iters, lam, mu, alpha, normals=True)
+    og = ref["grid"]
+    _log("duplicates pipeline: kept", st["egg_kept"], "msf", st["msf_edges"], st["msf_trees"], "oracle",
+         ref["orient_stats"][:4])
+    np.testing.assert_array_equal(rows, og.to_original(ref["egg"]))
+    for a, b in zip(out, ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    assert st["msf_edges"] == ref["orient_stats"][1] and st["msf_trees"] == ref["orient_stats"][2]
+    for a, b in zip(nrm, ref["normals"]):
+        np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def test_mesh_rows_and_smoothing(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     x, y, z, tri = D.icosphere(4, seed=61)
