@@ -202,6 +202,28 @@ def test_knn_invalidation_rules(egt_mod, oracle_mod, lanes):
     assert (g.GetKNNIndexes() == -1).all()
 
 
+@pytest.mark.parametrize("n", [1, 2, 41])
+def test_tiny_cloud_pipeline(egt_mod, oracle_mod, n):
+    """Clouds of 1, 2 and k + 1 points at k=40: the coarsest ring (bucket count 8) cannot certify their k-th
+    radius, so every row is blanked; the EGG, Taubin and normals then run on empty rows: coordinates are the
+    oracle's round trip and every normal is the fallback x axis."""
+    E, O = egt_mod, oracle_mod
+    pts = D.sphere(n, seed=300 + n)
+    g = _cloud(E, pts, 40)
+    assert (g.GetKNNIndexes() == -1).all()
+    E.EGT_CUDA(g).PerformSmoothing(5, 0.5, -0.53, 0.65)
+    out = E.getOutputPoints(g)
+    nrm = E.Normal_CUDA(g).GetNormals()
+    ref = O.pipeline(*pts, 40, 5, 0.5, -0.53, 0.65, normals=True)
+    _log("tiny cloud n", n, "egg kept", g.stats()["egg_kept"])
+    assert g.stats()["egg_kept"] == 0
+    for a, b in zip(out, ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(nrm, ref["normals"]):
+        np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def test_taubin_graph_replays_match_eager(egt_mod):
     """The captured Taubin graphs against eager launches: three smoothing calls of 60 iterations on one handle
     (two replays of the 25-iteration graph, then the 10-iteration one), with an eager kNN between the calls;
