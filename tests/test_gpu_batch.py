@@ -88,6 +88,38 @@ def test_batch_same_k_neighbouring_clouds(egt_mod, oracle_mod):
     b.close()
 
 
+def test_batch_with_one_point_clouds(egt_mod, oracle_mod):
+    """Clouds of 1 and 2 points (and a 700-point sphere whose rows the k=40 certification blanks) between clouds
+    that keep neighbours: kNN rows, EGG rows and smoothed coordinates equal the oracle's cloud by cloud, and
+    the blanked clouds' normals are the fallback x axis as in the oracle."""
+    E, O = egt_mod, oracle_mod
+    clouds = [D.sphere(1, seed=960), D.torus(3000, seed=961), D.sphere(2, seed=962), D.superquadric(1500, seed=963),
+              D.sphere(1, seed=964), D.sphere(700, seed=965)]
+    ks = np.array([40, 16, 8, 40, 16, 40], np.int32)
+    lam, mu, alpha, its = 0.5, -0.53, 0.65, 6
+    b = E.BatchGrid(clouds)
+    b.knn(ks)
+    refs = []
+    for c, pts in enumerate(clouds):
+        _log("cloud", c, "n", pts[0].size, "k", int(ks[c]))
+        ref = O.pipeline(*pts, int(ks[c]), its, lam, mu, alpha, normals=True)
+        refs.append(ref)
+        np.testing.assert_array_equal(b.GetKNNIndexes(c), ref["grid"].to_original(ref["knn"]), err_msg=f"cloud {c}")
+    E.EGT_CUDA(b).PerformSmoothing(its, lam, mu, alpha)
+    out = np.stack(E.getOutputPoints(b), 1)
+    nrm = np.stack(E.Normal_CUDA(b).GetNormals(), 1)
+    for c, (o, nn) in enumerate(zip(b.split(out), b.split(nrm))):
+        ref = refs[c]
+        np.testing.assert_array_equal(b.GetKNNIndexes(c), ref["grid"].to_original(ref["egg"]), err_msg=f"egg {c}")
+        np.testing.assert_array_equal(o, np.stack(ref["output"], 1), err_msg=f"coords {c}")
+        want = np.stack(ref["normals"], 1)
+        if (ref["egg"] < 0).all():
+            np.testing.assert_array_equal(nn, want, err_msg=f"normals {c}")
+        else:
+            assert np.mean(np.sum(nn * want, 1) > 0.99) >= 0.999, c
+    b.close()
+
+
 def test_batch_class_shape_tuner_keeps_rows(egt_mod):
     """The per-class shape tuner times each neighbour-count class in each launch shape and keeps the fastest;
     rows after it, and on a new handle given its choices through OPT_KNN_CLASS_SHAPES, equal the default
