@@ -434,6 +434,35 @@ def test_duplicate_points_pipeline_bit_exact(egt_mod, oracle_mod):
     g.close()
 
 
+def test_offset_scaled_cloud_bit_exact(egt_mod, oracle_mod):
+    """A torus scaled by 37.5 and moved to (1200, -3400, 560): the bounding cube spans every axis's extremes
+    (about 4,700 wide), so the cloud fills some 16 of 1024 cells per axis and the normalized coordinates and
+    the output round trip work at large magnitudes. kNN and EGG rows, coordinates and normals equal the
+    oracle's."""
+    E, O = egt_mod, oracle_mod
+    x, y, z = D.torus(20_000, seed=71)
+    s = np.float32(37.5)
+    pts = ((x * s + np.float32(1200.0)).astype(np.float32), (y * s - np.float32(3400.0)).astype(np.float32),
+           (z * s + np.float32(560.0)).astype(np.float32))
+    k, iters, lam, mu, alpha = 24, 10, 0.5, -0.53, 0.65
+    ref = O.pipeline(*pts, k, iters, lam, mu, alpha, normals=True)
+    og = ref["grid"]
+    for lanes in (1, 64):
+        g = _cloud(E, pts, k, lanes=lanes)
+        _log("offset cloud, lanes", lanes)
+        assert g.GetMinExtent() == np.float32(og.min_extent) and g.GetMaxExtent() == np.float32(og.max_extent)
+        np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(ref["knn"]))
+        g.close()
+    g = _cloud(E, pts, k)
+    E.EGT_CUDA(g).PerformSmoothing(iters, lam, mu, alpha)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(ref["egg"]))
+    for a, b in zip(E.getOutputPoints(g), ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(E.Normal_CUDA(g).GetNormals(), ref["normals"]):
+        np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def test_mesh_rows_and_smoothing(egt_mod, oracle_mod):
     E, O = egt_mod, oracle_mod
     x, y, z, tri = D.icosphere(4, seed=61)
