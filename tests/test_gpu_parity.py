@@ -487,6 +487,43 @@ def test_mesh_rows_and_smoothing(egt_mod, oracle_mod):
     assert st["msf_edges"] == ref["orient_stats"][1] and st["msf_trees"] == ref["orient_stats"][2]
 
 
+def _mesh_patch(nu=60, nv=40, seed=81):
+    """A wavy 60 x 40 vertex grid triangulated along one diagonal: corners of valence 2 and 3, boundary
+    vertices of 4, interior vertices of 6."""
+    rng = np.random.default_rng(seed)
+    u, v = np.meshgrid(np.linspace(0, 1.5, nu, dtype=np.float32), np.linspace(0, 1, nv, dtype=np.float32),
+                       indexing="ij")
+    z = 0.05 * np.sin(4 * u) * np.cos(3 * v) + rng.normal(0, 0.002, u.shape)
+    x, y, z = (a.ravel().astype(np.float32) for a in (u, v, z))
+    i = (np.arange(nu - 1)[:, None] * nv + np.arange(nv - 1)[None, :]).ravel()
+    tri = np.concatenate([np.stack([i, i + nv, i + nv + 1], 1), np.stack([i, i + nv + 1, i + 1], 1)])
+    return x, y, z, tri.astype(np.uint32)
+
+
+@pytest.mark.parametrize("k", [4, 40])
+def test_mesh_patch_with_boundary(egt_mod, oracle_mod, k):
+    """An open mesh: at k=4 the interior rows are cut to the first 4 of their 6 one-ring vertices in the
+    reference's order, at k=40 every valence is kept. Rows, smoothed coordinates and normals equal the
+    oracle's."""
+    E, O = egt_mod, oracle_mod
+    x, y, z, tri = _mesh_patch()
+    m = E.GridMeshCuda((x, y, z), tri)
+    m.GetNeighborsFromTriangles(k)
+    og = O.Grid(x, y, z)
+    want = og.mesh_adjacency(tri, k)
+    _log("mesh patch k", k, "valences", np.unique((want >= 0).sum(1)).tolist())
+    np.testing.assert_array_equal(m.GetKNNIndexes(), og.to_original(want))
+    E.EGT_CUDA(m).PerformSmoothing(20, 0.5, -0.53, 1e-8)
+    ref = O.pipeline(x, y, z, k, 20, 0.5, -0.53, 1e-8, normals=True, mesh_tri=tri)
+    for a, b in zip(E.getOutputPoints(m), ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(E.Normal_CUDA(m).GetNormals(), ref["normals"]):
+        np.testing.assert_array_equal(a, b)
+    st = m.stats()
+    assert st["msf_edges"] == ref["orient_stats"][1] and st["msf_trees"] == ref["orient_stats"][2]
+    m.close()
+
+
 @pytest.mark.parametrize("regularized", [False, True])
 @pytest.mark.parametrize("levels,ndeg", [(4, 300), (6, 3)])
 def test_mesh_degenerate_triangles_keep_self(egt_mod, oracle_mod, regularized, levels, ndeg):
