@@ -388,6 +388,28 @@ def test_normals_parity(egt_mod, oracle_mod):
     np.testing.assert_array_equal(nz, rz)
 
 
+@pytest.mark.parametrize("k", [1, 7, 9, 23, 31, 33, 47, 55, 63])
+def test_every_k_class_bit_exact(egt_mod, oracle_mod, k):
+    """One k in each list class (K = 8, 16, ..., 64, the first and last slot of a class among them): kNN rows
+    with a lane and with a wave per query, then EGG rows (all four EGG kernel widths) and smoothed coordinates
+    against the oracle."""
+    E, O = egt_mod, oracle_mod
+    pts = D.torus(8_000, seed=73)
+    ref = O.pipeline(*pts, k, 4, 0.5, -0.53, 0.65, normals=False)
+    og = ref["grid"]
+    for lanes in (1, 64):
+        g = _cloud(E, pts, k, lanes=lanes)
+        _log("k", k, "lanes", lanes)
+        np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(ref["knn"]))
+        g.close()
+    g = _cloud(E, pts, k)
+    E.EGT_CUDA(g).PerformSmoothing(4, 0.5, -0.53, 0.65)
+    np.testing.assert_array_equal(g.GetKNNIndexes(), og.to_original(ref["egg"]))
+    for a, b in zip(E.getOutputPoints(g), ref["output"]):
+        np.testing.assert_array_equal(a, b)
+    g.close()
+
+
 def _duplicated_sphere(n=6_000, seed=57):
     """A sphere whose every 5th point appears twice and every 50th three times, shuffled: a point's twins are
     its nearest neighbours at distance 0 (ties broken by sorted index), the EGG keeps them (half-distance
