@@ -40,7 +40,7 @@ def _check(E, O, ref, q, k, thres=1e8, maxc=None, bucket=1024, start_level=None)
     return got, st, inv, ost
 
 
-@pytest.mark.parametrize("case", ["sphere_k16", "torus_k40", "sphere_k64", "mixture_k8"])
+@pytest.mark.parametrize("case", ["sphere_k16", "torus_k40", "sphere_k64", "mixture_k8", "torus_k1", "sphere_k33"])
 def test_query_rows_bit_exact(egt_mod, oracle_mod, case):
     if case == "sphere_k16":
         ref, k, seed = D.sphere(20_000, seed=41), 16, 51
@@ -48,6 +48,10 @@ def test_query_rows_bit_exact(egt_mod, oracle_mod, case):
         ref, k, seed = D.torus(30_000, seed=42), 40, 52
     elif case == "sphere_k64":
         ref, k, seed = D.sphere(8_000, seed=43), 64, 53
+    elif case == "torus_k1":
+        ref, k, seed = D.torus(20_000, seed=47), 1, 57
+    elif case == "sphere_k33":
+        ref, k, seed = D.sphere(20_000, seed=48), 33, 58
     else:
         ref, k, seed = D.mixture(40_000, seed=44), 8, 54
     q = _queries_near(ref, 12_000, seed)
