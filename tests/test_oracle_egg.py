@@ -53,8 +53,24 @@ def _fp64_keep(P, q, row, j, alpha):
     return True, margin
 
 
-def test_matches_fp64_predicate(oracle_mod):
+def test_zero_distance_neighbour_kept(oracle_mod):
+    """Row of p: its segment midpoint m, then the far end f (m sits at the centre of the p-f ellipsoid, so f is
+    dropped), then a duplicate of p: half-distance 0 < 1e-6, kept whatever the earlier slots hold
+    (EGT_CUDA.cu:145)."""
+    P = np.array([[0.3, 0.3, 0.3], [0.3, 0.3, 0.3], [0.5, 0.3, 0.3], [0.4, 0.3, 0.3]], np.float32)
+    for alpha in (0.65, 1e-3, 2.0):
+        out = _egg_rows(oracle_mod, (P[:, 0], P[:, 1], P[:, 2]), [[3, 2, 1], [0, 3, 2], [3, 1, 0], [0, 2, 1]], alpha)
+        assert out[0].tolist() == [3, -1, 1]
+        # the duplicate p first (kept) and f dropped by m; m itself has p's twin on its ellipsoid, a rounding case
+        assert out[1, 0] == 0 and out[1, 2] == -1
+
+
+@pytest.mark.parametrize("dups", [False, True])
+def test_matches_fp64_predicate(oracle_mod, dups):
     pts = D.sphere(2_000, seed=21)
+    if dups:  # every 4th point twice: zero-distance slots, and twins of p as witnesses on the ellipsoid
+        idx = np.concatenate([np.arange(2_000), np.arange(0, 2_000, 4)])
+        pts = tuple(c[idx].copy() for c in pts)
     g = oracle_mod.Grid(*pts)
     knn, _, inv, _ = g.knn(20)
     alpha = 0.65
