@@ -29,7 +29,8 @@
  *                          points, small radii (short and empty rows), big-cell invalidation; the literal
  *                          reference schedule agrees row for row
  *   orc_egg             -- pinned: closed forms (collinear drop, unit-square alpha>1 rule, alpha->0 keeps,
- *                          EGG subset of kNN, monotone in alpha) with an FP64 predicate
+ *                          zero-distance neighbour kept, EGG subset of kNN, monotone in alpha) with an FP64
+ *                          predicate (also on a cloud with duplicated points)
  *   orc_taubin_f32/f64  -- pinned: lambda=mu=0 identity, symmetric-neighbour fixed point, planar stays planar,
  *                          FP32 vs FP64 bound, closed-form ring shrink; regularized weights (isRegularized,
  *                          EGT_CUDA.cu:263): ring closed form over both passes, two-neighbour closed form of
